@@ -1,0 +1,133 @@
+/*
+ * oracle.h -- CPU restatement of CFD-Lite's SIMPLE pressure-velocity loop.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker, never the product: only the
+ * tests, the smoke check and the CPU-baseline leg of the benchmark may load it.
+ * The HIP library under cfd-lite_amd/ does not link or call anything here.
+ *
+ * Every routine restates the reference (Fortran, real(8), serial) operation by
+ * operation, in the same evaluation order, so that results are bitwise equal
+ * to the reference built without FMA contraction.  File:line citations point
+ * into the reference tree (src/...).
+ *
+ * Conventions kept from the reference (src/modules/mod_util.f90:1428-1448):
+ *   - elements are numbered 1..ne (cells) then ne+1..ne+nbf (boundary halos);
+ *   - packed index = (elem << 5) | local_face, local_face 0 marks a halo;
+ *   - ef2nb is the CSR neighbour table, column 1 = packed neighbour,
+ *     column 2 = signed global face (+ on the owner side);
+ *   - C arrays are 0-based, the integers stored inside them are 1-based.
+ *
+ * Parity status of each function is stated in its comment: "pinned" means a
+ * test compares it with the reference compiled from its own sources
+ * (oracle/_ref) and/or with closed forms.
+ */
+#ifndef CFDL_ORACLE_H
+#define CFDL_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { OR_HEXA_8 = 17, OR_QUAD_4 = 7 };
+enum { OR_BC_WALL = 0, OR_BC_LID = 1, OR_BC_SYMMETRY = 2 };
+
+typedef struct {
+  int ne, nf, nbf;
+  int *ef2nb_idx; /* ne+1 */
+  int *ef2nb;     /* L = 2nf-nbf entries, halo entries packed (ne+k)<<5 */
+  double *ap, *b, *anb, *phic;
+} or_subd;
+
+typedef struct {
+  int c1, c2, ncs;
+  int *index1; /* owned */
+  int *index2; /* alias of intf(c2,c1).index1 */
+} or_intf;
+
+typedef struct {
+  int it;
+  double res_i, res_f, res_max;
+  char name[8];
+} or_logline;
+
+typedef struct or_case {
+  /* mesh input (CGNS-like element lists, sections sorted by element range) */
+  int nvx, nsec, nelem, ne, nbf, nf, L;
+  double *x, *y, *z;
+  int *e2vx;              /* 8 per element, 1-based vertex ids */
+  int *etype, *esec;      /* nsec, 2*nsec */
+  int *bckind;            /* nsec: BC kind of each boundary section (-1 for cells) */
+  /* connectivity */
+  int *ef2nb_idx, *ef2nb, *s2g, *bs;  /* bs has nbf entries (halo ne+1+k) */
+  /* geometry */
+  double *xc, *yc, *zc, *vol, *aip, *rip;
+  /* partition */
+  int nsubd;
+  int *gf2g, *g2gf, *g2gf_idx;
+  or_subd *sd;
+  or_intf *intf; /* nsubd*nsubd, (c,cnb) at c*nsubd+cnb */
+  /* physics */
+  double dt;
+  int nit;
+  double *rho, *mu;
+  /* equation state (mod_uvwp.f90:6-16) */
+  double *u, *v, *w, *p, *u0, *v0, *w0;
+  double *gu, *gv, *gw, *gp, *gpc;
+  double *mip, *mip0;
+  double *bu, *bv, *bw, *d, *dc;
+  /* shared work arrays (mod_physics.f90:58-63) */
+  double *ap, *anb, *b, *phic;
+  /* residual log */
+  int nlog, caplog;
+  or_logline *log;
+} or_case;
+
+/* ---- setup ---- */
+or_case *or_case_new(int nvx, const double *x, const double *y, const double *z,
+                     int nsec, const int *etype, const int *esec, const int *e2vx,
+                     const int *bckind);
+int or_connectivity(or_case *c);       /* returns 0 or error */
+void or_geometry(or_case *c);
+int or_partition(or_case *c, int nsubd);
+void or_construct_subdomains(or_case *c);
+void or_init_physics(or_case *c, double rho, double mu, double dt, int nit);
+void or_case_free(or_case *c);
+
+/* ---- helpers exported for tests ---- */
+void or_qsort_key_nrec(int *key, int *b, int n);
+double or_vec_weight(const double *r0, const double *r1, const double *r2);
+void or_matinv3(const double *A, double *B); /* column-major 3x3 */
+
+/* ---- hot path (plain-array routines mirror mod_solver.f90) ---- */
+void or_calc_grad(const double *phi, double *grad, const double *xc, const double *yc,
+                  const double *zc, const int *ef2nb_idx, const int *ef2nb,
+                  int ne, int nf, int nbf);
+void or_solve_gs(or_case *c, const char *name, double *phi, const double *ap,
+                 const double *anb, const double *b, const int *ef2nb_idx,
+                 const int *ef2nb, int ne, int nf, int nbf, int nit);
+void or_smoother_gs(const char *name, double *phi, const double *ap, const double *anb,
+                    const double *b, const int *ef2nb_idx, const int *ef2nb,
+                    int ne, int nf, int nbf, int nit);
+void or_calc_residual(const double *phi, const double *ap, const double *anb,
+                      const double *b, const int *ef2nb_idx, const int *ef2nb,
+                      int ne, int nf, int nbf, double *res, double *res_max);
+void or_solve(or_case *c, const char *name, double *phi);
+
+/* ---- hot path (equation routines mirror mod_uvwp.f90) ---- */
+void or_update_boundaries(or_case *c);
+void or_calc_coef_uvw(or_case *c);
+void or_calc_mip(or_case *c, int rhie_chow);
+void or_calc_coef_p(or_case *c);
+void or_adjust_pc(or_case *c, double *pc, double pref);
+void or_update_uvwp(or_case *c, const double *pc);
+void or_solve_uvwp(or_case *c);
+void or_update_time(or_case *c);
+void or_construct_uvwp(or_case *c); /* zero state, calc_mip(.false.), mip0=mip */
+
+/* run ntstep x ncoef outer iterations exactly as src/main.f90:50-63 */
+void or_run(or_case *c, int ntstep, int ncoef);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

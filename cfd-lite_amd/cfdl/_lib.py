@@ -1,0 +1,90 @@
+"""ctypes binding of libcfdl_hip.so (declarations in include/cfdl.h)."""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_PKG = os.path.dirname(_HERE)
+
+PI = C.POINTER(C.c_int)
+PD = C.POINTER(C.c_double)
+VP = C.c_void_p
+
+
+class CfdlError(RuntimeError):
+    """A libcfdl_hip call returned a non-zero status."""
+
+
+def library_path():
+    return os.environ.get("CFDL_LIBRARY", os.path.join(_PKG, "libcfdl_hip.so"))
+
+
+_lib = None
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "cfdl_last_error": (C.c_char_p, []),
+    "cfdl_version": (C.c_int, []),
+    "cfdl_mesh_box": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                C.c_double, C.c_uint64, C.POINTER(VP)]),
+    "cfdl_mesh_from_elements": (C.c_int, [C.c_int, PD, PD, PD, C.c_int, PI, PI, PI, C.POINTER(VP)]),
+    "cfdl_mesh_partition": (C.c_int, [VP, C.c_int]),
+    "cfdl_mesh_info": (C.c_int, [VP, PI, PI, PI, PI, PI, PI]),
+    "cfdl_mesh_get_i32": (C.c_int, [VP, C.c_char_p, PI, C.c_int64]),
+    "cfdl_mesh_get_f64": (C.c_int, [VP, C.c_char_p, PD, C.c_int64]),
+    "cfdl_mesh_free": (None, [VP]),
+    "cfdl_ctx_create": (C.c_int, [C.c_int, C.c_int, C.c_int, PI, PI, PI, PI, PD, PD, PD, PD, PD, PD,
+                                  C.c_int, PI, PI, C.c_int, PI, PI, PI, PD, PD, C.c_double, C.c_int,
+                                  C.POINTER(VP)]),
+    "cfdl_ctx_create_cavity": (C.c_int, [VP, C.c_double, C.c_double, C.c_double, C.c_int,
+                                         C.POINTER(VP)]),
+    "cfdl_ctx_destroy": (None, [VP]),
+    "cfdl_set_field": (C.c_int, [VP, C.c_int, PD, C.c_int64]),
+    "cfdl_get_field": (C.c_int, [VP, C.c_int, PD, C.c_int64]),
+    "cfdl_synchronize": (C.c_int, [VP]),
+    "cfdl_update_boundaries": (C.c_int, [VP]),
+    "cfdl_calc_coef_uvw": (C.c_int, [VP]),
+    "cfdl_solve_gs": (C.c_int, [VP, C.c_char_p, C.c_int, C.c_int]),
+    "cfdl_solve_gs_uvw": (C.c_int, [VP]),
+    "cfdl_calc_grad": (C.c_int, [VP, C.c_int, C.c_int]),
+    "cfdl_calc_mip": (C.c_int, [VP, C.c_int]),
+    "cfdl_calc_coef_p": (C.c_int, [VP]),
+    "cfdl_set_a_0": (C.c_int, [VP, C.c_int]),
+    "cfdl_solve": (C.c_int, [VP, C.c_char_p]),
+    "cfdl_adjust_pc": (C.c_int, [VP]),
+    "cfdl_update_uvwp": (C.c_int, [VP]),
+    "cfdl_solve_uvwp": (C.c_int, [VP]),
+    "cfdl_update_time": (C.c_int, [VP]),
+    "cfdl_run": (C.c_int, [VP, C.c_int, C.c_int]),
+    "cfdl_log_count": (C.c_int, [VP, PI]),
+    "cfdl_log_get": (C.c_int, [VP, C.c_int, C.c_char_p, PI, PD, PD, PD]),
+    "cfdl_log_clear": (C.c_int, [VP]),
+    "cfdl_stats": (C.c_int, [VP, PD, C.c_int]),
+}
+
+
+def exported_names():
+    return list(_SIGS)
+
+
+def load_library(path=None):
+    """Load libcfdl_hip.so once; raise if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = path or library_path()
+    if not os.path.exists(path):
+        raise CfdlError("libcfdl_hip.so not found at %s: build it first "
+                        "(python -c 'import __graft_entry__ as g; g.build()')" % path)
+    lib = C.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = _lib.cfdl_last_error().decode(errors="replace")
+        raise CfdlError("cfdl error %d: %s" % (rc, msg))

@@ -1,0 +1,96 @@
+"""Host-side setup: element lists -> connectivity, geometry, partition.
+
+Mirrors what the reference's pre-processing hands to the hot path
+(geometry_t, src/setup/mod_mg_lvl_uns.f90:46-58; cell_input,
+src/setup/cell_input.f90:10-165) using the C++ setup in libcfdl_hip.so.
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import PD, PI, VP, check, load_library
+
+HEXA_8, QUAD_4 = 17, 7
+
+_I32 = ("ef2nb_idx", "ef2nb", "s2g", "bs", "esec", "etype", "e2vx", "gf2g", "g2gf", "g2gf_idx")
+_F64 = ("xc", "yc", "zc", "vol", "aip", "rip", "x", "y", "z")
+
+
+class Mesh:
+    """A hexahedral mesh with the reference's connectivity and geometry."""
+
+    def __init__(self, handle, names=None):
+        self._lib = load_library()
+        self._h = handle
+        self.names = names
+
+    @classmethod
+    def box(cls, nx, ny=None, nz=None, lx=1.0, ly=1.0, lz=1.0, jitter=0.0, seed=20261202):
+        """Unit box of nx*ny*nz hexahedra (i fastest), QUAD_4 sections
+        bottom, south, east, north, west, top after the cells."""
+        lib = load_library()
+        ny = nx if ny is None else ny
+        nz = nx if nz is None else nz
+        h = VP()
+        check(lib.cfdl_mesh_box(nx, ny, nz, lx, ly, lz, jitter, seed, C.byref(h)))
+        return cls(h, ["cells", "bottom", "south", "east", "north", "west", "top"])
+
+    @classmethod
+    def from_elements(cls, x, y, z, etype, esec, e2vx, names=None):
+        lib = load_library()
+        x, y, z = (np.ascontiguousarray(a, np.float64) for a in (x, y, z))
+        et = np.ascontiguousarray(etype, np.int32)
+        es = np.ascontiguousarray(esec, np.int32)
+        ev = np.ascontiguousarray(e2vx, np.int32)
+        h = VP()
+        check(lib.cfdl_mesh_from_elements(len(x), x.ctypes.data_as(PD), y.ctypes.data_as(PD),
+                                          z.ctypes.data_as(PD), len(et), et.ctypes.data_as(PI),
+                                          es.ctypes.data_as(PI), ev.ctypes.data_as(PI), C.byref(h)))
+        return cls(h, names)
+
+    def partition(self, nsubd):
+        """Oct-tree bisection + subdomain-local order (cell_input.f90:131-151)."""
+        check(self._lib.cfdl_mesh_partition(self._h, nsubd))
+        return self
+
+    def info(self):
+        v = [C.c_int() for _ in range(6)]
+        check(self._lib.cfdl_mesh_info(self._h, *[C.byref(a) for a in v]))
+        return dict(zip(("ne", "nf", "nbf", "nsec", "nsubd", "nvx"), (a.value for a in v)))
+
+    def _size(self, name):
+        i = self.info()
+        ne, nf, nbf, nsec, nsubd, nvx = (i[k] for k in ("ne", "nf", "nbf", "nsec", "nsubd", "nvx"))
+        L = 2 * nf - nbf
+        return {"ef2nb_idx": ne + 1, "ef2nb": 2 * L, "s2g": nf, "bs": nbf, "esec": 2 * nsec,
+                "etype": nsec, "e2vx": 8 * (ne + nbf), "gf2g": ne if nsubd else 0,
+                "g2gf": ne if nsubd else 0, "g2gf_idx": nsubd + 1 if nsubd else 0,
+                "xc": ne + nbf, "yc": ne + nbf, "zc": ne + nbf, "vol": ne, "aip": 3 * nf,
+                "rip": 3 * nf, "x": nvx, "y": nvx, "z": nvx}[name]
+
+    def get(self, name):
+        n = self._size(name)
+        if name in _I32:
+            a = np.empty(n, np.int32)
+            check(self._lib.cfdl_mesh_get_i32(self._h, name.encode(), a.ctypes.data_as(PI), n))
+        elif name in _F64:
+            a = np.empty(n, np.float64)
+            check(self._lib.cfdl_mesh_get_f64(self._h, name.encode(), a.ctypes.data_as(PD), n))
+        else:
+            raise KeyError(name)
+        return a
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h is not None:
+            self._lib.cfdl_mesh_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,178 @@
+"""Device-resident uvwp equation: the hot path of the reference.
+
+Method names follow src/equations/mod_uvwp.f90 and src/modules/mod_solver.f90
+(solve_uvwp, calc_coef_uvw, solve_gs, calc_grad, calc_mip, calc_coef_p,
+solve, adjust_pc, update_uvwp) and src/modules/mod_physics.f90
+(update_boundaries, update_time).  Every call runs HIP kernels in
+libcfdl_hip.so; nothing here computes on the CPU.
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import PD, PI, VP, CfdlError, check, load_library
+
+BC_WALL, BC_LID, BC_SYMMETRY = 0, 1, 2
+
+FIELDS = {name: k for k, name in enumerate(
+    ["u", "v", "w", "p", "u0", "v0", "w0", "gu", "gv", "gw", "gp", "gpc", "mip", "mip0",
+     "bu", "bv", "bw", "d", "dc", "ap", "b", "anb", "phic"])}
+
+
+def _fid(name):
+    if isinstance(name, int):
+        return name
+    try:
+        return FIELDS[name]
+    except KeyError:
+        raise CfdlError("unknown field %r" % (name,))
+
+
+class UVWP:
+    """uvwp_t + the shared work arrays, resident on the GPU."""
+
+    def __init__(self, handle, sizes):
+        self._lib = load_library()
+        self._h = handle
+        self.sizes = sizes
+
+    @staticmethod
+    def _sizes(ne, nf, nbf):
+        n, L = ne + nbf, 2 * nf - nbf
+        s = {}
+        for k in ("u", "v", "w", "p", "u0", "v0", "w0", "phic"):
+            s[k] = n
+        for k in ("gu", "gv", "gw", "gp", "gpc"):
+            s[k] = 3 * n
+        for k in ("mip", "mip0"):
+            s[k] = nf
+        for k in ("bu", "bv", "bw", "d", "dc", "ap", "b"):
+            s[k] = ne
+        s["anb"] = L
+        return s
+
+    @classmethod
+    def cavity(cls, mesh, rho=1.0, mu=0.01, dt=0.01, nit=100):
+        """Lid-driven cavity binding ('top' -> lid, others -> wall,
+        mod_uvwp.f90:73-78) on a partitioned Mesh."""
+        lib = load_library()
+        h = VP()
+        check(lib.cfdl_ctx_create_cavity(mesh.handle, rho, mu, dt, nit, C.byref(h)))
+        i = mesh.info()
+        return cls(h, cls._sizes(i["ne"], i["nf"], i["nbf"]))
+
+    @classmethod
+    def from_arrays(cls, ne, nf, nbf, ef2nb_idx, ef2nb, s2g, bs, xc, yc, zc, vol, aip, rip,
+                    esec, bckind, nsubd, gf2g, g2gf, g2gf_idx, rho, mu, dt, nit=100):
+        """Context from the reference's own arrays (geometry_t, meshds_t,
+        mg_lvl_t%g2gf/gf2g, properties_t)."""
+        lib = load_library()
+        i32 = lambda a: np.ascontiguousarray(a, np.int32)  # noqa: E731
+        f64 = lambda a: np.ascontiguousarray(a, np.float64)  # noqa: E731
+        keep = [i32(ef2nb_idx), i32(ef2nb), i32(s2g), i32(bs), f64(xc), f64(yc), f64(zc),
+                f64(vol), f64(aip), f64(rip), i32(esec), i32(bckind),
+                i32(gf2g if gf2g is not None else np.zeros(1)),
+                i32(g2gf if g2gf is not None else np.zeros(1)),
+                i32(g2gf_idx if g2gf_idx is not None else np.zeros(2)), f64(rho), f64(mu)]
+        ptr = [a.ctypes.data_as(PI if a.dtype == np.int32 else PD) for a in keep]
+        h = VP()
+        check(lib.cfdl_ctx_create(ne, nf, nbf, ptr[0], ptr[1], ptr[2], ptr[3], ptr[4], ptr[5],
+                                  ptr[6], ptr[7], ptr[8], ptr[9], len(keep[11]), ptr[10], ptr[11],
+                                  nsubd, ptr[12], ptr[13], ptr[14], ptr[15], ptr[16], dt, nit,
+                                  C.byref(h)))
+        return cls(h, cls._sizes(ne, nf, nbf))
+
+    # ---- fields ----
+    def get(self, name):
+        fid = _fid(name)
+        key = [k for k, v in FIELDS.items() if v == fid][0]
+        a = np.empty(self.sizes[key], np.float64)
+        check(self._lib.cfdl_get_field(self._h, fid, a.ctypes.data_as(PD), a.size))
+        return a
+
+    def set(self, name, values):
+        fid = _fid(name)
+        key = [k for k, v in FIELDS.items() if v == fid][0]
+        a = np.ascontiguousarray(values, np.float64)
+        if a.size != self.sizes[key]:
+            raise CfdlError("field %s needs %d values, got %d" % (key, self.sizes[key], a.size))
+        check(self._lib.cfdl_set_field(self._h, fid, a.ctypes.data_as(PD), a.size))
+
+    # ---- routines (one per reference procedure) ----
+    def update_boundaries(self):
+        check(self._lib.cfdl_update_boundaries(self._h))
+
+    def calc_coef_uvw(self):
+        check(self._lib.cfdl_calc_coef_uvw(self._h))
+
+    def solve_gs(self, cname, phi, b):
+        check(self._lib.cfdl_solve_gs(self._h, cname.encode(), _fid(phi), _fid(b)))
+
+    def solve_gs_uvw(self):
+        check(self._lib.cfdl_solve_gs_uvw(self._h))
+
+    def calc_grad(self, phi, grad):
+        check(self._lib.cfdl_calc_grad(self._h, _fid(phi), _fid(grad)))
+
+    def calc_mip(self, rhie_chow):
+        check(self._lib.cfdl_calc_mip(self._h, 1 if rhie_chow else 0))
+
+    def calc_coef_p(self):
+        check(self._lib.cfdl_calc_coef_p(self._h))
+
+    def set_a_0(self, name):
+        check(self._lib.cfdl_set_a_0(self._h, _fid(name)))
+
+    def solve(self, cname="pc"):
+        check(self._lib.cfdl_solve(self._h, cname.encode()))
+
+    def adjust_pc(self):
+        check(self._lib.cfdl_adjust_pc(self._h))
+
+    def update_uvwp(self):
+        check(self._lib.cfdl_update_uvwp(self._h))
+
+    def solve_uvwp(self):
+        check(self._lib.cfdl_solve_uvwp(self._h))
+
+    def update_time(self):
+        check(self._lib.cfdl_update_time(self._h))
+
+    def run(self, ntstep, ncoef):
+        check(self._lib.cfdl_run(self._h, ntstep, ncoef))
+
+    def synchronize(self):
+        check(self._lib.cfdl_synchronize(self._h))
+
+    def log(self):
+        """[(name, it, res_i, res_f, res_max)] one per linear solve."""
+        n = C.c_int()
+        check(self._lib.cfdl_log_count(self._h, C.byref(n)))
+        out = []
+        name = C.create_string_buffer(8)
+        it = C.c_int()
+        ri, rf, rm = C.c_double(), C.c_double(), C.c_double()
+        for k in range(n.value):
+            check(self._lib.cfdl_log_get(self._h, k, name, C.byref(it), C.byref(ri), C.byref(rf),
+                                         C.byref(rm)))
+            out.append((name.value.decode(), it.value, ri.value, rf.value, rm.value))
+        return out
+
+    def clear_log(self):
+        check(self._lib.cfdl_log_clear(self._h))
+
+    def stats(self):
+        s = (C.c_double * 3)()
+        check(self._lib.cfdl_stats(self._h, s, 3))
+        return {"gs_launches": s[0], "gs_ms": s[1], "outer_iterations": s[2]}
+
+    def close(self):
+        if self._h is not None:
+            self._lib.cfdl_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,1445 @@
+// cfdl_device.hip -- device context, FP64 kernels and the C ABI of the hot
+// path (CFD-Lite solve_uvwp, src/equations/mod_uvwp.f90:95-134, and the
+// solvers of src/modules/mod_solver.f90) for MI355X / gfx950.
+//
+// Numerics: every kernel evaluates the reference's expressions in the
+// reference's order (Fortran a*b/c = (a*b)/c, a-b-c = (a-b)-c) and the file
+// is compiled with -ffp-contract=off, so each per-cell and per-face value is
+// bit-identical to the reference's.  Independent cell / face loops become one
+// thread per cell / face, reading neighbours through the CSR (gathers, no
+// atomics).  The Gauss-Seidel sweeps keep the reference's exact order: cells
+// are grouped by dependency level (a cell's level is one more than the
+// highest level among its already-updated neighbours) and each level is one
+// launch, so every cell sees exactly the neighbour values it sees in the
+// sequential sweep.  Norms are fixed-order tree reductions (reproducible;
+// equal to the sequential sums up to rounding in the last bits).
+//
+// Device layout (HBM, 0-based; cells 0..ne-1, boundary halos ne..ne+nbf-1):
+//   row[ne+1], nbr[L], fsg[L]   CSR of ef2nb: neighbour, signed 1-based face
+//   own[nf], fcell[nf]          owner-side CSR slot / owner cell of a face
+//   bcrow[ne+1], bcslot[nbf]    per cell, its boundary slots in halo order
+//   hcell[nbf], hslot[nbf], hkind[nbf]   per halo: owner cell, slot, BC kind
+//   fields: element arrays of ne+nbf, gradients 3 per element (x,y,z).
+#include <execinfo.h>
+#include <hip/hip_runtime.h>
+#include <signal.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstddef>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "cfdl_internal.h"
+
+namespace cfdl {
+
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+
+// CFDL_TRACE=1 prints set-up stages to stderr and a native backtrace on a
+// segmentation fault (diagnostics for host-side faults).
+static bool trace_on() {
+  static const bool on = [] {
+    const char *v = std::getenv("CFDL_TRACE");
+    return v && v[0] == '1';
+  }();
+  return on;
+}
+static void segv_handler(int sig) {
+  void *frames[64];
+  const int n = backtrace(frames, 64);
+  const char msg[] = "cfdl: fatal signal, native backtrace:\n";
+  (void)!write(2, msg, sizeof(msg) - 1);
+  backtrace_symbols_fd(frames, n, 2);
+  signal(sig, SIG_DFL);
+  raise(sig);
+}
+static void trace(const char *stage) {
+  if (!trace_on()) return;
+  static bool installed = false;
+  if (!installed) {
+    signal(SIGSEGV, segv_handler);
+    installed = true;
+  }
+  std::fprintf(stderr, "cfdl: %s\n", stage);
+  std::fflush(stderr);
+}
+
+#define HIPCHK(x)                                                                   \
+  do {                                                                              \
+    hipError_t e_ = (x);                                                            \
+    if (e_ != hipSuccess) {                                                         \
+      set_error(std::string(#x) + ": " + hipGetErrorString(e_));                    \
+      return CFDL_EHIP;                                                             \
+    }                                                                               \
+  } while (0)
+
+#define CALL(x)              \
+  do {                       \
+    int rc_ = (x);           \
+    if (rc_) return rc_;     \
+  } while (0)
+
+constexpr int kBlock = 256;
+constexpr int kMaxSys = 16;
+constexpr int kLogCap = 1 << 15;
+
+struct MeshDev {
+  int ne, nf, nbf, L;
+  const int *row, *nbr, *fsg, *own, *fcell, *bcrow, *bcslot, *hcell, *hslot, *hkind;
+  const double *xc, *yc, *zc, *vol, *aip, *rip, *rho, *mu;
+};
+
+struct FieldsDev {
+  double *u, *v, *w, *p, *u0, *v0, *w0;
+  double *gu, *gv, *gw, *gp, *gpc;
+  double *mip, *mip0;
+  double *bu, *bv, *bw, *d, *dc;
+  double *ap, *b, *anb, *phic;
+};
+
+// state of a solve_gs call (up to 3 right-hand sides sharing ap/anb)
+struct GsCtl {
+  int it[3], active[3];
+  double res_i[3], res_f[3], res_max[3], target[3];
+};
+
+// state of multi_subdomain_solver, src/modules/mod_solver.f90:124-189
+struct PcCtl {
+  int it, active;
+  double ri, rf, rmt, target;
+};
+
+struct LogLine {
+  char name[8];
+  int it, pad;
+  double res_i, res_f, res_max;
+};
+
+// A linear system for the residual / sweep kernels: ny right-hand sides or
+// subdomains, each a contiguous cell range sharing row/nbr/ap/anb.
+struct SysDesc {
+  int ny;
+  int beg[kMaxSys], end[kMaxSys];
+  double *phi[kMaxSys];
+  const double *b[kMaxSys];
+  const int *row, *nbr;
+  const double *ap, *anb;
+};
+
+__device__ inline double dot3(const double *a, const double *b) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+__device__ inline double area_of(const double *a) {
+  return sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+}
+
+// vec_weight, src/modules/mod_util.f90:729-743 (r0 = face centre)
+__device__ inline double vec_weight(const double *r0, const double *r1, const double *r2) {
+  double ra[3], rb[3];
+  for (int i = 0; i < 3; i++) {
+    ra[i] = r1[i] - r0[i];
+    rb[i] = r2[i] - r0[i];
+  }
+  const double la = sqrt(ra[0] * ra[0] + ra[1] * ra[1] + ra[2] * ra[2]);
+  const double lb = sqrt(rb[0] * rb[0] + rb[1] * rb[1] + rb[2] * rb[2]);
+  if (la + lb > 0.0) return la / (la + lb);
+  return 0.5;
+}
+
+__device__ inline void centre(const MeshDev &m, int e, double *r) {
+  r[0] = m.xc[e];
+  r[1] = m.yc[e];
+  r[2] = m.zc[e];
+}
+
+// ---------------------------------------------------------------------------
+// BC setters lid / dirichlet0 / symmetry (mod_uvwp.f90:493-570), per halo.
+// ---------------------------------------------------------------------------
+__global__ void k_bc_update(MeshDev m, FieldsDev f) {
+  const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= m.nbf) return;
+  const int e = m.hcell[h], H = m.ne + h, kind = m.hkind[h];
+  const int fg = abs(m.fsg[m.hslot[h]]) - 1;
+  if (kind == CFDL_BC_SYMMETRY) {
+    const double *a = m.aip + 3 * fg;
+    const double area = area_of(a);
+    const double nrm[3] = {a[0] / area, a[1] / area, a[2] / area};
+    const double vel[3] = {f.u[e], f.v[e], f.w[e]};
+    const double t = dot3(vel, nrm);
+    double vn[3], vt[3];
+    for (int i = 0; i < 3; i++) {
+      vn[i] = t * nrm[i];
+      vt[i] = vel[i] - vn[i];
+    }
+    f.u[H] = vt[0] - 2.0 * vn[0];
+    f.v[H] = vt[1] - 2.0 * vn[1];
+    f.w[H] = vt[2] - 2.0 * vn[2];
+  } else {
+    f.u[H] = kind == CFDL_BC_LID ? 1.0 : 0.0;
+    f.v[H] = 0.0;
+    f.w[H] = 0.0;
+  }
+  f.p[H] = f.p[e];
+  f.mip[fg] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// calc_coef_uvw, mod_uvwp.f90:161-286: one thread per cell; the boundary
+// loop (:243-273) runs for the cell's own boundary faces in halo order, then
+// the Rhie-Chow / SIMPLEC coefficients (:276-284).
+// ---------------------------------------------------------------------------
+__global__ void k_coef_uvw(MeshDev m, FieldsDev f, double dt) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m.ne) return;
+  double rp[3];
+  centre(m, e, rp);
+  double ap = 0.0, sumf = 0.0, sumss[3] = {0.0, 0.0, 0.0}, sumdefc[3] = {0.0, 0.0, 0.0};
+  const int r0 = m.row[e], r1 = m.row[e + 1];
+  for (int k = r0; k < r1; k++) {
+    const int nb = m.nbr[k];
+    double fnb = 0.0, d = 0.0;
+    if (nb < m.ne) {
+      const int fs = m.fsg[k];
+      const double sg = fs >= 0 ? 1.0 : -1.0;
+      const int fg = abs(fs) - 1;
+      const double *a = m.aip + 3 * fg, *rip = m.rip + 3 * fg;
+      const double area = area_of(a);
+      double nrm[3], rpnb[3], dr[3], drip[3], rpp[3], rnp[3], drp[3];
+      for (int i = 0; i < 3; i++) nrm[i] = sg * a[i] / area;
+      centre(m, nb, rpnb);
+      for (int i = 0; i < 3; i++) dr[i] = rpnb[i] - rp[i];
+      const double ds = sqrt(dot3(dr, dr));
+      const double wt = vec_weight(rip, rp, rpnb);
+      for (int i = 0; i < 3; i++) drip[i] = rip[i] - rp[i];
+      double t = dot3(drip, nrm);
+      for (int i = 0; i < 3; i++) rpp[i] = rip[i] - t * nrm[i];
+      for (int i = 0; i < 3; i++) drip[i] = rip[i] - rpnb[i];
+      t = dot3(drip, nrm);
+      for (int i = 0; i < 3; i++) rnp[i] = rip[i] - t * nrm[i];
+      for (int i = 0; i < 3; i++) drp[i] = rnp[i] - rpp[i];
+      const double dsp = sqrt(dot3(drp, drp));
+      const double fl = -(sg * f.mip[fg]);
+      fnb = fmax(fl, 0.0);
+      sumf = sumf + fl;
+      const double muip = (1.0 - wt) * m.mu[e] + wt * m.mu[nb];
+      d = muip * area / ds;
+      const double ge[3][3] = {{f.gu[3 * e], f.gu[3 * e + 1], f.gu[3 * e + 2]},
+                               {f.gv[3 * e], f.gv[3 * e + 1], f.gv[3 * e + 2]},
+                               {f.gw[3 * e], f.gw[3 * e + 1], f.gw[3 * e + 2]}};
+      const double gn[3][3] = {{f.gu[3 * nb], f.gu[3 * nb + 1], f.gu[3 * nb + 2]},
+                               {f.gv[3 * nb], f.gv[3 * nb + 1], f.gv[3 * nb + 2]},
+                               {f.gw[3 * nb], f.gw[3 * nb + 1], f.gw[3 * nb + 2]}};
+      // secondary stress, :213-218 (gip = m-th column of grad(u,v,w))
+      for (int mm = 0; mm < 3; mm++) {
+        double gip[3];
+        for (int q = 0; q < 3; q++) gip[q] = (1.0 - wt) * ge[q][mm] + wt * gn[q][mm];
+        sumss[mm] = sumss[mm] + muip * area * dot3(gip, dr) / ds;
+      }
+      // deferred non-orthogonal correction, :220-225
+      for (int q = 0; q < 3; q++) {
+        double gip[3];
+        for (int i = 0; i < 3; i++) gip[i] = (1.0 - wt) * ge[q][i] + wt * gn[q][i];
+        sumdefc[q] = sumdefc[q] + muip * area * (dot3(gip, drp) / dsp - dot3(gip, dr) / ds);
+      }
+    }
+    f.anb[k] = d + fnb;
+    ap = ap + d + fnb;
+  }
+  const double vol = m.vol[e];
+  const double ap0 = m.rho[e] * vol / dt;
+  ap = ap + ap0;
+  double bu = ap0 * f.u0[e] + sumf * f.u[e] - vol * f.gp[3 * e] + sumss[0] + sumdefc[0];
+  double bv = ap0 * f.v0[e] + sumf * f.v[e] - vol * f.gp[3 * e + 1] + sumss[1] + sumdefc[1];
+  double bw = ap0 * f.w0[e] + sumf * f.w[e] - vol * f.gp[3 * e + 2] + sumss[2] + sumdefc[2];
+  for (int q = m.bcrow[e]; q < m.bcrow[e + 1]; q++) {
+    const int k = m.bcslot[q], H = m.nbr[k];
+    const int fg = m.fsg[k] - 1;  // boundary faces are always owner side
+    const double *a = m.aip + 3 * fg;
+    const double area = area_of(a);
+    const double nrm[3] = {a[0] / area, a[1] / area, a[2] / area};
+    const double dr[3] = {m.xc[H] - rp[0], m.yc[H] - rp[1], m.zc[H] - rp[2]};
+    const double ds = sqrt(dot3(dr, dr));
+    const double fz = 0.0, d = m.mu[e] * area / ds;
+    if (m.hkind[H - m.ne] != CFDL_BC_SYMMETRY) {  // 'dirichlet', :255-265
+      const double ue = f.u[e], ve = f.v[e], we = f.w[e];
+      double vr[3] = {ue, ve, we};
+      const double t = dot3(vr, nrm);
+      for (int i = 0; i < 3; i++) vr[i] = vr[i] - t * nrm[i];
+      vr[0] = f.u[H] - vr[0];
+      vr[1] = f.v[H] - vr[1];
+      vr[2] = f.w[H] - vr[2];
+      bu = bu + d * vr[0] - d * ue;
+      bv = bv + d * vr[1] - d * ve;
+      bw = bw + d * vr[2] - d * we;
+    }
+    ap = ap + d + fz;
+    f.anb[k] = f.anb[k] + d + fz;
+  }
+  f.ap[e] = ap;
+  f.bu[e] = bu;
+  f.bv[e] = bv;
+  f.bw[e] = bw;
+  f.d[e] = vol / ap;
+  double dc = ap;
+  for (int k = r0; k < r1; k++) dc = dc - f.anb[k];
+  f.dc[e] = vol / dc;
+}
+
+// ---------------------------------------------------------------------------
+// calc_grad + matinv3, mod_solver.f90:8-81: weighted least squares over all
+// CSR neighbours (boundary halos included), one thread per cell.
+// ---------------------------------------------------------------------------
+__global__ void k_grad(MeshDev m, const double *__restrict__ phi, double *__restrict__ grad) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m.ne) return;
+  double a11 = 0.0, a12 = 0.0, a13 = 0.0, a22 = 0.0, a23 = 0.0, a33 = 0.0;
+  double g[3] = {0.0, 0.0, 0.0};
+  const double xe = m.xc[e], ye = m.yc[e], ze = m.zc[e], pe = phi[e];
+  for (int k = m.row[e]; k < m.row[e + 1]; k++) {
+    const int nb = m.nbr[k];
+    const double dr[3] = {m.xc[nb] - xe, m.yc[nb] - ye, m.zc[nb] - ze};
+    const double wt = 1.0 / (dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
+    const double dphi = phi[nb] - pe;
+    for (int i = 0; i < 3; i++) g[i] = g[i] + wt * dphi * dr[i];
+    a11 = a11 + wt * dr[0] * dr[0];
+    a12 = a12 + wt * dr[0] * dr[1];
+    a13 = a13 + wt * dr[0] * dr[2];
+    a22 = a22 + wt * dr[1] * dr[1];
+    a23 = a23 + wt * dr[1] * dr[2];
+    a33 = a33 + wt * dr[2] * dr[2];
+  }
+  const double a21 = a12, a31 = a13, a32 = a23;
+  double b11, b12, b13, b21, b22, b23, b31, b32, b33;
+  double det = a11 * a22 * a33 - a11 * a23 * a32 - a12 * a21 * a33 + a12 * a23 * a31 +
+               a13 * a21 * a32 - a13 * a22 * a31;
+  if (fabs(det) > 2.2250738585072014e-308) {
+    const double di = 1.0 / det;
+    b11 = di * (a22 * a33 - a23 * a32);
+    b21 = -(di * (a21 * a33 - a23 * a31));
+    b31 = di * (a21 * a32 - a22 * a31);
+    b12 = -(di * (a12 * a33 - a13 * a32));
+    b22 = di * (a11 * a33 - a13 * a31);
+    b32 = -(di * (a11 * a32 - a12 * a31));
+    b13 = di * (a12 * a23 - a13 * a22);
+    b23 = -(di * (a11 * a23 - a13 * a21));
+    b33 = di * (a11 * a22 - a12 * a21);
+  } else {
+    det = a11 + a22 + a33;
+    const double di = 1.0 / det;
+    b11 = di; b22 = di; b33 = di;
+    b12 = b13 = b21 = b23 = b31 = b32 = 0.0;
+  }
+  grad[3 * e] = b11 * g[0] + b12 * g[1] + b13 * g[2];
+  grad[3 * e + 1] = b21 * g[0] + b22 * g[1] + b23 * g[2];
+  grad[3 * e + 2] = b31 * g[0] + b32 * g[1] + b33 * g[2];
+}
+
+// ---------------------------------------------------------------------------
+// calc_mip, mod_uvwp.f90:438-490: one thread per face, owner side, interior
+// faces only (boundary mip is set by the BC setters).
+// ---------------------------------------------------------------------------
+__global__ void k_mip(MeshDev m, FieldsDev f, int rc, double dt) {
+  const int fg = blockIdx.x * blockDim.x + threadIdx.x;
+  if (fg >= m.nf) return;
+  const int k = m.own[fg], e = m.fcell[fg], nb = m.nbr[k];
+  if (nb >= m.ne) return;
+  double rp[3], rpnb[3];
+  centre(m, e, rp);
+  centre(m, nb, rpnb);
+  const double *a = m.aip + 3 * fg, *rip = m.rip + 3 * fg;
+  const double area = area_of(a);
+  const double nrm[3] = {a[0] / area, a[1] / area, a[2] / area};
+  const double wt = vec_weight(rip, rp, rpnb);
+  const double v1[3] = {f.u[e], f.v[e], f.w[e]}, v2[3] = {f.u[nb], f.v[nb], f.w[nb]};
+  double vip[3];
+  for (int i = 0; i < 3; i++) vip[i] = (1.0 - wt) * v1[i] + wt * v2[i];
+  const double rhoip = m.rho[e] * (1.0 - wt) + m.rho[nb] * wt;
+  double mip = dot3(vip, nrm) * rhoip * area;
+  if (rc) {
+    double dr[3], gpip[3], vip0[3];
+    for (int i = 0; i < 3; i++) dr[i] = rpnb[i] - rp[i];
+    for (int i = 0; i < 3; i++) gpip[i] = (1.0 - wt) * f.gp[3 * e + i] + wt * f.gp[3 * nb + i];
+    const double dip = (1.0 - wt) * f.d[e] + wt * f.d[nb];
+    const double w1[3] = {f.u0[e], f.v0[e], f.w0[e]}, w2[3] = {f.u0[nb], f.v0[nb], f.w0[nb]};
+    for (int i = 0; i < 3; i++) vip0[i] = (1.0 - wt) * w1[i] + wt * w2[i];
+    mip = mip - rhoip * area * dip / dot3(dr, nrm) * (f.p[nb] - f.p[e] - dot3(gpip, dr)) -
+          rhoip / dt * dip * (f.mip0[fg] - dot3(vip0, nrm) * rhoip * area);
+  }
+  f.mip[fg] = mip;
+}
+
+// ---------------------------------------------------------------------------
+// calc_coef_p, mod_uvwp.f90:289-368: Neumann pressure-correction matrix and
+// net inflow, one thread per cell; boundary loop (:344-366) in halo order.
+// ---------------------------------------------------------------------------
+__global__ void k_coef_p(MeshDev m, FieldsDev f) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m.ne) return;
+  double rp[3];
+  centre(m, e, rp);
+  double ap = 0.0, sumf = 0.0;
+  for (int k = m.row[e]; k < m.row[e + 1]; k++) {
+    const int nb = m.nbr[k];
+    double d = 0.0;
+    if (nb < m.ne) {
+      const int fs = m.fsg[k];
+      const double sg = fs >= 0 ? 1.0 : -1.0;
+      const int fg = abs(fs) - 1;
+      const double *a = m.aip + 3 * fg, *rip = m.rip + 3 * fg;
+      const double area = area_of(a);
+      double nrm[3], rpnb[3], dr[3];
+      for (int i = 0; i < 3; i++) nrm[i] = sg * a[i] / area;
+      centre(m, nb, rpnb);
+      for (int i = 0; i < 3; i++) dr[i] = rpnb[i] - rp[i];
+      const double wt = vec_weight(rip, rp, rpnb);
+      const double fl = -(sg * f.mip[fg]);
+      sumf = sumf + fl;
+      const double rhoip = (1.0 - wt) * m.rho[e] + wt * m.rho[nb];
+      d = ((1.0 - wt) * f.dc[e] + wt * f.dc[nb]) / dot3(dr, nrm) * rhoip * area;
+    }
+    f.anb[k] = d;
+    ap = ap + d;
+  }
+  double b = sumf;
+  for (int q = m.bcrow[e]; q < m.bcrow[e + 1]; q++) {
+    const int k = m.bcslot[q];
+    const double d = 0.0;  // 'dirichlet' and 'zero_flux' both add nothing
+    ap = ap + d;
+    f.anb[k] = f.anb[k] + d;
+    b = b - f.mip[m.fsg[k] - 1];
+  }
+  f.ap[e] = ap;
+  f.b[e] = b;
+}
+
+// update_uvwp cell part, mod_uvwp.f90:383-392 (no velocity correction)
+__global__ void k_update_cells(int ne, FieldsDev f) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  f.p[e] = f.p[e] + f.phic[e];
+  for (int i = 0; i < 3; i++) f.gp[3 * e + i] = f.gp[3 * e + i] + f.gpc[3 * e + i];
+}
+
+// update_uvwp face part, mod_uvwp.f90:394-415 (arithmetic-mean rho)
+__global__ void k_update_faces(MeshDev m, FieldsDev f) {
+  const int fg = blockIdx.x * blockDim.x + threadIdx.x;
+  if (fg >= m.nf) return;
+  const int k = m.own[fg], e = m.fcell[fg], nb = m.nbr[k];
+  if (nb >= m.ne) return;
+  const double *a = m.aip + 3 * fg, *rip = m.rip + 3 * fg;
+  const double area = area_of(a);
+  const double nrm[3] = {a[0] / area, a[1] / area, a[2] / area};
+  double rp[3], rpnb[3], dr[3];
+  centre(m, e, rp);
+  centre(m, nb, rpnb);
+  for (int i = 0; i < 3; i++) dr[i] = rpnb[i] - rp[i];
+  const double wt = vec_weight(rip, rp, rpnb);
+  const double dip = (1.0 - wt) * f.dc[e] + wt * f.dc[nb];
+  const double rhoip = (m.rho[e] + m.rho[nb]) / 2.0;
+  const double dmip = rhoip * area * dip * (f.phic[nb] - f.phic[e]) / dot3(dr, nrm);
+  f.mip[fg] = f.mip[fg] - dmip;
+}
+
+// adjust_pc, mod_uvwp.f90:136-158 with pref = phic(1) (:129)
+__global__ void k_pref(const double *phic, double *scal) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) scal[0] = phic[0];
+}
+__global__ void k_sub_pref(int ne, double *phic, const double *scal) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < ne) phic[e] = phic[e] - scal[0];
+}
+__global__ void k_halo_from_owner(MeshDev m, double *phi) {
+  const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h < m.nbf) phi[m.ne + h] = phi[m.hcell[h]];
+}
+
+__global__ void k_fill(double *p, int64_t n, double v) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_copy(double *dst, const double *src, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+// ---------------------------------------------------------------------------
+// Gauss-Seidel: one dependency level of a sweep, up to NC right-hand sides
+// sharing ap/anb (smoother_gs / solve_gs inner loop, mod_solver.f90:205-224,
+// 287-307).  phi(e) = (sumnb + (sor-1)*ap*phi(e))/ap/sor.
+// ---------------------------------------------------------------------------
+template <int NC>
+__global__ void k_gs_level(SysDesc s, const int *__restrict__ cells, int n, double sor,
+                           const int *__restrict__ active) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int e = cells[t];
+  const int r0 = s.row[e], r1 = s.row[e + 1];
+  const double a = s.ap[e];
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    if (!active[c]) continue;
+    double *phi = s.phi[c];
+    double sum = s.b[c][e];
+    for (int k = r0; k < r1; k++) sum = sum + s.anb[k] * phi[s.nbr[k]];
+    phi[e] = (sum + (sor - 1.0) * a * phi[e]) / a / sor;
+  }
+}
+
+// Residual partial sums: block (x, y) covers cells [beg_y + x*chunk, ...).
+// absmax: |r| and max|r| (solve_gs, :309-321); else signed r and max(r,0)
+// (calc_residual, :239-251).  Fixed-order tree, so reproducible.
+__global__ void k_resid_partial(SysDesc s, int chunk, int nblk, int absmax, double *part) {
+  __shared__ double ssum[kBlock], smax[kBlock];
+  const int y = blockIdx.y, x = blockIdx.x, tid = threadIdx.x;
+  const int lo = s.beg[y] + x * chunk;
+  const int hi = min(lo + chunk, s.end[y]);
+  const double *phi = s.phi[y], *b = s.b[y];
+  double sum = 0.0, mx = 0.0;
+  for (int e = lo + tid; e < hi; e += blockDim.x) {
+    double r = b[e];
+    for (int k = s.row[e]; k < s.row[e + 1]; k++) r = r + s.anb[k] * phi[s.nbr[k]];
+    r = r - s.ap[e] * phi[e];
+    if (absmax) r = fabs(r);
+    mx = fmax(r, mx);
+    sum = sum + r * r;
+  }
+  ssum[tid] = sum;
+  smax[tid] = mx;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if (tid < off) {
+      ssum[tid] = ssum[tid] + ssum[tid + off];
+      smax[tid] = fmax(smax[tid], smax[tid + off]);
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    part[2 * (y * nblk + x)] = ssum[0];
+    part[2 * (y * nblk + x) + 1] = smax[0];
+  }
+}
+
+__global__ void k_resid_final(const double *part, int nblk, double *red) {
+  __shared__ double ssum[kBlock], smax[kBlock];
+  const int y = blockIdx.x, tid = threadIdx.x;
+  double sum = 0.0, mx = 0.0;
+  for (int i = tid; i < nblk; i += blockDim.x) {
+    sum = sum + part[2 * (y * nblk + i)];
+    mx = fmax(mx, part[2 * (y * nblk + i) + 1]);
+  }
+  ssum[tid] = sum;
+  smax[tid] = mx;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if (tid < off) {
+      ssum[tid] = ssum[tid] + ssum[tid + off];
+      smax[tid] = fmax(smax[tid], smax[tid + off]);
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    red[2 * y] = ssum[0];
+    red[2 * y + 1] = smax[0];
+  }
+}
+
+// solve_gs bookkeeping, mod_solver.f90:271-287 and :309-322
+__global__ void k_gs_init(GsCtl *c, const double *red, int ny, double ne, int nit) {
+  if (threadIdx.x != 0) return;
+  for (int y = 0; y < ny; y++) {
+    const double ri = sqrt(red[2 * y] / ne);
+    c->res_i[y] = ri;
+    c->res_f[y] = ri;
+    c->target[y] = ri / 10.0;
+    c->res_max[y] = 0.0;
+    c->it[y] = 0;
+    c->active[y] = (0 < nit && ri > c->target[y]) ? 1 : 0;
+  }
+}
+__global__ void k_gs_iter_end(GsCtl *c, const double *red, int ny, double ne, int nit) {
+  if (threadIdx.x != 0) return;
+  for (int y = 0; y < ny; y++) {
+    if (!c->active[y]) continue;
+    c->it[y] = c->it[y] + 1;
+    c->res_max[y] = red[2 * y + 1];
+    c->res_f[y] = sqrt(red[2 * y] / ne);
+    c->active[y] = (c->it[y] < nit && c->res_f[y] > c->target[y]) ? 1 : 0;
+  }
+}
+struct Names {
+  char n[3][8];
+};
+__global__ void k_log_gs(const GsCtl *c, int ny, Names nm, LogLine *log, int *nlog) {
+  if (threadIdx.x != 0) return;
+  for (int y = 0; y < ny; y++) {
+    const int k = *nlog;
+    if (k >= kLogCap) return;
+    LogLine &l = log[k];
+    for (int i = 0; i < 8; i++) l.name[i] = nm.n[y][i];
+    l.it = c->it[y];
+    l.res_i = c->res_i[y];
+    l.res_f = c->res_f[y];
+    l.res_max = c->res_max[y];
+    *nlog = k + 1;
+  }
+}
+
+// multi_subdomain_solver bookkeeping, mod_solver.f90:140-181 (Q1: res_f_tot
+// accumulates onto its previous value; Q2: signed max)
+struct SubCounts {
+  int n;
+  double ne[kMaxSys];
+};
+__global__ void k_pc_init(PcCtl *c, const double *red, SubCounts sc, int nit) {
+  if (threadIdx.x != 0) return;
+  double ri = 0.0, rmt = 0.0;
+  for (int y = 0; y < sc.n; y++) {
+    const double res = sqrt(red[2 * y] / sc.ne[y]);
+    ri = ri + res * res;
+    rmt = fmax(rmt, red[2 * y + 1]);
+  }
+  ri = sqrt(ri / static_cast<double>(sc.n));
+  c->ri = ri;
+  c->rf = ri;
+  c->rmt = rmt;
+  c->target = ri / 10.0;
+  c->it = 0;
+  c->active = (0 < nit && c->rf > c->target) ? 1 : 0;
+}
+__global__ void k_pc_block_end(PcCtl *c, const double *red, SubCounts sc, int nit, int has_res) {
+  if (threadIdx.x != 0 || !c->active) return;
+  c->it = c->it + 2;
+  if (has_res) {
+    double rf = c->rf, rmt = c->rmt;
+    for (int y = 0; y < sc.n; y++) {
+      const double res = sqrt(red[2 * y] / sc.ne[y]);
+      rf = rf + res * res;
+      rmt = fmax(rmt, red[2 * y + 1]);
+    }
+    c->rf = sqrt(rf / static_cast<double>(sc.n));
+    c->rmt = rmt;
+  }
+  c->active = (c->it < nit && c->rf > c->target) ? 1 : 0;
+}
+__global__ void k_log_pc(const PcCtl *c, LogLine *log, int *nlog) {
+  if (threadIdx.x != 0) return;
+  const int k = *nlog;
+  if (k >= kLogCap) return;
+  LogLine &l = log[k];
+  for (int i = 0; i < 8; i++) l.name[i] = 0;
+  l.name[0] = 'p';
+  l.name[1] = 'c';
+  l.it = c->it;
+  l.res_i = c->ri;
+  l.res_f = c->rf;
+  l.res_max = c->rmt;
+  *nlog = k + 1;
+}
+
+// assemble_coef / update_halos / update_phi, mod_subdomains.f90:162-229
+__global__ void k_pc_assemble(int ne, const int *l2g_cell, const int *rowp, const int *rowg,
+                              const double *ap, const double *b, const double *anb,
+                              const double *phi, double *ap_p, double *b_p, double *anb_p,
+                              double *phic_p) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ne) return;
+  const int g = l2g_cell[c];
+  ap_p[c] = ap[g];
+  b_p[c] = b[g];
+  phic_p[c] = phi[g];
+  const int kg = rowg[g];
+  for (int k = rowp[c]; k < rowp[c + 1]; k++) anb_p[k] = anb[kg + (k - rowp[c])];
+}
+__global__ void k_gather(int n, const int *idx, const double *src, double *dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[idx[i]];
+}
+__global__ void k_exchange(int n, const int *dst, const int *src, double *phic,
+                           const int *active) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && active[0]) phic[dst[i]] = phic[src[i]];
+}
+__global__ void k_scatter(int n, const int *idx, const double *src, double *dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[idx[i]] = src[i];
+}
+
+}  // namespace cfdl
+
+using namespace cfdl;
+
+// ===========================================================================
+// Host side
+// ===========================================================================
+namespace {
+
+struct Sched {
+  std::vector<int> ptr;  // level starts (host)
+  int *cells = nullptr;  // device, cells sorted by level
+  int nlev = 0;
+};
+
+struct DevBuf {
+  std::vector<void *> ptrs;
+  ~DevBuf() {
+    for (void *p : ptrs) (void)hipFree(p);
+  }
+};
+
+}  // namespace
+
+struct cfdl_ctx {
+  int dev = 0;
+  hipStream_t st = nullptr;
+  int ne = 0, nf = 0, nbf = 0, L = 0, nsubd = 1, nit = 100;
+  double dt = 0.01;
+  DevBuf mem;
+  MeshDev md{};
+  FieldsDev fd{};
+  double *fields[CFDL_NFIELDS] = {};
+  int64_t fsize[CFDL_NFIELDS] = {};
+  Sched gf, gb;  // global system, forward / backward sweeps
+  // pressure-correction subdomain systems (concatenated)
+  int Lp = 0, nh = 0, nx = 0;
+  int *rowp = nullptr, *nbrp = nullptr, *l2g_cell = nullptr, *halo_src = nullptr;
+  int *xdst = nullptr, *xsrc = nullptr, *rowg = nullptr;
+  double *ap_p = nullptr, *b_p = nullptr, *anb_p = nullptr, *phic_p = nullptr;
+  std::vector<int> co;  // subdomain cell offsets (nsubd+1)
+  Sched pf, pb;
+  // control
+  GsCtl *gctl = nullptr;
+  PcCtl *pctl = nullptr;
+  double *part = nullptr, *red = nullptr, *scal = nullptr;
+  int nblk_g = 1, chunk_g = 1, nblk_p = 1, chunk_p = 1;
+  int *host_flags = nullptr;  // pinned
+  LogLine *dlog = nullptr;
+  int *dlog_n = nullptr;
+  std::vector<LogLine> hlog;
+  // stats
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double gs_ms = 0.0;
+  int64_t gs_launches = 0, outer = 0;
+  bool timing = true;
+};
+
+template <typename T>
+static int dalloc(cfdl_ctx *c, T **p, int64_t n) {
+  void *q = nullptr;
+  const size_t bytes = sizeof(T) * static_cast<size_t>(n > 0 ? n : 1);
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess) {
+    set_error(std::string("hipMalloc of ") + std::to_string(bytes) + " bytes: " + hipGetErrorString(e));
+    return CFDL_ENOMEM;
+  }
+  c->mem.ptrs.push_back(q);
+  *p = static_cast<T *>(q);
+  return CFDL_OK;
+}
+
+template <typename T>
+static int upload(cfdl_ctx *c, T **p, const std::vector<T> &v) {
+  CALL(dalloc(c, p, static_cast<int64_t>(v.size())));
+  if (!v.empty()) HIPCHK(hipMemcpy(*p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+  return CFDL_OK;
+}
+
+static inline int nblocks(int64_t n) { return static_cast<int>((n + kBlock - 1) / kBlock); }
+
+// Dependency levels of a sweep over cells [beg, end) in increasing (fwd) or
+// decreasing (bwd) order: a cell waits for neighbours in the same range that
+// the sweep visits before it.  Cells of one level are independent.
+static void levels(const std::vector<int> &row, const std::vector<int> &nbr, int beg, int end,
+                   bool fwd, std::vector<int> &lev) {
+  if (fwd) {
+    for (int e = beg; e < end; e++) {
+      int l = 0;
+      for (int k = row[e]; k < row[e + 1]; k++) {
+        const int n = nbr[k];
+        if (n >= beg && n < e) l = std::max(l, lev[n]);
+      }
+      lev[e] = l + 1;
+    }
+  } else {
+    for (int e = end - 1; e >= beg; e--) {
+      int l = 0;
+      for (int k = row[e]; k < row[e + 1]; k++) {
+        const int n = nbr[k];
+        if (n > e && n < end) l = std::max(l, lev[n]);
+      }
+      lev[e] = l + 1;
+    }
+  }
+}
+
+static int make_sched(cfdl_ctx *c, const std::vector<int> &lev, int n, Sched *s) {
+  int nlev = 0;
+  for (int e = 0; e < n; e++) nlev = std::max(nlev, lev[e]);
+  std::vector<int> cnt(nlev + 1, 0);
+  for (int e = 0; e < n; e++) cnt[lev[e]]++;
+  s->ptr.assign(nlev + 1, 0);
+  for (int l = 1; l <= nlev; l++) s->ptr[l] = s->ptr[l - 1] + cnt[l];
+  std::vector<int> pos(s->ptr.begin(), s->ptr.end() - 1), cells(n);
+  for (int e = 0; e < n; e++) cells[pos[lev[e] - 1]++] = e;
+  s->nlev = nlev;
+  return upload(c, &s->cells, cells);
+}
+
+static int sweep(cfdl_ctx *c, const Sched &s, const SysDesc &sys, int nc, double sor,
+                 const int *active) {
+  for (int l = 0; l < s.nlev; l++) {
+    const int n = s.ptr[l + 1] - s.ptr[l];
+    if (nc == 3)
+      hipLaunchKernelGGL(k_gs_level<3>, dim3(nblocks(n)), dim3(kBlock), 0, c->st, sys,
+                         s.cells + s.ptr[l], n, sor, active);
+    else
+      hipLaunchKernelGGL(k_gs_level<1>, dim3(nblocks(n)), dim3(kBlock), 0, c->st, sys,
+                         s.cells + s.ptr[l], n, sor, active);
+  }
+  c->gs_launches += s.nlev;
+  HIPCHK(hipGetLastError());
+  return CFDL_OK;
+}
+
+static int residual(cfdl_ctx *c, const SysDesc &sys, int nblk, int chunk, int absmax) {
+  hipLaunchKernelGGL(k_resid_partial, dim3(nblk, sys.ny), dim3(kBlock), 0, c->st, sys, chunk,
+                     nblk, absmax, c->part);
+  hipLaunchKernelGGL(k_resid_final, dim3(sys.ny), dim3(kBlock), 0, c->st, c->part, nblk, c->red);
+  HIPCHK(hipGetLastError());
+  return CFDL_OK;
+}
+
+static double sor_of(const char *name) {
+  return (name && name[0] == 'p' && name[1] == 'c') ? 1.02 : 1.0;
+}
+
+// solve_gs, mod_solver.f90:255-327, for ny right-hand sides on the global
+// system; the host checks the per-RHS stop flags after every iteration.
+static int *gs_active(cfdl_ctx *c) {
+  return reinterpret_cast<int *>(reinterpret_cast<char *>(c->gctl) + offsetof(GsCtl, active));
+}
+static int *pc_active(cfdl_ctx *c) {
+  return reinterpret_cast<int *>(reinterpret_cast<char *>(c->pctl) + offsetof(PcCtl, active));
+}
+
+static int solve_gs_n(cfdl_ctx *c, int ny, double **phi, const double **b,
+                      const char *const *names, double sor) {
+  Names nm{};
+  for (int y = 0; y < ny; y++) std::strncpy(nm.n[y], names[y], 7);
+  SysDesc s{};
+  s.ny = ny;
+  for (int y = 0; y < ny; y++) {
+    s.beg[y] = 0;
+    s.end[y] = c->ne;
+    s.phi[y] = phi[y];
+    s.b[y] = b[y];
+  }
+  s.row = c->md.row;
+  s.nbr = c->md.nbr;
+  s.ap = c->fd.ap;
+  s.anb = c->fd.anb;
+  CALL(residual(c, s, c->nblk_g, c->chunk_g, 1));
+  hipLaunchKernelGGL(k_gs_init, dim3(1), dim3(64), 0, c->st, c->gctl, c->red, ny,
+                     static_cast<double>(c->ne), c->nit);
+  // one SysDesc per sweep kernel with its RHS subset in slots 0..ny-1
+  for (int it = 0; it < c->nit; it++) {
+    if (c->timing) HIPCHK(hipEventRecord(c->ev0, c->st));
+    CALL(sweep(c, c->gf, s, ny, sor, gs_active(c)));
+    CALL(sweep(c, c->gb, s, ny, sor, gs_active(c)));
+    if (c->timing) {
+      HIPCHK(hipEventRecord(c->ev1, c->st));
+    }
+    CALL(residual(c, s, c->nblk_g, c->chunk_g, 1));
+    hipLaunchKernelGGL(k_gs_iter_end, dim3(1), dim3(64), 0, c->st, c->gctl, c->red, ny,
+                       static_cast<double>(c->ne), c->nit);
+    HIPCHK(hipMemcpyAsync(c->host_flags, gs_active(c), sizeof(int) * 3, hipMemcpyDeviceToHost,
+                          c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (c->timing) {
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+      c->gs_ms += ms;
+    }
+    bool any = false;
+    for (int y = 0; y < ny; y++) any = any || c->host_flags[y];
+    if (!any) break;
+  }
+  hipLaunchKernelGGL(k_log_gs, dim3(1), dim3(64), 0, c->st, c->gctl, ny, nm, c->dlog, c->dlog_n);
+  HIPCHK(hipGetLastError());
+  return CFDL_OK;
+}
+
+static int multi_subdomain_solver(cfdl_ctx *c) {
+  const int ne = c->ne, ns = c->nsubd;
+  // assemble_coef, mod_subdomains.f90:162-189
+  hipLaunchKernelGGL(k_pc_assemble, dim3(nblocks(ne)), dim3(kBlock), 0, c->st, ne, c->l2g_cell,
+                     c->rowp, c->md.row, c->fd.ap, c->fd.b, c->fd.anb, c->fd.phic, c->ap_p,
+                     c->b_p, c->anb_p, c->phic_p);
+  if (c->nh > 0)
+    hipLaunchKernelGGL(k_gather, dim3(nblocks(c->nh)), dim3(kBlock), 0, c->st, c->nh, c->halo_src,
+                       c->fd.phic, c->phic_p + ne);
+  SysDesc s{};
+  s.ny = ns;
+  SubCounts sc{};
+  sc.n = ns;
+  for (int y = 0; y < ns; y++) {
+    s.beg[y] = c->co[y];
+    s.end[y] = c->co[y + 1];
+    s.phi[y] = c->phic_p;
+    s.b[y] = c->b_p;
+    sc.ne[y] = static_cast<double>(c->co[y + 1] - c->co[y]);
+  }
+  s.row = c->rowp;
+  s.nbr = c->nbrp;
+  s.ap = c->ap_p;
+  s.anb = c->anb_p;
+  CALL(residual(c, s, c->nblk_p, c->chunk_p, 0));
+  hipLaunchKernelGGL(k_pc_init, dim3(1), dim3(64), 0, c->st, c->pctl, c->red, sc, c->nit);
+  SysDesc s1 = s;  // the sweeps see the whole concatenated system as one RHS
+  s1.ny = 1;
+  const int *act = pc_active(c);
+  const int nblock = (c->nit + 1) / 2;
+  if (c->timing) HIPCHK(hipEventRecord(c->ev0, c->st));
+  for (int blk = 1; blk <= nblock; blk++) {
+    for (int rep = 0; rep < 2; rep++) {  // smoother_gs(..., 2): two symmetric iterations
+      CALL(sweep(c, c->pf, s1, 1, 1.02, act));
+      CALL(sweep(c, c->pb, s1, 1, 1.02, act));
+    }
+    if (c->nx > 0)
+      hipLaunchKernelGGL(k_exchange, dim3(nblocks(c->nx)), dim3(kBlock), 0, c->st, c->nx, c->xdst,
+                         c->xsrc, c->phic_p, act);
+    const int has_res = ((2 * blk) % 10 == 0) ? 1 : 0;
+    if (has_res) CALL(residual(c, s, c->nblk_p, c->chunk_p, 0));
+    hipLaunchKernelGGL(k_pc_block_end, dim3(1), dim3(64), 0, c->st, c->pctl, c->red, sc, c->nit,
+                       has_res);
+  }
+  if (c->timing) {
+    HIPCHK(hipEventRecord(c->ev1, c->st));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->gs_ms += ms;
+  }
+  // update_phi, mod_subdomains.f90:214-229
+  hipLaunchKernelGGL(k_scatter, dim3(nblocks(ne)), dim3(kBlock), 0, c->st, ne, c->l2g_cell,
+                     c->phic_p, c->fd.phic);
+  hipLaunchKernelGGL(k_log_pc, dim3(1), dim3(64), 0, c->st, c->pctl, c->dlog, c->dlog_n);
+  HIPCHK(hipGetLastError());
+  return CFDL_OK;
+}
+
+static double *field_ptr(cfdl_ctx *c, int f) {
+  if (f < 0 || f >= CFDL_NFIELDS) return nullptr;
+  return c->fields[f];
+}
+
+static int check_ctx(cfdl_ctx *c) {
+  if (!c) {
+    set_error("null context");
+    return CFDL_EINVAL;
+  }
+  return CFDL_OK;
+}
+
+extern "C" {
+
+const char *cfdl_last_error(void) { return g_err.c_str(); }
+int cfdl_version(void) { return 100; }
+
+int cfdl_ctx_create(int ne, int nf, int nbf, const int *ef2nb_idx, const int *ef2nb,
+                    const int *s2g, const int *bs, const double *xc, const double *yc,
+                    const double *zc, const double *vol, const double *aip, const double *rip,
+                    int nsec, const int *esec, const int *bckind, int nsubd, const int *gf2g,
+                    const int *g2gf, const int *g2gf_idx, const double *rho, const double *mu,
+                    double dt, int nit, cfdl_ctx **out) {
+  if (!out || ne <= 0 || nf <= 0 || nbf < 0 || !ef2nb_idx || !ef2nb || !s2g || !xc || !yc ||
+      !zc || !vol || !aip || !rip || !rho || !mu || nsec <= 0 || !esec || !bckind ||
+      nsubd < 1 || nsubd > kMaxSys || nit < 0 || !(dt > 0.0) || (nbf > 0 && !bs) ||
+      (nsubd > 1 && (!gf2g || !g2gf || !g2gf_idx))) {
+    set_error("cfdl_ctx_create: bad arguments");
+    return CFDL_EINVAL;
+  }
+  trace("ctx_create: begin");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    set_error("cfdl_ctx_create: no HIP device visible");
+    return CFDL_ENODEV;
+  }
+  std::unique_ptr<cfdl_ctx> c(new cfdl_ctx());
+  HIPCHK(hipGetDevice(&c->dev));
+  trace("ctx_create: device");
+  HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+  trace("ctx_create: stream");
+  c->ne = ne;
+  c->nf = nf;
+  c->nbf = nbf;
+  c->L = 2 * nf - nbf;
+  c->nsubd = nsubd;
+  c->nit = nit;
+  c->dt = dt;
+  const int L = c->L, n = ne + nbf;
+  if (ef2nb_idx[0] != 1 || ef2nb_idx[ne] != L + 1) {
+    set_error("ef2nb_idx does not describe 2nf-nbf entries");
+    return CFDL_EINVAL;
+  }
+  // --- CSR, faces, halos (0-based) ---
+  std::vector<int> row(ne + 1), nbr(L), fsg(L), own(nf, -1), fcell(nf, -1);
+  for (int e = 0; e <= ne; e++) row[e] = ef2nb_idx[e] - 1;
+  for (int k = 0; k < L; k++) {
+    const int enb = pk_elem(ef2nb[k]) - 1, lf = pk_face(ef2nb[k]);
+    if (enb < 0 || enb >= n || (lf == 0) != (enb >= ne)) {
+      set_error("ef2nb entry " + std::to_string(k + 1) + " is inconsistent");
+      return CFDL_EINVAL;
+    }
+    nbr[k] = enb;
+    fsg[k] = ef2nb[L + k];
+    if (fsg[k] == 0 || std::abs(fsg[k]) > nf) {
+      set_error("ef2nb(:,2) entry " + std::to_string(k + 1) + " is not a face");
+      return CFDL_EINVAL;
+    }
+  }
+  for (int f = 0; f < nf; f++) {
+    const int e = pk_elem(s2g[f]) - 1, lf = pk_face(s2g[f]);
+    if (e < 0 || e >= ne || lf < 1 || row[e] + lf - 1 >= row[e + 1]) {
+      set_error("s2g entry " + std::to_string(f + 1) + " is inconsistent");
+      return CFDL_EINVAL;
+    }
+    own[f] = row[e] + lf - 1;
+    fcell[f] = e;
+  }
+  std::vector<int> hcell(nbf), hslot(nbf), hkind(nbf, CFDL_BC_WALL), bcrow(ne + 1, 0), bcslot(nbf);
+  for (int h = 0; h < nbf; h++) {
+    const int p = std::abs(bs[h]);
+    const int e = pk_elem(p) - 1, lf = pk_face(p);
+    if (e < 0 || e >= ne || lf < 1 || row[e] + lf - 1 >= row[e + 1] ||
+        nbr[row[e] + lf - 1] != ne + h) {
+      set_error("bs entry " + std::to_string(h + 1) + " is inconsistent");
+      return CFDL_EINVAL;
+    }
+    hcell[h] = e;
+    hslot[h] = row[e] + lf - 1;
+    bcrow[e + 1]++;
+  }
+  for (int s = 0; s < nsec; s++)
+    for (int el = esec[2 * s]; el <= esec[2 * s + 1]; el++)
+      if (el > ne && el <= n) hkind[el - ne - 1] = bckind[s];
+  for (int e = 0; e < ne; e++) bcrow[e + 1] += bcrow[e];
+  {
+    std::vector<int> pos(bcrow.begin(), bcrow.end() - 1);
+    for (int h = 0; h < nbf; h++) bcslot[pos[hcell[h]]++] = hslot[h];  // halo order per cell
+  }
+  trace("ctx_create: host tables");
+  cfdl_ctx *C = c.get();
+  int *d_row, *d_nbr, *d_fsg, *d_own, *d_fcell, *d_bcrow, *d_bcslot, *d_hcell, *d_hslot, *d_hkind;
+  CALL(upload(C, &d_row, row));
+  CALL(upload(C, &d_nbr, nbr));
+  CALL(upload(C, &d_fsg, fsg));
+  CALL(upload(C, &d_own, own));
+  CALL(upload(C, &d_fcell, fcell));
+  CALL(upload(C, &d_bcrow, bcrow));
+  CALL(upload(C, &d_bcslot, bcslot));
+  CALL(upload(C, &d_hcell, hcell));
+  CALL(upload(C, &d_hslot, hslot));
+  CALL(upload(C, &d_hkind, hkind));
+  auto up64 = [&](double **p, const double *src, int64_t cnt) -> int {
+    CALL(dalloc(C, p, cnt));
+    HIPCHK(hipMemcpy(*p, src, 8 * cnt, hipMemcpyHostToDevice));
+    return CFDL_OK;
+  };
+  double *d_xc, *d_yc, *d_zc, *d_vol, *d_aip, *d_rip, *d_rho, *d_mu;
+  CALL(up64(&d_xc, xc, n));
+  CALL(up64(&d_yc, yc, n));
+  CALL(up64(&d_zc, zc, n));
+  CALL(up64(&d_vol, vol, ne));
+  CALL(up64(&d_aip, aip, 3 * int64_t{nf}));
+  CALL(up64(&d_rip, rip, 3 * int64_t{nf}));
+  CALL(up64(&d_rho, rho, ne));
+  CALL(up64(&d_mu, mu, ne));
+  C->md = MeshDev{ne, nf, nbf, L, d_row, d_nbr, d_fsg, d_own, d_fcell, d_bcrow, d_bcslot,
+                  d_hcell, d_hslot, d_hkind, d_xc, d_yc, d_zc, d_vol, d_aip, d_rip, d_rho, d_mu};
+  trace("ctx_create: mesh uploaded");
+  // --- fields ---
+  const int64_t sz[CFDL_NFIELDS] = {n, n, n, n, n, n, n, 3 * int64_t{n}, 3 * int64_t{n},
+                                    3 * int64_t{n}, 3 * int64_t{n}, 3 * int64_t{n}, nf, nf, ne,
+                                    ne, ne, ne, ne, ne, ne, L, n};
+  for (int f = 0; f < CFDL_NFIELDS; f++) {
+    C->fsize[f] = sz[f];
+    CALL(dalloc(C, &C->fields[f], sz[f]));
+    hipLaunchKernelGGL(k_fill, dim3(nblocks(sz[f])), dim3(kBlock), 0, C->st, C->fields[f], sz[f], 0.0);
+  }
+  double **F = C->fields;
+  C->fd = FieldsDev{F[CFDL_U], F[CFDL_V], F[CFDL_W], F[CFDL_P], F[CFDL_U0], F[CFDL_V0],
+                    F[CFDL_W0], F[CFDL_GU], F[CFDL_GV], F[CFDL_GW], F[CFDL_GP], F[CFDL_GPC],
+                    F[CFDL_MIP], F[CFDL_MIP0], F[CFDL_BU], F[CFDL_BV], F[CFDL_BW], F[CFDL_D],
+                    F[CFDL_DC], F[CFDL_AP], F[CFDL_B], F[CFDL_ANB], F[CFDL_PHIC]};
+  trace("ctx_create: fields");
+  // --- global Gauss-Seidel schedules ---
+  {
+    std::vector<int> lev(ne);
+    levels(row, nbr, 0, ne, true, lev);
+    CALL(make_sched(C, lev, ne, &C->gf));
+    levels(row, nbr, 0, ne, false, lev);
+    CALL(make_sched(C, lev, ne, &C->gb));
+  }
+  // --- subdomain systems (construct_subdomains, mod_subdomains.f90:18-160):
+  // local cells in g2gf order, CSR rows in the global row order, halo slots
+  // numbered in encounter order, each interface face a copy pair ---
+  trace("ctx_create: global schedules");
+  if (nsubd > 1) {
+    C->co.assign(nsubd + 1, 0);
+    std::vector<int> cid_of(ne, -1), sd_of(ne, -1), l2g(ne);
+    for (int s = 0; s < nsubd; s++) {
+      C->co[s] = g2gf_idx[s] - 1;
+      if (g2gf_idx[s + 1] <= g2gf_idx[s]) {
+        set_error("g2gf_idx describes an empty subdomain");
+        return CFDL_EINVAL;
+      }
+    }
+    C->co[nsubd] = g2gf_idx[nsubd] - 1;
+    if (C->co[0] != 0 || C->co[nsubd] != ne) {
+      set_error("g2gf_idx does not cover the cells");
+      return CFDL_EINVAL;
+    }
+    for (int s = 0; s < nsubd; s++)
+      for (int q = C->co[s]; q < C->co[s + 1]; q++) {
+        const int g = g2gf[q] - 1;
+        if (g < 0 || g >= ne || cid_of[g] >= 0 || gf2g[g] != s + 1) {
+          set_error("g2gf / gf2g are not a consistent partition");
+          return CFDL_EINVAL;
+        }
+        cid_of[g] = q;
+        sd_of[g] = s;
+        l2g[q] = g;
+      }
+    std::vector<int> rowp(ne + 1, 0), nbrp, halo_src, xdst, xsrc;
+    nbrp.reserve(L);
+    for (int q = 0; q < ne; q++) rowp[q + 1] = rowp[q] + (row[l2g[q] + 1] - row[l2g[q]]);
+    int nh = 0;
+    for (int q = 0; q < ne; q++) {
+      const int g = l2g[q], s = sd_of[g];
+      for (int k = row[g]; k < row[g + 1]; k++) {
+        const int nb = nbr[k];
+        if (nb < ne && sd_of[nb] == s) {
+          nbrp.push_back(cid_of[nb]);
+        } else {
+          const int slot = ne + nh++;
+          nbrp.push_back(slot);
+          halo_src.push_back(nb);
+          if (nb < ne) {
+            xdst.push_back(slot);
+            xsrc.push_back(cid_of[nb]);
+          }
+        }
+      }
+    }
+    C->Lp = static_cast<int>(nbrp.size());
+    C->nh = nh;
+    C->nx = static_cast<int>(xdst.size());
+    CALL(upload(C, &C->rowp, rowp));
+    CALL(upload(C, &C->nbrp, nbrp));
+    CALL(upload(C, &C->l2g_cell, l2g));
+    CALL(upload(C, &C->halo_src, halo_src));
+    CALL(upload(C, &C->xdst, xdst));
+    CALL(upload(C, &C->xsrc, xsrc));
+    CALL(dalloc(C, &C->ap_p, ne));
+    CALL(dalloc(C, &C->b_p, ne));
+    CALL(dalloc(C, &C->anb_p, C->Lp));
+    CALL(dalloc(C, &C->phic_p, int64_t{ne} + nh));
+    std::vector<int> lev(ne);
+    for (int s = 0; s < nsubd; s++) levels(rowp, nbrp, C->co[s], C->co[s + 1], true, lev);
+    CALL(make_sched(C, lev, ne, &C->pf));
+    for (int s = 0; s < nsubd; s++) levels(rowp, nbrp, C->co[s], C->co[s + 1], false, lev);
+    CALL(make_sched(C, lev, ne, &C->pb));
+    int maxsd = 0;
+    for (int s = 0; s < nsubd; s++) maxsd = std::max(maxsd, C->co[s + 1] - C->co[s]);
+    C->chunk_p = std::max(kBlock * 8, (maxsd + 1023) / 1024);
+    C->nblk_p = (maxsd + C->chunk_p - 1) / C->chunk_p;
+  }
+  trace("ctx_create: subdomains");
+  C->chunk_g = std::max(kBlock * 8, (ne + 1023) / 1024);
+  C->nblk_g = (ne + C->chunk_g - 1) / C->chunk_g;
+  const int nblk = std::max(C->nblk_g * 3, C->nblk_p * nsubd);
+  CALL(dalloc(C, &C->part, 2 * int64_t{nblk}));
+  CALL(dalloc(C, &C->red, 2 * kMaxSys));
+  CALL(dalloc(C, &C->scal, 8));
+  CALL(dalloc(C, &C->gctl, 1));
+  CALL(dalloc(C, &C->pctl, 1));
+  CALL(dalloc(C, &C->dlog, kLogCap));
+  CALL(dalloc(C, &C->dlog_n, 1));
+  {
+    const int zero = 0;
+    HIPCHK(hipMemcpy(C->dlog_n, &zero, sizeof(int), hipMemcpyHostToDevice));
+  }
+  HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&C->host_flags), 64, hipHostMallocDefault));
+  HIPCHK(hipEventCreate(&C->ev0));
+  HIPCHK(hipEventCreate(&C->ev1));
+  HIPCHK(hipStreamSynchronize(C->st));
+  trace("ctx_create: done");
+  *out = c.release();
+  return CFDL_OK;
+}
+
+void cfdl_ctx_destroy(cfdl_ctx *c) {
+  if (!c) return;
+  (void)hipStreamSynchronize(c->st);
+  if (c->host_flags) (void)hipHostFree(c->host_flags);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->st) (void)hipStreamDestroy(c->st);
+  delete c;
+}
+
+int cfdl_ctx_create_cavity(const cfdl_mesh *m, double rho, double mu, double dt, int nit,
+                           cfdl_ctx **out) {
+  if (!m || m->nsubd < 1) {
+    set_error("cfdl_ctx_create_cavity: mesh is not partitioned (call cfdl_mesh_partition)");
+    return CFDL_ESTATE;
+  }
+  std::vector<int> kind(m->nsec, CFDL_BC_WALL);
+  for (int s = 0; s < m->nsec; s++)
+    if (s < static_cast<int>(m->secname.size()) && m->secname[s] == "top") kind[s] = CFDL_BC_LID;
+  std::vector<double> r(m->ne, rho), u(m->ne, mu);
+  return cfdl_ctx_create(m->ne, m->nf, m->nbf, m->ef2nb_idx.data(), m->ef2nb.data(),
+                         m->s2g.data(), m->bs.data(), m->xc.data(), m->yc.data(), m->zc.data(),
+                         m->vol.data(), m->aip.data(), m->rip.data(), m->nsec, m->esec.data(),
+                         kind.data(), m->nsubd, m->gf2g.data(), m->g2gf.data(),
+                         m->g2gf_idx.data(), r.data(), u.data(), dt, nit, out);
+}
+
+int cfdl_set_field(cfdl_ctx *c, int field, const double *src, int64_t n) {
+  CALL(check_ctx(c));
+  double *p = field_ptr(c, field);
+  if (!p || !src || n != c->fsize[field]) {
+    set_error("cfdl_set_field: bad field id or size");
+    return CFDL_EINVAL;
+  }
+  HIPCHK(hipMemcpyAsync(p, src, 8 * n, hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return CFDL_OK;
+}
+
+int cfdl_get_field(cfdl_ctx *c, int field, double *dst, int64_t n) {
+  CALL(check_ctx(c));
+  double *p = field_ptr(c, field);
+  if (!p || !dst || n != c->fsize[field]) {
+    set_error("cfdl_get_field: bad field id or size");
+    return CFDL_EINVAL;
+  }
+  HIPCHK(hipMemcpyAsync(dst, p, 8 * n, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return CFDL_OK;
+}
+
+int cfdl_synchronize(cfdl_ctx *c) {
+  CALL(check_ctx(c));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return CFDL_OK;
+}
+
+int cfdl_update_boundaries(cfdl_ctx *c) {
+  CALL(check_ctx(c));
+  if (c->nbf > 0)
+    hipLaunchKernelGGL(k_bc_update, dim3(nblocks(c->nbf)), dim3(kBlock), 0, c->st, c->md, c->fd);
+  HIPCHK(hipGetLastError());
+  return CFDL_OK;
+}
+
+int cfdl_calc_coef_uvw(cfdl_ctx *c) {
+  CALL(check_ctx(c));
+  hipLaunchKernelGGL(k_coef_uvw, dim3(nblocks(c->ne)), dim3(kBlock), 0, c->st, c->md, c->fd, c->dt);
+  HIPCHK(hipGetLastError());
+  return CFDL_OK;
+}
+
+int cfdl_solve_gs(cfdl_ctx *c, const char *name, int field, int rhs) {
+  CALL(check_ctx(c));
+  double *phi = field_ptr(c, field);
+  double *b = field_ptr(c, rhs);
+  if (!phi || !b || c->fsize[field] != c->ne + c->nbf || c->fsize[rhs] != c->ne || !name) {
+    set_error("cfdl_solve_gs: phi must be an element field and rhs a cell field");
+    return CFDL_EINVAL;
+  }
+  const double *bb = b;
+  return solve_gs_n(c, 1, &phi, &bb, &name, sor_of(name));
+}
+
+int cfdl_solve_gs_uvw(cfdl_ctx *c) {
+  CALL(check_ctx(c));
+  double *phi[3] = {c->fd.u, c->fd.v, c->fd.w};
+  const double *b[3] = {c->fd.bu, c->fd.bv, c->fd.bw};
+  const char *names[3] = {"u", "v", "w"};
+  return solve_gs_n(c, 3, phi, b, names, 1.0);
+}
+
+int cfdl_calc_grad(cfdl_ctx *c, int src, int dst) {
+  CALL(check_ctx(c));
+  double *p = field_ptr(c, src), *g = field_ptr(c, dst);
+  if (!p || !g || c->fsize[src] != c->ne + c->nbf || c->fsize[dst] != 3 * int64_t{c->ne + c->nbf}) {
+    set_error("cfdl_calc_grad: src must be an element field, dst a gradient field");
+    return CFDL_EINVAL;
+  }
+  hipLaunchKernelGGL(k_grad, dim3(nblocks(c->ne)), dim3(kBlock), 0, c->st, c->md, p, g);
+  HIPCHK(hipGetLastError());
+  return CFDL_OK;
+}
+
+int cfdl_calc_mip(cfdl_ctx *c, int rhie_chow) {
+  CALL(check_ctx(c));
+  hipLaunchKernelGGL(k_mip, dim3(nblocks(c->nf)), dim3(kBlock), 0, c->st, c->md, c->fd,
+                     rhie_chow ? 1 : 0, c->dt);
+  HIPCHK(hipGetLastError());
+  return CFDL_OK;
+}
+
+int cfdl_calc_coef_p(cfdl_ctx *c) {
+  CALL(check_ctx(c));
+  hipLaunchKernelGGL(k_coef_p, dim3(nblocks(c->ne)), dim3(kBlock), 0, c->st, c->md, c->fd);
+  HIPCHK(hipGetLastError());
+  return CFDL_OK;
+}
+
+int cfdl_set_a_0(cfdl_ctx *c, int field) {
+  CALL(check_ctx(c));
+  double *p = field_ptr(c, field);
+  if (!p) {
+    set_error("cfdl_set_a_0: bad field id");
+    return CFDL_EINVAL;
+  }
+  hipLaunchKernelGGL(k_fill, dim3(nblocks(c->fsize[field])), dim3(kBlock), 0, c->st, p,
+                     c->fsize[field], 0.0);
+  HIPCHK(hipGetLastError());
+  return CFDL_OK;
+}
+
+int cfdl_solve(cfdl_ctx *c, const char *name) {
+  CALL(check_ctx(c));
+  if (c->nsubd <= 1) {
+    double *phi = c->fd.phic;
+    const double *b = c->fd.b;
+    const char *nm = name ? name : "pc";
+    return solve_gs_n(c, 1, &phi, &b, &nm, sor_of(nm));
+  }
+  return multi_subdomain_solver(c);
+}
+
+int cfdl_adjust_pc(cfdl_ctx *c) {
+  CALL(check_ctx(c));
+  hipLaunchKernelGGL(k_pref, dim3(1), dim3(64), 0, c->st, c->fd.phic, c->scal);
+  hipLaunchKernelGGL(k_sub_pref, dim3(nblocks(c->ne)), dim3(kBlock), 0, c->st, c->ne, c->fd.phic,
+                     c->scal);
+  if (c->nbf > 0)
+    hipLaunchKernelGGL(k_halo_from_owner, dim3(nblocks(c->nbf)), dim3(kBlock), 0, c->st, c->md,
+                       c->fd.phic);
+  HIPCHK(hipGetLastError());
+  return CFDL_OK;
+}
+
+int cfdl_update_uvwp(cfdl_ctx *c) {
+  CALL(check_ctx(c));
+  hipLaunchKernelGGL(k_update_cells, dim3(nblocks(c->ne)), dim3(kBlock), 0, c->st, c->ne, c->fd);
+  hipLaunchKernelGGL(k_update_faces, dim3(nblocks(c->nf)), dim3(kBlock), 0, c->st, c->md, c->fd);
+  HIPCHK(hipGetLastError());
+  return CFDL_OK;
+}
+
+int cfdl_solve_uvwp(cfdl_ctx *c) {
+  CALL(check_ctx(c));
+  CALL(cfdl_calc_coef_uvw(c));
+  CALL(cfdl_solve_gs_uvw(c));
+  CALL(cfdl_calc_grad(c, CFDL_U, CFDL_GU));
+  CALL(cfdl_calc_grad(c, CFDL_V, CFDL_GV));
+  CALL(cfdl_calc_grad(c, CFDL_W, CFDL_GW));
+  CALL(cfdl_calc_mip(c, 1));
+  CALL(cfdl_calc_coef_p(c));
+  CALL(cfdl_set_a_0(c, CFDL_PHIC));
+  CALL(cfdl_solve(c, "pc"));
+  CALL(cfdl_adjust_pc(c));
+  CALL(cfdl_calc_grad(c, CFDL_PHIC, CFDL_GPC));
+  CALL(cfdl_update_uvwp(c));
+  c->outer++;
+  return CFDL_OK;
+}
+
+int cfdl_update_time(cfdl_ctx *c) {
+  CALL(check_ctx(c));
+  const int64_t n = c->ne + c->nbf;
+  hipLaunchKernelGGL(k_copy, dim3(nblocks(n)), dim3(kBlock), 0, c->st, c->fd.u0, c->fd.u, n);
+  hipLaunchKernelGGL(k_copy, dim3(nblocks(n)), dim3(kBlock), 0, c->st, c->fd.v0, c->fd.v, n);
+  hipLaunchKernelGGL(k_copy, dim3(nblocks(n)), dim3(kBlock), 0, c->st, c->fd.w0, c->fd.w, n);
+  hipLaunchKernelGGL(k_copy, dim3(nblocks(c->nf)), dim3(kBlock), 0, c->st, c->fd.mip0, c->fd.mip,
+                     int64_t{c->nf});
+  HIPCHK(hipGetLastError());
+  return CFDL_OK;
+}
+
+int cfdl_run(cfdl_ctx *c, int ntstep, int ncoef) {
+  CALL(check_ctx(c));
+  for (int t = 0; t < ntstep; t++) {
+    for (int k = 0; k < ncoef; k++) {
+      CALL(cfdl_update_boundaries(c));
+      CALL(cfdl_solve_uvwp(c));
+    }
+    CALL(cfdl_update_time(c));
+  }
+  return CFDL_OK;
+}
+
+static int drain_log(cfdl_ctx *c) {
+  HIPCHK(hipStreamSynchronize(c->st));
+  int n = 0;
+  HIPCHK(hipMemcpy(&n, c->dlog_n, sizeof(int), hipMemcpyDeviceToHost));
+  if (n > 0) {
+    std::vector<LogLine> tmp(n);
+    HIPCHK(hipMemcpy(tmp.data(), c->dlog, sizeof(LogLine) * n, hipMemcpyDeviceToHost));
+    c->hlog.insert(c->hlog.end(), tmp.begin(), tmp.end());
+    const int zero = 0;
+    HIPCHK(hipMemcpy(c->dlog_n, &zero, sizeof(int), hipMemcpyHostToDevice));
+  }
+  return CFDL_OK;
+}
+
+int cfdl_log_count(cfdl_ctx *c, int *n) {
+  CALL(check_ctx(c));
+  CALL(drain_log(c));
+  if (n) *n = static_cast<int>(c->hlog.size());
+  return CFDL_OK;
+}
+
+int cfdl_log_get(cfdl_ctx *c, int k, char name[8], int *it, double *res_i, double *res_f,
+                 double *res_max) {
+  CALL(check_ctx(c));
+  if (k < 0 || k >= static_cast<int>(c->hlog.size())) {
+    set_error("cfdl_log_get: index out of range (call cfdl_log_count first)");
+    return CFDL_EINVAL;
+  }
+  const LogLine &l = c->hlog[k];
+  if (name) std::memcpy(name, l.name, 8);
+  if (it) *it = l.it;
+  if (res_i) *res_i = l.res_i;
+  if (res_f) *res_f = l.res_f;
+  if (res_max) *res_max = l.res_max;
+  return CFDL_OK;
+}
+
+int cfdl_log_clear(cfdl_ctx *c) {
+  CALL(check_ctx(c));
+  CALL(drain_log(c));
+  c->hlog.clear();
+  return CFDL_OK;
+}
+
+int cfdl_stats(cfdl_ctx *c, double *stats, int n) {
+  CALL(check_ctx(c));
+  const double v[3] = {static_cast<double>(c->gs_launches), c->gs_ms, static_cast<double>(c->outer)};
+  for (int i = 0; i < n && i < 3; i++) stats[i] = v[i];
+  return CFDL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,170 @@
+/*
+ * cfdl.h -- C ABI of libcfdl_hip.so, the MI355X (gfx950) implementation of
+ * CFD-Lite's SIMPLE pressure-velocity coupling loop.
+ *
+ * The reference is Fortran; its hot path is the module procedures of
+ * src/equations/mod_uvwp.f90 and src/modules/mod_solver.f90.  Their derived
+ * types (uvwp_t, geometry_t, subdomain_t) carry allocatables and type-bound
+ * procedures and cannot cross bind(C), so this ABI takes the component arrays
+ * (plain pointers + sizes) and keeps device copies in an opaque context.  A
+ * Fortran host binds every function below with iso_c_binding (the interface
+ * module is cfd-lite_amd/fortran/cfdl_c.f90).
+ *
+ * Conventions are the reference's: reals are IEEE double, integers int32,
+ * arrays are passed as the reference stores them (Fortran column-major,
+ * 1-based packed indices (elem << 5) | local_face inside ef2nb/s2g/bs,
+ * src/modules/mod_util.f90:1428-1448).  Every function returns 0 on success
+ * or a CFDL_E* code; cfdl_last_error() gives the message (the reference stops
+ * the program instead, src/modules/mod_eqn_setup.f90:63-66).
+ *
+ * Threading: one context = one HIP stream = one calling thread.  Calls on a
+ * context are asynchronous with respect to the host except where stated;
+ * cfdl_get_field and the log read-out synchronise.
+ */
+#ifndef CFDL_H
+#define CFDL_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum cfdl_status {
+  CFDL_OK = 0,
+  CFDL_EINVAL = 1, /* bad argument or size */
+  CFDL_EHIP = 2,   /* HIP runtime error */
+  CFDL_ENOMEM = 3, /* host or device allocation failed */
+  CFDL_ESTATE = 4, /* call out of order (e.g. no partition) */
+  CFDL_ENODEV = 5  /* no GPU visible */
+};
+
+/* boundary-section kinds: the BC setters of src/equations/mod_uvwp.f90 */
+enum cfdl_bc_kind {
+  CFDL_BC_WALL = 0,     /* dirichlet0, mod_uvwp.f90:493-515 */
+  CFDL_BC_LID = 1,      /* lid (u=1), mod_uvwp.f90:548-570 */
+  CFDL_BC_SYMMETRY = 2  /* symmetry (zero_flux), mod_uvwp.f90:517-546 */
+};
+
+/* fields of uvwp_t (mod_uvwp.f90:6-16) and the shared work arrays of phys_t
+   (mod_physics.f90:58-63).  Sizes: CELLS+HALOS = ne+nbf, CELLS = ne,
+   FACES = nf, CSR = 2nf-nbf, gradients 3 per element (x,y,z interleaved). */
+enum cfdl_field {
+  CFDL_U = 0, CFDL_V, CFDL_W, CFDL_P, CFDL_U0, CFDL_V0, CFDL_W0, /* ne+nbf   */
+  CFDL_GU, CFDL_GV, CFDL_GW, CFDL_GP, CFDL_GPC,                   /* 3(ne+nbf) */
+  CFDL_MIP, CFDL_MIP0,                                            /* nf       */
+  CFDL_BU, CFDL_BV, CFDL_BW, CFDL_D, CFDL_DC,                     /* ne       */
+  CFDL_AP, CFDL_B,                                                /* ne       */
+  CFDL_ANB,                                                       /* 2nf-nbf  */
+  CFDL_PHIC,                                                      /* ne+nbf   */
+  CFDL_NFIELDS
+};
+
+const char *cfdl_last_error(void);
+int cfdl_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Host-side setup.  Replaces the reference's pre-processing on a generated   */
+/* or given element list: find_element_nb (src/setup/mod_mg_lvl_uns.f90:     */
+/* 283-488), calc_aip_xyzip_uns (src/setup/calc_aip_xyzip.f90:7-72),          */
+/* calc_vol_cv_centers_uns (src/setup/calc_vol_cv_centers.f90:4-59) and the   */
+/* subdomain partition generate_seeds + add_transformation_bt                 */
+/* (src/setup/mod_mg_lvl_uns.f90:95-118, 873-895).                            */
+/* ------------------------------------------------------------------------ */
+typedef struct cfdl_mesh cfdl_mesh;
+
+/* Hexahedral box, CGNS HEXA_8 cells (i fastest) followed by QUAD_4 sections
+   bottom, south, east, north, west, top.  jitter > 0 moves interior vertices
+   by uniform(-jitter*h, jitter*h) per coordinate (splitmix64 stream, seed). */
+int cfdl_mesh_box(int nx, int ny, int nz, double lx, double ly, double lz, double jitter,
+                  uint64_t seed, cfdl_mesh **out);
+/* Same input the reference reads from CGNS (cell_input.f90:36-93): sections
+   sorted by element range, e2vx with 8 slots per element. */
+int cfdl_mesh_from_elements(int nvx, const double *x, const double *y, const double *z,
+                            int nsec, const int *etype, const int *esec, const int *e2vx,
+                            cfdl_mesh **out);
+/* Oct-tree bisection into nsubd subdomains and the subdomain-local order
+   (nsubd = 1: no partition, the pc solve uses solve_gs). */
+int cfdl_mesh_partition(cfdl_mesh *m, int nsubd);
+int cfdl_mesh_info(const cfdl_mesh *m, int *ne, int *nf, int *nbf, int *nsec, int *nsubd,
+                   int *nvx);
+/* Copy out one setup array by name: "ef2nb_idx" (ne+1), "ef2nb" (2(2nf-nbf),
+   column-major), "s2g" (nf), "bs" (nbf), "esec" (2nsec), "etype" (nsec),
+   "gf2g" (ne), "g2gf" (ne), "g2gf_idx" (nsubd+1), "e2vx" (8(ne+nbf));
+   f64: "xc","yc","zc" (ne+nbf), "vol" (ne), "aip","rip" (3nf), "x","y","z". */
+int cfdl_mesh_get_i32(const cfdl_mesh *m, const char *name, int *dst, int64_t n);
+int cfdl_mesh_get_f64(const cfdl_mesh *m, const char *name, double *dst, int64_t n);
+void cfdl_mesh_free(cfdl_mesh *m);
+
+/* ------------------------------------------------------------------------ */
+/* Device context: the drop-in for the hot path.                             */
+/* ------------------------------------------------------------------------ */
+typedef struct cfdl_ctx cfdl_ctx;
+
+/* Uploads the static mesh (geometry_t + meshds_t arrays, mod_mg_lvl_uns.f90:
+   46-58, mod_meshds_uns.f90:14-26), the BC binding per section, the
+   partition (g2gf(1), gf2g(1)) from which the subdomain systems of
+   construct_subdomains (src/modules/mod_subdomains.f90:18-160) are built,
+   rho/mu per cell, dt and the linear-solver cap nit (phys_t,
+   mod_physics.f90:13-35).  ef2nb is the reference's ef2nb(L,2).  bckind
+   has one entry per section (ignored for cell sections).  Fields start at
+   zero and mip = mip0 = 0, as construct_uvwp leaves them
+   (mod_uvwp.f90:57-82). */
+int cfdl_ctx_create(int ne, int nf, int nbf, const int *ef2nb_idx, const int *ef2nb,
+                    const int *s2g, const int *bs, const double *xc, const double *yc,
+                    const double *zc, const double *vol, const double *aip, const double *rip,
+                    int nsec, const int *esec, const int *bckind, int nsubd, const int *gf2g,
+                    const int *g2gf, const int *g2gf_idx, const double *rho, const double *mu,
+                    double dt, int nit, cfdl_ctx **out);
+/* Convenience: a context from a cfdl_mesh with uniform rho, mu and the
+   lid-driven-cavity binding ('top' -> lid, every other section -> wall,
+   mod_uvwp.f90:73-78). */
+int cfdl_ctx_create_cavity(const cfdl_mesh *m, double rho, double mu, double dt, int nit,
+                           cfdl_ctx **out);
+void cfdl_ctx_destroy(cfdl_ctx *c);
+
+/* host <-> device copies of one field (n = element count of that field) */
+int cfdl_set_field(cfdl_ctx *c, int field, const double *src, int64_t n);
+int cfdl_get_field(cfdl_ctx *c, int field, double *dst, int64_t n);
+int cfdl_synchronize(cfdl_ctx *c);
+
+/* One entry point per reference routine, all on device-resident state.    */
+int cfdl_update_boundaries(cfdl_ctx *c);   /* mod_physics.f90:38-50 (uvwp BCs) */
+int cfdl_calc_coef_uvw(cfdl_ctx *c);       /* mod_uvwp.f90:161-286             */
+/* solve_gs (mod_solver.f90:255-327) on the shared ap/anb for phi = field
+   (CFDL_U/V/W or CFDL_PHIC) with right-hand side rhs (CFDL_BU/BV/BW or
+   CFDL_B); name decides the SOR factor ("pc..." -> 1.02) and labels the log. */
+int cfdl_solve_gs(cfdl_ctx *c, const char *name, int field, int rhs);
+/* the three momentum solves of solve_uvwp (mod_uvwp.f90:114-116) at once */
+int cfdl_solve_gs_uvw(cfdl_ctx *c);
+int cfdl_calc_grad(cfdl_ctx *c, int src, int dst); /* mod_solver.f90:40-81 */
+int cfdl_calc_mip(cfdl_ctx *c, int rhie_chow);     /* mod_uvwp.f90:438-490 */
+int cfdl_calc_coef_p(cfdl_ctx *c);                 /* mod_uvwp.f90:289-368 */
+int cfdl_set_a_0(cfdl_ctx *c, int field);          /* mod_util.f90:425-435 */
+/* solve (mod_solver.f90:329-344): multi_subdomain_solver (:124-189) on
+   phic with ap/anb/b, or solve_gs when the context has one subdomain */
+int cfdl_solve(cfdl_ctx *c, const char *name);
+int cfdl_adjust_pc(cfdl_ctx *c);   /* pref = phic(1); mod_uvwp.f90:129-130,136-158 */
+int cfdl_update_uvwp(cfdl_ctx *c); /* mod_uvwp.f90:370-436 */
+int cfdl_solve_uvwp(cfdl_ctx *c);  /* mod_uvwp.f90:95-134, fused */
+int cfdl_update_time(cfdl_ctx *c); /* mod_physics.f90:101-112 (uvwp part) */
+/* ntstep x ncoef outer iterations with the loop nesting of src/main.f90:50-63 */
+int cfdl_run(cfdl_ctx *c, int ntstep, int ncoef);
+
+/* Residual log: one line per linear solve, the values the reference prints
+   with format (5x,A16,x,i5,x,15x,es9.3e2,...) at mod_solver.f90:6,184,325.
+   Reading synchronises the context. */
+int cfdl_log_count(cfdl_ctx *c, int *n);
+int cfdl_log_get(cfdl_ctx *c, int k, char name[8], int *it, double *res_i, double *res_f,
+                 double *res_max);
+int cfdl_log_clear(cfdl_ctx *c);
+
+/* Timing of the last cfdl_run (device time of the dominant kernel class):
+   stats[0] = GS level launches, stats[1] = total GS kernel milliseconds
+   (HIP events on the context stream), stats[2] = outer iterations. */
+int cfdl_stats(cfdl_ctx *c, double *stats, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,199 @@
+"""SIMPLE throughput on one MI355X: cell-iterations per second.
+
+Workload (the GPU FP64 configuration): lid-driven cavity on a uniform
+128x128x128 hex box (2,097,152 cells, unstructured face lists), Re=400
+(rho=1, mu=0.0025), lid u=1, fields start at zero; the reference SIMPLE(C)
+outer iteration (solve_uvwp) with its exact-order Gauss-Seidel solvers and
+4-subdomain pressure-correction solver, FP64, pseudo-time step dt=0.01
+with 3 outer iterations per step, plus config-mode under-relaxation 0.7 (u)
+and 0.3 (p).  One step = one outer iteration (update_boundaries +
+solve_uvwp; update_time after every third).  Inputs are resident in HBM when
+the timed region starts.
+
+Prints one JSON line.  value = 2,097,152 * K / seconds for K timed steps.
+roofline: the pressure-correction Gauss-Seidel level kernel k_gs_range (the dominant
+kernel), 108 algorithmic bytes per cell update (FP64 CSR sweep: b, ap, 6
+anb, 6 neighbour indices, neighbour/own phi read, phi write) times the cells
+of one launch, over the mean launch duration from HIP events on the context
+stream around every 97th launch of the timed region.
+cpu_baseline: the reference Fortran (flang -O2 build in oracle/_ref) on one
+host core, 64^3 sample (see --cpu-*), or the oracle's C restatement when the
+reference build is absent.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "cfd-lite_amd"))
+
+METRIC = "SIMPLE cell-iters/s at 1e-6 residual; batched solves/s + p50/p99 latency"
+BYTES_PER_CELL_SWEEP = 108.0
+HBM_PEAK_GBS = 8000.0
+
+
+def scaled_residual(log):
+    """Per-equation initial residual divided by its max over the first five
+    outer iterations; returns the largest over u, v, w, pc at the last one."""
+    hist = {}
+    for name, it, ri, rf, rm in log:
+        hist.setdefault(name, []).append(ri)
+    worst = 0.0
+    for name, h in hist.items():
+        ref = max(h[:5]) if h else 0.0
+        if ref > 0:
+            worst = max(worst, h[-1] / ref)
+    return worst
+
+
+def write_ref_mesh(mesh, path):
+    """Mesh file read by the reference driver (oracle/refdrv/drv_cell.f90)."""
+    info = mesh.info()
+    names = mesh.names
+    etype, esec = mesh.get("etype"), mesh.get("esec")
+    with open(path, "wb") as f:
+        f.write(np.array([info["nvx"], info["nsec"]], np.int32).tobytes())
+        for s in range(info["nsec"]):
+            f.write(names[s].encode().ljust(32, b" "))
+            f.write(np.array([etype[s], esec[2 * s], esec[2 * s + 1]], np.int32).tobytes())
+        for k in ("x", "y", "z"):
+            f.write(mesh.get(k).astype("<f8").tobytes())
+        f.write(mesh.get("e2vx").astype("<i4").tobytes())
+
+
+def cpu_baseline(n, nouter, mu):
+    import cfdl
+    ref = os.path.join(REPO, "oracle", "_ref", "refsim_O2")
+    ncells = n ** 3
+    if os.path.exists(ref):
+        mesh = cfdl.Mesh.box(n)
+        with tempfile.TemporaryDirectory() as td:
+            mf, of = os.path.join(td, "mesh.bin"), os.path.join(td, "out.bin")
+            write_ref_mesh(mesh, mf)
+
+            def unlimited():
+                import resource
+                resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY,) * 2)
+                os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+
+            p = subprocess.run([ref, mf, of, "4", "1.0", repr(mu), "0.01", "1", str(nouter), "100"],
+                               capture_output=True, text=True, timeout=600, preexec_fn=unlimited)
+            if p.returncode != 0:
+                raise RuntimeError("reference run failed: " + p.stderr[-500:])
+            secs = [float(l.split()[-1]) for l in p.stdout.splitlines() if l.startswith("refsim: seconds")][0]
+        return {"value": ncells * nouter / secs, "unit": "cell-iterations/s", "cores": 1,
+                "kind": "reference",
+                "sample": "%d^3 lid-driven cavity, Re=%g, %d outer iterations of the reference "
+                          "Fortran (flang -O2, 4 subdomains, dt=0.01, no relaxation) on one host "
+                          "core, setup excluded; %.1f s" % (n, 1.0 / mu, nouter, secs)}
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import oracle_lib  # test infrastructure: the oracle as checker/baseline only
+    mesh = cfdl.Mesh.box(n)
+    g = {"nvx": mesh.info()["nvx"], "x": mesh.get("x"), "y": mesh.get("y"), "z": mesh.get("z"),
+         "etype": mesh.get("etype"), "esec": mesh.get("esec"), "e2vx": mesh.get("e2vx")}
+    kind = np.array([-1, 0, 0, 0, 0, 0, 1], np.int32)
+    oc = oracle_lib.OracleCase(g, kind, nsubd=4, rho=1.0, mu=mu, dt=0.01)
+    t = time.perf_counter()
+    oc.run(1, nouter)
+    secs = time.perf_counter() - t
+    return {"value": ncells * nouter / secs, "unit": "cell-iterations/s", "cores": 1,
+            "kind": "port",
+            "sample": "%d^3 lid-driven cavity, Re=%g, %d outer iterations of the oracle's C "
+                      "restatement on one host core; %.1f s" % (n, 1.0 / mu, nouter, secs)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=6)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=128, help="cells per box edge")
+    ap.add_argument("--re", type=float, default=400.0)
+    ap.add_argument("--alpha-u", type=float, default=0.7)
+    ap.add_argument("--alpha-p", type=float, default=0.3)
+    ap.add_argument("--cpu-n", type=int, default=64)
+    ap.add_argument("--cpu-iters", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sample-every", type=int, default=97)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
+              "(the multi-GPU sharding is a design in DESIGN.md, not built)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    import cfdl
+    mu = 1.0 / args.re
+    t0 = time.perf_counter()
+    mesh = cfdl.Mesh.box(args.n).partition(4)
+    eq = cfdl.UVWP.cavity(mesh, rho=1.0, mu=mu, dt=0.01, nit=100)
+    eq.set_relaxation(args.alpha_u, args.alpha_p)
+    ne = mesh.info()["ne"]
+    print("bench: setup %.1f s, %d cells" % (time.perf_counter() - t0, ne), file=sys.stderr, flush=True)
+
+    step_no = [0]
+
+    def step():
+        eq.update_boundaries()
+        eq.solve_uvwp()
+        step_no[0] += 1
+        if step_no[0] % 3 == 0:
+            eq.update_time()
+
+    for _ in range(args.warmup):
+        step()
+    eq.synchronize()
+    print("bench: warmup done", file=sys.stderr, flush=True)
+    eq.reset_stats()
+    eq.set_profile(args.sample_every)
+    eq.synchronize()
+    t = time.perf_counter()
+    for k in range(args.steps):
+        step()
+        if k == 0 or (k + 1) % 5 == 0:
+            print("bench: step %d queued" % (k + 1), file=sys.stderr, flush=True)
+    eq.synchronize()
+    secs = time.perf_counter() - t
+    eq.set_profile(0)
+    st = eq.stats()
+    log = eq.log()
+    value = ne * args.steps / secs
+    kernel_ms = st["sample_ms"]
+    cells = st["sample_cells"]
+    achieved = BYTES_PER_CELL_SWEEP * cells / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else 0.0
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "kernel": "k_gs_range (pressure-correction Gauss-Seidel level)",
+                "launch_ms": round(kernel_ms, 5), "cells_per_launch": round(cells, 1),
+                "samples": int(st["samples"])}
+    base = None if args.no_cpu else cpu_baseline(args.cpu_n, args.cpu_iters, mu)
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "cell-iterations/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * secs / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "lid-driven cavity %d^3 hex (%d cells), Re=%g, FP64 SIMPLE(C) with "
+                               "exact-order Gauss-Seidel and 4-subdomain pc solver, "
+                               "under-relaxation %g/%g" % (args.n, ne, args.re, args.alpha_u,
+                                                          args.alpha_p),
+                   "mesh": "%dx%dx%d" % (args.n, args.n, args.n), "subdomains": 4, "dt": 0.01,
+                   "outer_iterations_timed": args.steps,
+                   "scaled_residual_reached": scaled_residual(log)},
+        "roofline": roofline,
+        "cpu_baseline": base,
+    }
+    print(json.dumps(out), flush=True)
+    print("bench: pc level launches %d (%.0f cells each), %d levels per symmetric sweep; "
+          "outer iteration %.1f ms" % (st["pc_launches"], st["pc_cells"] / max(st["pc_launches"], 1),
+                                       st["pc_levels_per_sym_sweep"], 1e3 * secs / args.steps),
+          file=sys.stderr)
+
+
+if __name__ == "__main__":
+    main()

@@ -38,6 +38,7 @@ _SIGS = {
     "cfdl_ctx_create_cavity": (C.c_int, [VP, C.c_double, C.c_double, C.c_double, C.c_int,
                                          C.POINTER(VP)]),
     "cfdl_ctx_destroy": (None, [VP]),
+    "cfdl_set_relaxation": (C.c_int, [VP, C.c_double, C.c_double]),
     "cfdl_set_field": (C.c_int, [VP, C.c_int, PD, C.c_int64]),
     "cfdl_get_field": (C.c_int, [VP, C.c_int, PD, C.c_int64]),
     "cfdl_synchronize": (C.c_int, [VP]),
@@ -59,6 +60,8 @@ _SIGS = {
     "cfdl_log_get": (C.c_int, [VP, C.c_int, C.c_char_p, PI, PD, PD, PD]),
     "cfdl_log_clear": (C.c_int, [VP]),
     "cfdl_stats": (C.c_int, [VP, PD, C.c_int]),
+    "cfdl_set_profile": (C.c_int, [VP, C.c_int]),
+    "cfdl_stats_reset": (C.c_int, [VP]),
 }
 
 

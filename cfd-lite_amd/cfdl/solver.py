@@ -82,6 +82,10 @@ class UVWP:
                                   C.byref(h)))
         return cls(h, cls._sizes(ne, nf, nbf))
 
+    def set_relaxation(self, alpha_u=1.0, alpha_p=1.0):
+        """Config-mode under-relaxation (1, 1 = the reference algorithm)."""
+        check(self._lib.cfdl_set_relaxation(self._h, alpha_u, alpha_p))
+
     # ---- fields ----
     def get(self, name):
         fid = _fid(name)
@@ -161,10 +165,19 @@ class UVWP:
     def clear_log(self):
         check(self._lib.cfdl_log_clear(self._h))
 
+    def set_profile(self, sample_every):
+        """Time every sample_every-th pressure-correction sweep launch."""
+        check(self._lib.cfdl_set_profile(self._h, sample_every))
+
     def stats(self):
-        s = (C.c_double * 3)()
-        check(self._lib.cfdl_stats(self._h, s, 3))
-        return {"gs_launches": s[0], "gs_ms": s[1], "outer_iterations": s[2]}
+        s = (C.c_double * 8)()
+        check(self._lib.cfdl_stats(self._h, s, 8))
+        keys = ("gs_launches", "pc_launches", "pc_cells", "outer_iterations", "samples",
+                "sample_ms", "sample_cells", "pc_levels_per_sym_sweep")
+        return dict(zip(keys, list(s)))
+
+    def reset_stats(self):
+        check(self._lib.cfdl_stats_reset(self._h))
 
     def close(self):
         if self._h is not None:

@@ -196,7 +196,7 @@ __global__ void k_bc_update(MeshDev m, FieldsDev f) {
 // loop (:243-273) runs for the cell's own boundary faces in halo order, then
 // the Rhie-Chow / SIMPLEC coefficients (:276-284).
 // ---------------------------------------------------------------------------
-__global__ void k_coef_uvw(MeshDev m, FieldsDev f, double dt) {
+__global__ void k_coef_uvw(MeshDev m, FieldsDev f, double dt, double alpha_u) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= m.ne) return;
   double rp[3];
@@ -282,6 +282,16 @@ __global__ void k_coef_uvw(MeshDev m, FieldsDev f, double dt) {
     }
     ap = ap + d + fz;
     f.anb[k] = f.anb[k] + d + fz;
+  }
+  // config mode (no counterpart in the reference): Patankar implicit
+  // under-relaxation before the Rhie-Chow / SIMPLEC coefficients; skipped at
+  // alpha_u = 1 so that reference mode stays bit-identical.
+  if (alpha_u != 1.0) {
+    const double apr = ap / alpha_u, da = apr - ap;
+    bu = bu + da * f.u[e];
+    bv = bv + da * f.v[e];
+    bw = bw + da * f.w[e];
+    ap = apr;
   }
   f.ap[e] = ap;
   f.bu[e] = bu;
@@ -421,11 +431,12 @@ __global__ void k_coef_p(MeshDev m, FieldsDev f) {
 }
 
 // update_uvwp cell part, mod_uvwp.f90:383-392 (no velocity correction)
-__global__ void k_update_cells(int ne, FieldsDev f) {
+// (config mode scales the correction by alpha_p; alpha_p = 1 is exact)
+__global__ void k_update_cells(int ne, FieldsDev f, double alpha_p) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= ne) return;
-  f.p[e] = f.p[e] + f.phic[e];
-  for (int i = 0; i < 3; i++) f.gp[3 * e + i] = f.gp[3 * e + i] + f.gpc[3 * e + i];
+  f.p[e] = f.p[e] + alpha_p * f.phic[e];
+  for (int i = 0; i < 3; i++) f.gp[3 * e + i] = f.gp[3 * e + i] + alpha_p * f.gpc[3 * e + i];
 }
 
 // update_uvwp face part, mod_uvwp.f90:394-415 (arithmetic-mean rho)
@@ -491,6 +502,29 @@ __global__ void k_gs_level(SysDesc s, const int *__restrict__ cells, int n, doub
     for (int k = r0; k < r1; k++) sum = sum + s.anb[k] * phi[s.nbr[k]];
     phi[e] = (sum + (sor - 1.0) * a * phi[e]) / a / sor;
   }
+}
+
+// One dependency level of the pressure-correction sweeps.  The subdomain
+// systems are numbered subdomain-major and, inside a subdomain, by forward
+// level, so level l of every subdomain is one contiguous range; thread t
+// maps to range s with pre[s] <= t < pre[s+1].  Coefficient reads are
+// coalesced; only the neighbour values are gathered.
+struct LevelRanges {
+  int ns;
+  int lo[kMaxSys], pre[kMaxSys + 1];
+};
+__global__ void k_gs_range(SysDesc s, LevelRanges r, double sor, const int *__restrict__ active) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= r.pre[r.ns] || !active[0]) return;
+  int q = 0;
+  while (t >= r.pre[q + 1]) q++;
+  const int e = r.lo[q] + (t - r.pre[q]);
+  const int r0 = s.row[e], r1 = s.row[e + 1];
+  const double a = s.ap[e];
+  double *phi = s.phi[0];
+  double sum = s.b[0][e];
+  for (int k = r0; k < r1; k++) sum = sum + s.anb[k] * phi[s.nbr[k]];
+  phi[e] = (sum + (sor - 1.0) * a * phi[e]) / a / sor;
 }
 
 // Residual partial sums: block (x, y) covers cells [beg_y + x*chunk, ...).
@@ -701,6 +735,7 @@ struct cfdl_ctx {
   hipStream_t st = nullptr;
   int ne = 0, nf = 0, nbf = 0, L = 0, nsubd = 1, nit = 100;
   double dt = 0.01;
+  double alpha_u = 1.0, alpha_p = 1.0;
   DevBuf mem;
   MeshDev md{};
   FieldsDev fd{};
@@ -713,7 +748,9 @@ struct cfdl_ctx {
   int *xdst = nullptr, *xsrc = nullptr, *rowg = nullptr;
   double *ap_p = nullptr, *b_p = nullptr, *anb_p = nullptr, *phic_p = nullptr;
   std::vector<int> co;  // subdomain cell offsets (nsubd+1)
-  Sched pf, pb;
+  // lvs[s][l]: first cell of forward level l of subdomain s (nlev_s+1 each)
+  std::vector<std::vector<int>> lvs;
+  int pc_nlev = 0;
   // control
   GsCtl *gctl = nullptr;
   PcCtl *pctl = nullptr;
@@ -723,11 +760,16 @@ struct cfdl_ctx {
   LogLine *dlog = nullptr;
   int *dlog_n = nullptr;
   std::vector<LogLine> hlog;
-  // stats
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  double gs_ms = 0.0;
-  int64_t gs_launches = 0, outer = 0;
-  bool timing = true;
+  // launch counters and sampled timing of the pressure-correction sweep
+  // kernel (events around every sample_every-th level launch)
+  int64_t gs_launches = 0, pc_launches = 0, pc_cells = 0, outer = 0;
+  int sample_every = 0;
+  struct Sample {
+    hipEvent_t a, b;
+    int cells;
+  };
+  std::vector<Sample> samples;
+  size_t nsamples = 0;
 };
 
 template <typename T>
@@ -793,17 +835,76 @@ static int make_sched(cfdl_ctx *c, const std::vector<int> &lev, int n, Sched *s)
 }
 
 static int sweep(cfdl_ctx *c, const Sched &s, const SysDesc &sys, int nc, double sor,
-                 const int *active) {
+                 const int *active, bool pc = false) {
   for (int l = 0; l < s.nlev; l++) {
     const int n = s.ptr[l + 1] - s.ptr[l];
+    cfdl_ctx::Sample *smp = nullptr;
+    if (pc) {
+      if (c->sample_every > 0 && c->pc_launches % c->sample_every == 0) {
+        if (c->nsamples == c->samples.size()) {
+          cfdl_ctx::Sample x{};
+          HIPCHK(hipEventCreate(&x.a));
+          HIPCHK(hipEventCreate(&x.b));
+          c->samples.push_back(x);
+        }
+        smp = &c->samples[c->nsamples++];
+        smp->cells = n;
+        HIPCHK(hipEventRecord(smp->a, c->st));
+      }
+      c->pc_launches++;
+      c->pc_cells += n;
+    }
     if (nc == 3)
       hipLaunchKernelGGL(k_gs_level<3>, dim3(nblocks(n)), dim3(kBlock), 0, c->st, sys,
                          s.cells + s.ptr[l], n, sor, active);
     else
       hipLaunchKernelGGL(k_gs_level<1>, dim3(nblocks(n)), dim3(kBlock), 0, c->st, sys,
                          s.cells + s.ptr[l], n, sor, active);
+    if (smp) HIPCHK(hipEventRecord(smp->b, c->st));
   }
   c->gs_launches += s.nlev;
+  HIPCHK(hipGetLastError());
+  return CFDL_OK;
+}
+
+// A pressure-correction sweep: forward visits levels 0..L-1, backward the
+// same levels in reverse (a cell's neighbours on lower forward levels come
+// earlier in the sequential order, so reversing the level order is a valid
+// backward schedule).
+static int pc_sweep(cfdl_ctx *c, const SysDesc &sys, bool forward, const int *active) {
+  const int ns = c->nsubd;
+  for (int i = 0; i < c->pc_nlev; i++) {
+    const int l = forward ? i : c->pc_nlev - 1 - i;
+    LevelRanges r{};
+    r.ns = ns;
+    r.pre[0] = 0;
+    for (int q = 0; q < ns; q++) {
+      const std::vector<int> &v = c->lvs[q];
+      const int nl = static_cast<int>(v.size()) - 1;
+      const int lo = l < nl ? v[l] : 0, hi = l < nl ? v[l + 1] : 0;
+      r.lo[q] = lo;
+      r.pre[q + 1] = r.pre[q] + (hi - lo);
+    }
+    const int n = r.pre[ns];
+    if (n == 0) continue;
+    cfdl_ctx::Sample *smp = nullptr;
+    if (c->sample_every > 0 && c->pc_launches % c->sample_every == 0) {
+      if (c->nsamples == c->samples.size()) {
+        cfdl_ctx::Sample x{};
+        HIPCHK(hipEventCreate(&x.a));
+        HIPCHK(hipEventCreate(&x.b));
+        c->samples.push_back(x);
+      }
+      smp = &c->samples[c->nsamples++];
+      smp->cells = n;
+      HIPCHK(hipEventRecord(smp->a, c->st));
+    }
+    c->pc_launches++;
+    c->pc_cells += n;
+    hipLaunchKernelGGL(k_gs_range, dim3(nblocks(n)), dim3(kBlock), 0, c->st, sys, r, 1.02, active);
+    if (smp) HIPCHK(hipEventRecord(smp->b, c->st));
+    c->gs_launches++;
+  }
   HIPCHK(hipGetLastError());
   return CFDL_OK;
 }
@@ -850,23 +951,14 @@ static int solve_gs_n(cfdl_ctx *c, int ny, double **phi, const double **b,
                      static_cast<double>(c->ne), c->nit);
   // one SysDesc per sweep kernel with its RHS subset in slots 0..ny-1
   for (int it = 0; it < c->nit; it++) {
-    if (c->timing) HIPCHK(hipEventRecord(c->ev0, c->st));
     CALL(sweep(c, c->gf, s, ny, sor, gs_active(c)));
     CALL(sweep(c, c->gb, s, ny, sor, gs_active(c)));
-    if (c->timing) {
-      HIPCHK(hipEventRecord(c->ev1, c->st));
-    }
     CALL(residual(c, s, c->nblk_g, c->chunk_g, 1));
     hipLaunchKernelGGL(k_gs_iter_end, dim3(1), dim3(64), 0, c->st, c->gctl, c->red, ny,
                        static_cast<double>(c->ne), c->nit);
     HIPCHK(hipMemcpyAsync(c->host_flags, gs_active(c), sizeof(int) * 3, hipMemcpyDeviceToHost,
                           c->st));
     HIPCHK(hipStreamSynchronize(c->st));
-    if (c->timing) {
-      float ms = 0.f;
-      HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-      c->gs_ms += ms;
-    }
     bool any = false;
     for (int y = 0; y < ny; y++) any = any || c->host_flags[y];
     if (!any) break;
@@ -906,11 +998,10 @@ static int multi_subdomain_solver(cfdl_ctx *c) {
   s1.ny = 1;
   const int *act = pc_active(c);
   const int nblock = (c->nit + 1) / 2;
-  if (c->timing) HIPCHK(hipEventRecord(c->ev0, c->st));
   for (int blk = 1; blk <= nblock; blk++) {
     for (int rep = 0; rep < 2; rep++) {  // smoother_gs(..., 2): two symmetric iterations
-      CALL(sweep(c, c->pf, s1, 1, 1.02, act));
-      CALL(sweep(c, c->pb, s1, 1, 1.02, act));
+      CALL(pc_sweep(c, s1, true, act));
+      CALL(pc_sweep(c, s1, false, act));
     }
     if (c->nx > 0)
       hipLaunchKernelGGL(k_exchange, dim3(nblocks(c->nx)), dim3(kBlock), 0, c->st, c->nx, c->xdst,
@@ -919,13 +1010,6 @@ static int multi_subdomain_solver(cfdl_ctx *c) {
     if (has_res) CALL(residual(c, s, c->nblk_p, c->chunk_p, 0));
     hipLaunchKernelGGL(k_pc_block_end, dim3(1), dim3(64), 0, c->st, c->pctl, c->red, sc, c->nit,
                        has_res);
-  }
-  if (c->timing) {
-    HIPCHK(hipEventRecord(c->ev1, c->st));
-    HIPCHK(hipEventSynchronize(c->ev1));
-    float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    c->gs_ms += ms;
   }
   // update_phi, mod_subdomains.f90:214-229
   hipLaunchKernelGGL(k_scatter, dim3(nblocks(ne)), dim3(kBlock), 0, c->st, ne, c->l2g_cell,
@@ -1118,19 +1202,22 @@ int cfdl_ctx_create(int ne, int nf, int nbf, const int *ef2nb_idx, const int *ef
         sd_of[g] = s;
         l2g[q] = g;
       }
-    std::vector<int> rowp(ne + 1, 0), nbrp, halo_src, xdst, xsrc;
-    nbrp.reserve(L);
-    for (int q = 0; q < ne; q++) rowp[q + 1] = rowp[q] + (row[l2g[q] + 1] - row[l2g[q]]);
+    // subdomain systems in g2gf order first (old index q = position in
+    // g2gf), halo slots numbered in encounter order; then each subdomain's
+    // cells are renumbered by forward dependency level
+    std::vector<int> rowo(ne + 1, 0), nbro, halo_src, xdst, xsrc;
+    nbro.reserve(L);
+    for (int q = 0; q < ne; q++) rowo[q + 1] = rowo[q] + (row[l2g[q] + 1] - row[l2g[q]]);
     int nh = 0;
     for (int q = 0; q < ne; q++) {
       const int g = l2g[q], s = sd_of[g];
       for (int k = row[g]; k < row[g + 1]; k++) {
         const int nb = nbr[k];
         if (nb < ne && sd_of[nb] == s) {
-          nbrp.push_back(cid_of[nb]);
+          nbro.push_back(cid_of[nb]);
         } else {
           const int slot = ne + nh++;
-          nbrp.push_back(slot);
+          nbro.push_back(slot);
           halo_src.push_back(nb);
           if (nb < ne) {
             xdst.push_back(slot);
@@ -1139,12 +1226,43 @@ int cfdl_ctx_create(int ne, int nf, int nbf, const int *ef2nb_idx, const int *ef
         }
       }
     }
+    std::vector<int> lev(ne), o2n(ne), n2o(ne);
+    C->lvs.assign(nsubd, std::vector<int>());
+    C->pc_nlev = 0;
+    for (int s = 0; s < nsubd; s++) {
+      levels(rowo, nbro, C->co[s], C->co[s + 1], true, lev);
+      int nl = 0;
+      for (int q = C->co[s]; q < C->co[s + 1]; q++) nl = std::max(nl, lev[q]);
+      std::vector<int> &st = C->lvs[s];
+      st.assign(nl + 1, 0);
+      for (int q = C->co[s]; q < C->co[s + 1]; q++) st[lev[q]]++;
+      st[0] = C->co[s];
+      for (int l = 1; l <= nl; l++) st[l] += st[l - 1];
+      std::vector<int> pos(st.begin(), st.end() - 1);
+      for (int q = C->co[s]; q < C->co[s + 1]; q++) {
+        o2n[q] = pos[lev[q] - 1]++;
+        n2o[o2n[q]] = q;
+      }
+      C->pc_nlev = std::max(C->pc_nlev, nl);
+    }
+    std::vector<int> rowp(ne + 1, 0), nbrp, l2gn(ne);
+    nbrp.reserve(nbro.size());
+    for (int nq = 0; nq < ne; nq++) {
+      const int q = n2o[nq];
+      l2gn[nq] = l2g[q];
+      rowp[nq + 1] = rowp[nq] + (rowo[q + 1] - rowo[q]);
+      for (int k = rowo[q]; k < rowo[q + 1]; k++) {
+        const int v = nbro[k];
+        nbrp.push_back(v < ne ? o2n[v] : v);
+      }
+    }
+    for (auto &v : xsrc) v = o2n[v];
     C->Lp = static_cast<int>(nbrp.size());
     C->nh = nh;
     C->nx = static_cast<int>(xdst.size());
     CALL(upload(C, &C->rowp, rowp));
     CALL(upload(C, &C->nbrp, nbrp));
-    CALL(upload(C, &C->l2g_cell, l2g));
+    CALL(upload(C, &C->l2g_cell, l2gn));
     CALL(upload(C, &C->halo_src, halo_src));
     CALL(upload(C, &C->xdst, xdst));
     CALL(upload(C, &C->xsrc, xsrc));
@@ -1152,11 +1270,6 @@ int cfdl_ctx_create(int ne, int nf, int nbf, const int *ef2nb_idx, const int *ef
     CALL(dalloc(C, &C->b_p, ne));
     CALL(dalloc(C, &C->anb_p, C->Lp));
     CALL(dalloc(C, &C->phic_p, int64_t{ne} + nh));
-    std::vector<int> lev(ne);
-    for (int s = 0; s < nsubd; s++) levels(rowp, nbrp, C->co[s], C->co[s + 1], true, lev);
-    CALL(make_sched(C, lev, ne, &C->pf));
-    for (int s = 0; s < nsubd; s++) levels(rowp, nbrp, C->co[s], C->co[s + 1], false, lev);
-    CALL(make_sched(C, lev, ne, &C->pb));
     int maxsd = 0;
     for (int s = 0; s < nsubd; s++) maxsd = std::max(maxsd, C->co[s + 1] - C->co[s]);
     C->chunk_p = std::max(kBlock * 8, (maxsd + 1023) / 1024);
@@ -1178,8 +1291,6 @@ int cfdl_ctx_create(int ne, int nf, int nbf, const int *ef2nb_idx, const int *ef
     HIPCHK(hipMemcpy(C->dlog_n, &zero, sizeof(int), hipMemcpyHostToDevice));
   }
   HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&C->host_flags), 64, hipHostMallocDefault));
-  HIPCHK(hipEventCreate(&C->ev0));
-  HIPCHK(hipEventCreate(&C->ev1));
   HIPCHK(hipStreamSynchronize(C->st));
   trace("ctx_create: done");
   *out = c.release();
@@ -1190,8 +1301,10 @@ void cfdl_ctx_destroy(cfdl_ctx *c) {
   if (!c) return;
   (void)hipStreamSynchronize(c->st);
   if (c->host_flags) (void)hipHostFree(c->host_flags);
-  if (c->ev0) (void)hipEventDestroy(c->ev0);
-  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  for (auto &x : c->samples) {
+    (void)hipEventDestroy(x.a);
+    (void)hipEventDestroy(x.b);
+  }
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
 }
@@ -1211,6 +1324,17 @@ int cfdl_ctx_create_cavity(const cfdl_mesh *m, double rho, double mu, double dt,
                          m->vol.data(), m->aip.data(), m->rip.data(), m->nsec, m->esec.data(),
                          kind.data(), m->nsubd, m->gf2g.data(), m->g2gf.data(),
                          m->g2gf_idx.data(), r.data(), u.data(), dt, nit, out);
+}
+
+int cfdl_set_relaxation(cfdl_ctx *c, double alpha_u, double alpha_p) {
+  CALL(check_ctx(c));
+  if (!(alpha_u > 0.0 && alpha_u <= 1.0 && alpha_p > 0.0 && alpha_p <= 1.0)) {
+    set_error("cfdl_set_relaxation: factors must lie in (0, 1]");
+    return CFDL_EINVAL;
+  }
+  c->alpha_u = alpha_u;
+  c->alpha_p = alpha_p;
+  return CFDL_OK;
 }
 
 int cfdl_set_field(cfdl_ctx *c, int field, const double *src, int64_t n) {
@@ -1253,7 +1377,8 @@ int cfdl_update_boundaries(cfdl_ctx *c) {
 
 int cfdl_calc_coef_uvw(cfdl_ctx *c) {
   CALL(check_ctx(c));
-  hipLaunchKernelGGL(k_coef_uvw, dim3(nblocks(c->ne)), dim3(kBlock), 0, c->st, c->md, c->fd, c->dt);
+  hipLaunchKernelGGL(k_coef_uvw, dim3(nblocks(c->ne)), dim3(kBlock), 0, c->st, c->md, c->fd, c->dt,
+                     c->alpha_u);
   HIPCHK(hipGetLastError());
   return CFDL_OK;
 }
@@ -1343,7 +1468,8 @@ int cfdl_adjust_pc(cfdl_ctx *c) {
 
 int cfdl_update_uvwp(cfdl_ctx *c) {
   CALL(check_ctx(c));
-  hipLaunchKernelGGL(k_update_cells, dim3(nblocks(c->ne)), dim3(kBlock), 0, c->st, c->ne, c->fd);
+  hipLaunchKernelGGL(k_update_cells, dim3(nblocks(c->ne)), dim3(kBlock), 0, c->st, c->ne, c->fd,
+                     c->alpha_p);
   hipLaunchKernelGGL(k_update_faces, dim3(nblocks(c->nf)), dim3(kBlock), 0, c->st, c->md, c->fd);
   HIPCHK(hipGetLastError());
   return CFDL_OK;
@@ -1435,10 +1561,40 @@ int cfdl_log_clear(cfdl_ctx *c) {
   return CFDL_OK;
 }
 
+int cfdl_set_profile(cfdl_ctx *c, int sample_every) {
+  CALL(check_ctx(c));
+  if (sample_every < 0) {
+    set_error("cfdl_set_profile: sample_every must be >= 0");
+    return CFDL_EINVAL;
+  }
+  c->sample_every = sample_every;
+  return CFDL_OK;
+}
+
 int cfdl_stats(cfdl_ctx *c, double *stats, int n) {
   CALL(check_ctx(c));
-  const double v[3] = {static_cast<double>(c->gs_launches), c->gs_ms, static_cast<double>(c->outer)};
-  for (int i = 0; i < n && i < 3; i++) stats[i] = v[i];
+  HIPCHK(hipStreamSynchronize(c->st));
+  double ms = 0.0, cells = 0.0;
+  for (size_t k = 0; k < c->nsamples; k++) {
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, c->samples[k].a, c->samples[k].b));
+    ms += t;
+    cells += c->samples[k].cells;
+  }
+  const double ns = static_cast<double>(c->nsamples);
+  const double v[8] = {static_cast<double>(c->gs_launches), static_cast<double>(c->pc_launches),
+                       static_cast<double>(c->pc_cells), static_cast<double>(c->outer), ns,
+                       ns > 0 ? ms / ns : 0.0, ns > 0 ? cells / ns : 0.0,
+                       static_cast<double>(2 * c->pc_nlev)};
+  for (int i = 0; i < n && i < 8; i++) stats[i] = v[i];
+  return CFDL_OK;
+}
+
+int cfdl_stats_reset(cfdl_ctx *c) {
+  CALL(check_ctx(c));
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->gs_launches = c->pc_launches = c->pc_cells = c->outer = 0;
+  c->nsamples = 0;
   return CFDL_OK;
 }
 

@@ -123,6 +123,13 @@ int cfdl_ctx_create_cavity(const cfdl_mesh *m, double rho, double mu, double dt,
                            cfdl_ctx **out);
 void cfdl_ctx_destroy(cfdl_ctx *c);
 
+/* Config mode (no counterpart in the reference, which relaxes only through
+   its pseudo-time step dt): Patankar implicit under-relaxation of the
+   momentum equations, ap -> ap/alpha_u and b += (ap/alpha_u - ap)*phi before
+   d and dc are formed, and p += alpha_p*pc, gp += alpha_p*gpc.  Both 1 (the
+   default) is the reference algorithm bit for bit. */
+int cfdl_set_relaxation(cfdl_ctx *c, double alpha_u, double alpha_p);
+
 /* host <-> device copies of one field (n = element count of that field) */
 int cfdl_set_field(cfdl_ctx *c, int field, const double *src, int64_t n);
 int cfdl_get_field(cfdl_ctx *c, int field, double *dst, int64_t n);
@@ -159,10 +166,17 @@ int cfdl_log_get(cfdl_ctx *c, int k, char name[8], int *it, double *res_i, doubl
                  double *res_max);
 int cfdl_log_clear(cfdl_ctx *c);
 
-/* Timing of the last cfdl_run (device time of the dominant kernel class):
-   stats[0] = GS level launches, stats[1] = total GS kernel milliseconds
-   (HIP events on the context stream), stats[2] = outer iterations. */
+/* Launch counters and sampled kernel timing (no counterpart in the
+   reference, which has no timers).  cfdl_set_profile(c, k) brackets every
+   k-th level launch of the pressure-correction sweeps with HIP events on the
+   context stream (0 = off).  cfdl_stats synchronises and fills up to 8 values:
+   [0] Gauss-Seidel level launches (all solves), [1] pressure-correction level
+   launches, [2] cells updated by those launches, [3] outer iterations,
+   [4] sampled launches, [5] mean sampled launch duration (ms), [6] mean cells
+   per sampled launch, [7] levels per forward + backward pc sweep. */
+int cfdl_set_profile(cfdl_ctx *c, int sample_every);
 int cfdl_stats(cfdl_ctx *c, double *stats, int n);
+int cfdl_stats_reset(cfdl_ctx *c);
 
 #ifdef __cplusplus
 }

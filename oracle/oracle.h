@@ -69,6 +69,7 @@ typedef struct or_case {
   /* physics */
   double dt;
   int nit;
+  double alpha_u, alpha_p; /* config-mode under-relaxation (1 = reference) */
   double *rho, *mu;
   /* equation state (mod_uvwp.f90:6-16) */
   double *u, *v, *w, *p, *u0, *v0, *w0;

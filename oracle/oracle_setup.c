@@ -662,6 +662,7 @@ void or_construct_subdomains(or_case *c) {
 void or_init_physics(or_case *c, double rho, double mu, double dt, int nit) {
   int ne = c->ne, n = c->ne + c->nbf, nf = c->nf;
   c->dt = dt; c->nit = nit;
+  c->alpha_u = 1.0; c->alpha_p = 1.0;
   c->rho = xcalloc(ne, 8); c->mu = xcalloc(ne, 8);
   for (int e = 0; e < ne; e++) { c->rho[e] = rho; c->mu[e] = mu; }
   double **f1[] = {&c->u, &c->v, &c->w, &c->p, &c->u0, &c->v0, &c->w0, &c->phic};

@@ -403,6 +403,19 @@ void or_calc_coef_uvw(or_case *c) {
     c->ap[e - 1] = c->ap[e - 1] + d + f;
     c->anb[idx - 1] = c->anb[idx - 1] + d + f;
   }
+  /* config mode (no counterpart in the reference): Patankar implicit
+     under-relaxation, ap -> ap/alpha_u and b += (ap/alpha_u - ap)*phi, applied
+     before the Rhie-Chow / SIMPLEC coefficients; alpha_u = 1 is skipped so
+     that reference mode stays bit-identical. */
+  if (c->alpha_u != 1.0) {
+    for (int e = 1; e <= ne; e++) {
+      double apr = c->ap[e - 1] / c->alpha_u, da = apr - c->ap[e - 1];
+      c->bu[e - 1] = c->bu[e - 1] + da * c->u[e - 1];
+      c->bv[e - 1] = c->bv[e - 1] + da * c->v[e - 1];
+      c->bw[e - 1] = c->bw[e - 1] + da * c->w[e - 1];
+      c->ap[e - 1] = apr;
+    }
+  }
   for (int e = 1; e <= ne; e++) {
     c->d[e - 1] = c->vol[e - 1] / c->ap[e - 1];
     double dc = c->ap[e - 1];
@@ -500,9 +513,12 @@ void or_adjust_pc(or_case *c, double *pc, double pref) {
 /* update_uvwp, src/equations/mod_uvwp.f90:370-436 (no velocity correction) */
 void or_update_uvwp(or_case *c, const double *pc) {
   const int *g1 = c->ef2nb;
+  /* config mode: p and gp take alpha_p * correction (alpha_p = 1 multiplies
+     by one exactly, i.e. the reference's p+pc, gp+gpc) */
   for (int e = 0; e < c->ne; e++) {
-    c->p[e] = c->p[e] + pc[e];
-    for (int i = 0; i < 3; i++) c->gp[3 * e + i] = c->gp[3 * e + i] + c->gpc[3 * e + i];
+    c->p[e] = c->p[e] + c->alpha_p * pc[e];
+    for (int i = 0; i < 3; i++)
+      c->gp[3 * e + i] = c->gp[3 * e + i] + c->alpha_p * c->gpc[3 * e + i];
   }
   for (int fg = 1; fg <= c->nf; fg++) {
     int e = PK_E(c->s2g[fg - 1]), lf = PK_F(c->s2g[fg - 1]);

@@ -40,7 +40,8 @@ class OrCase(C.Structure):
                 ("xc", PD), ("yc", PD), ("zc", PD), ("vol", PD), ("aip", PD), ("rip", PD),
                 ("nsubd", C.c_int), ("gf2g", PI), ("g2gf", PI), ("g2gf_idx", PI),
                 ("sd", C.POINTER(OrSubd)), ("intf", C.POINTER(OrIntf)),
-                ("dt", C.c_double), ("nit", C.c_int), ("rho", PD), ("mu", PD),
+                ("dt", C.c_double), ("nit", C.c_int), ("alpha_u", C.c_double),
+                ("alpha_p", C.c_double), ("rho", PD), ("mu", PD),
                 ("u", PD), ("v", PD), ("w", PD), ("p", PD), ("u0", PD), ("v0", PD), ("w0", PD),
                 ("gu", PD), ("gv", PD), ("gw", PD), ("gp", PD), ("gpc", PD),
                 ("mip", PD), ("mip0", PD),
@@ -148,6 +149,10 @@ class OracleCase:
         c = self.c
         return [(c.log[k].name.decode(), c.log[k].it, c.log[k].res_i, c.log[k].res_f,
                  c.log[k].res_max) for k in range(c.nlog)]
+
+    def set_relaxation(self, alpha_u=1.0, alpha_p=1.0):
+        self.c.alpha_u = alpha_u
+        self.c.alpha_p = alpha_p
 
     def run(self, ntstep, ncoef):
         lib().or_run(self.p, ntstep, ncoef)

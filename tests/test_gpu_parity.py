@@ -57,3 +57,101 @@ def test_run_bitwise(n, jitter, nsubd, ntstep, ncoef):
     oc.run(ntstep, ncoef)
     assert_fields_equal(eq, oc)
     assert_logs_match(eq.log(), oc.log())
+
+
+def test_relaxation_config_mode_bitwise():
+    """Config mode (alpha_u=0.7, alpha_p=0.3) against the oracle's same
+    extension; the reference has no relaxation, so this pins the GPU path to
+    the restatement, and alpha=1 pins the restatement to the reference."""
+    eq, oc = make_pair(8, 0.1, 4, mu=0.0025)
+    eq.set_relaxation(0.7, 0.3)
+    oc.set_relaxation(0.7, 0.3)
+    eq.run(2, 3)
+    oc.run(2, 3)
+    assert_fields_equal(eq, oc)
+    assert_logs_match(eq.log(), oc.log())
+
+
+def test_32cube_one_step():
+    eq, oc = make_pair(32, 0.0, 4, mu=0.0025)
+    eq.run(1, 2)
+    oc.run(1, 2)
+    assert_fields_equal(eq, oc, ["u", "v", "w", "p", "mip", "gp", "phic"])
+    assert_logs_match(eq.log(), oc.log())
+
+
+def _ocall(oc, fn, *args):
+    getattr(oracle_lib.lib(), fn)(oc.p, *args)
+
+
+def test_routines_one_by_one():
+    """Each C-ABI routine against the matching oracle routine, from a state
+    reached after two outer iterations (non-trivial fields)."""
+    eq, oc = make_pair(7, 0.2, 4)
+    eq.run(1, 2)
+    oc.run(1, 2)
+    C = __import__("ctypes")
+    eq.update_boundaries()
+    _ocall(oc, "or_update_boundaries")
+    assert_fields_equal(eq, oc, ["u", "v", "w", "p", "mip"])
+    eq.calc_coef_uvw()
+    _ocall(oc, "or_calc_coef_uvw")
+    assert_fields_equal(eq, oc, ["ap", "anb", "bu", "bv", "bw", "d", "dc"])
+    eq.solve_gs("u", "u", "bu")
+    L = oracle_lib.lib()
+    c = oc.c
+    L.or_solve_gs.argtypes = [C.POINTER(oracle_lib.OrCase), C.c_char_p, oracle_lib.PD, oracle_lib.PD,
+                              oracle_lib.PD, oracle_lib.PD, oracle_lib.PI, oracle_lib.PI,
+                              C.c_int, C.c_int, C.c_int, C.c_int]
+    L.or_solve_gs(oc.p, b"u", c.u, c.ap, c.anb, c.bu, c.ef2nb_idx, c.ef2nb, c.ne, c.nf, c.nbf, c.nit)
+    assert_fields_equal(eq, oc, ["u"])
+    eq.calc_grad("u", "gu")
+    L.or_calc_grad(c.u, c.gu, c.xc, c.yc, c.zc, c.ef2nb_idx, c.ef2nb, c.ne, c.nf, c.nbf)
+    assert_fields_equal(eq, oc, ["gu"])
+    eq.calc_mip(True)
+    _ocall(oc, "or_calc_mip", 1)
+    assert_fields_equal(eq, oc, ["mip"])
+    eq.calc_coef_p()
+    _ocall(oc, "or_calc_coef_p")
+    assert_fields_equal(eq, oc, ["ap", "anb", "b"])
+    eq.set_a_0("phic")
+    oc.arr("phic")[:] = 0.0
+    eq.solve("pc")
+    L.or_solve(oc.p, b"pc", c.phic)
+    assert_fields_equal(eq, oc, ["phic"])
+    eq.adjust_pc()
+    L.or_adjust_pc(oc.p, c.phic, oc.arr("phic")[0])
+    assert_fields_equal(eq, oc, ["phic"])
+    eq.calc_grad("phic", "gpc")
+    L.or_calc_grad(c.phic, c.gpc, c.xc, c.yc, c.zc, c.ef2nb_idx, c.ef2nb, c.ne, c.nf, c.nbf)
+    assert_fields_equal(eq, oc, ["gpc"])
+    eq.update_uvwp()
+    L.or_update_uvwp(oc.p, c.phic)
+    assert_fields_equal(eq, oc, ["p", "gp", "mip"])
+    eq.update_time()
+    _ocall(oc, "or_update_time")
+    assert_fields_equal(eq, oc, ["u0", "v0", "w0", "mip0"])
+    assert_logs_match(eq.log(), oc.log())
+
+
+def test_repeatable():
+    """Two runs of the same case give identical bits (fixed-order reductions,
+    no atomics)."""
+    a, _ = make_pair(12, 0.1, 4)
+    b, _ = make_pair(12, 0.1, 4)
+    a.run(1, 2)
+    b.run(1, 2)
+    for name in ("u", "p", "mip"):
+        assert np.array_equal(a.get(name), b.get(name))
+    assert a.log() == b.log()
+
+
+def test_set_get_roundtrip_and_errors():
+    eq, _ = make_pair(4)
+    x = np.arange(eq.sizes["u"], dtype=np.float64)
+    eq.set("u", x)
+    assert np.array_equal(eq.get("u"), x)
+    with pytest.raises(cfdl.CfdlError):
+        eq.set("u", x[:-1])
+    with pytest.raises(cfdl.CfdlError):
+        eq.set_relaxation(0.0, 1.0)

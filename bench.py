@@ -11,7 +11,7 @@ solve_uvwp; update_time after every third).  Inputs are resident in HBM when
 the timed region starts.
 
 Prints one JSON line.  value = 2,097,152 * K / seconds for K timed steps.
-roofline: the pressure-correction Gauss-Seidel level kernel k_gs_range (the dominant
+roofline: the pressure-correction Gauss-Seidel sweep kernel k_gs_tiles_lds (the dominant
 kernel), 108 algorithmic bytes per cell update (FP64 CSR sweep: b, ap, 6
 anb, 6 neighbour indices, neighbour/own phi read, phi write) times the cells
 of one launch, over the mean launch duration from HIP events on the context
@@ -169,7 +169,7 @@ def main():
     achieved = BYTES_PER_CELL_SWEEP * cells / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                "kernel": "k_gs_range (pressure-correction Gauss-Seidel level)",
+                "kernel": "k_gs_tiles_lds<true> (pressure-correction Gauss-Seidel, space-time tiles in LDS)",
                 "launch_ms": round(kernel_ms, 5), "cells_per_launch": round(cells, 1),
                 "samples": int(st["samples"])}
     base = None if args.no_cpu else cpu_baseline(args.cpu_n, args.cpu_iters, mu)

@@ -33,6 +33,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "cfdl_internal.h"
@@ -527,6 +528,166 @@ __global__ void k_gs_range(SysDesc s, LevelRanges r, double sor, const int *__re
   phi[e] = (sum + (sor - 1.0) * a * phi[e]) / a / sor;
 }
 
+// Space-time tiled sweep: one workgroup per (launch, tile) batch walks the
+// batch's dependency-level segments in order, __syncthreads between them.
+// The host schedule puts every dependency that crosses tiles into a later
+// launch, so each workgroup only waits for its own earlier segments; the
+// sequential order of every cell's update is unchanged.
+constexpr int kTileThreads = 64;
+__global__ void __launch_bounds__(kTileThreads)
+k_gs_tiles(SysDesc s, const int *__restrict__ bseg, const int *__restrict__ seg,
+           const int *__restrict__ cells, int batch0, double sor,
+           const int *__restrict__ active) {
+  if (!active[0]) return;
+  const int b = batch0 + blockIdx.x;
+  const int q0 = bseg[b], q1 = bseg[b + 1];
+  double *phi = s.phi[0];
+  const double *rhs = s.b[0];
+  for (int q = q0; q < q1; q++) {
+    const int c1 = seg[q + 1];
+    for (int t = seg[q] + threadIdx.x; t < c1; t += kTileThreads) {
+      const int e = cells[t];
+      const int r0 = s.row[e], r1 = s.row[e + 1];
+      const double a = s.ap[e];
+      double sum = rhs[e];
+      for (int k = r0; k < r1; k++) sum = sum + s.anb[k] * phi[s.nbr[k]];
+      phi[e] = (sum + (sor - 1.0) * a * phi[e]) / a / sor;
+    }
+    __syncthreads();
+  }
+}
+
+// The same schedule with the batch resident in LDS: the workgroup gathers
+// its cells' coefficients and current values plus the values of every
+// outside neighbour (constant during the launch: a neighbour in another tile
+// is never updated in the same launch) into LDS, walks the level segments
+// on LDS only, and writes its cells back.  Per batch: cells [c0, c0+nc) of
+// the schedule's cell list, CSR entries [brow[c0], brow[c0+nc)) with local
+// slot indices lnbr (< nc: a batch cell, else outside slot nc + j whose
+// global index is xslot[xptr[b] + j]).
+constexpr int kLdsThreads = 64;  // one wave: segments average about a dozen cells
+template <bool HEX>
+__global__ void __launch_bounds__(kLdsThreads)
+k_gs_tiles_lds(SysDesc s, const int *__restrict__ bseg, const int *__restrict__ seg,
+               const int *__restrict__ cells, const int *__restrict__ brow,
+               const int *__restrict__ growp, const unsigned short *__restrict__ lnbr,
+               const int *__restrict__ xptr, const int *__restrict__ xslot, int batch0,
+               double sor, const int *__restrict__ active) {
+  extern __shared__ double lds[];
+  if (!active[0]) return;
+  const int b = batch0 + blockIdx.x;
+  const int q0 = bseg[b], q1 = bseg[b + 1], nq = q1 - q0;
+  const int c0 = seg[q0], nc = seg[q1] - c0;
+  const int e0 = brow[c0], ne_b = brow[c0 + nc] - e0;
+  const int x0 = xptr[b], nx = xptr[b + 1] - x0;
+  double *lphi = lds;              // nc + nx values
+  double *lap = lphi + nc + nx;    // nc
+  double *lb = lap + nc;           // nc
+  double *lanb = lb + nc;          // ne_b
+  int *lrow = reinterpret_cast<int *>(lanb + ne_b);  // nc + 1 local row starts
+  int *lseg = lrow + nc + 1;                         // nq + 1 local segment starts
+  int *lcell = lseg + nq + 1;                        // nc cell indices
+  unsigned short *lnb = reinterpret_cast<unsigned short *>(lcell + nc);  // ne_b slots
+  double *phi = s.phi[0];
+  const double *rhs = s.b[0];
+  // gather: every load of a round is issued before the first LDS store so
+  // that the round costs two memory latencies (index, then value)
+  constexpr int U = 8;
+  for (int base = 0; base < nc; base += U * kLdsThreads) {
+    int ee[U], gg[U], oo[U], nn[U];
+    unsigned short sl[U][6];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int i = base + u * kLdsThreads + threadIdx.x;
+      const bool ok = i < nc;
+      ee[u] = ok ? cells[c0 + i] : 0;
+      gg[u] = ok ? growp[c0 + i] : 0;
+      oo[u] = ok ? brow[c0 + i] - e0 : 0;
+      nn[u] = ok ? brow[c0 + i + 1] - brow[c0 + i] : 0;
+      if (HEX) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) sl[u][k] = ok ? lnbr[e0 + 6 * i + k] : 0;
+      }
+    }
+    double vp[U], va[U], vb[U], vn[U][6];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      vp[u] = phi[ee[u]];
+      va[u] = s.ap[ee[u]];
+      vb[u] = rhs[ee[u]];
+      if (HEX) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) vn[u][k] = s.anb[gg[u] + k];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int i = base + u * kLdsThreads + threadIdx.x;
+      if (i < nc) {
+        lphi[i] = vp[u];
+        lap[i] = va[u];
+        lb[i] = vb[u];
+        lrow[i] = oo[u];
+        lcell[i] = ee[u];
+        if (HEX) {
+#pragma unroll
+          for (int k = 0; k < 6; k++) {
+            lanb[oo[u] + k] = vn[u][k];
+            lnb[oo[u] + k] = sl[u][k];
+          }
+        } else {
+          for (int k = 0; k < nn[u]; k++) {
+            lanb[oo[u] + k] = s.anb[gg[u] + k];
+            lnb[oo[u] + k] = lnbr[e0 + oo[u] + k];
+          }
+        }
+      }
+    }
+  }
+  if (threadIdx.x == 0) lrow[nc] = ne_b;
+  for (int base = 0; base < nx; base += U * kLdsThreads) {
+    int xs[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int j = base + u * kLdsThreads + threadIdx.x;
+      xs[u] = j < nx ? xslot[x0 + j] : 0;
+    }
+    double vx[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) vx[u] = phi[xs[u]];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int j = base + u * kLdsThreads + threadIdx.x;
+      if (j < nx) lphi[nc + j] = vx[u];
+    }
+  }
+  for (int q = threadIdx.x; q <= nq; q += kLdsThreads) lseg[q] = seg[q0 + q] - c0;
+  __syncthreads();
+  for (int q = 0; q < nq; q++) {
+    const int i1 = lseg[q + 1];
+    for (int i = lseg[q] + threadIdx.x; i < i1; i += kLdsThreads) {
+      const double a = lap[i];
+      double sum = lb[i];
+      if (HEX) {  // six faces: gather all slots first, then sum in row order
+        const int k0 = 6 * i;
+        int sl[6];
+        double v[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) sl[k] = lnb[k0 + k];
+#pragma unroll
+        for (int k = 0; k < 6; k++) v[k] = lphi[sl[k]];
+#pragma unroll
+        for (int k = 0; k < 6; k++) sum = sum + lanb[k0 + k] * v[k];
+      } else {
+        for (int k = lrow[i]; k < lrow[i + 1]; k++) sum = sum + lanb[k] * lphi[lnb[k]];
+      }
+      lphi[i] = (sum + (sor - 1.0) * a * lphi[i]) / a / sor;
+    }
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < nc; i += kLdsThreads) phi[lcell[i]] = lphi[i];
+}
+
 // Residual partial sums: block (x, y) covers cells [beg_y + x*chunk, ...).
 // absmax: |r| and max|r| (solve_gs, :309-321); else signed r and max(r,0)
 // (calc_residual, :239-251).  Fixed-order tree, so reproducible.
@@ -751,6 +912,18 @@ struct cfdl_ctx {
   // lvs[s][l]: first cell of forward level l of subdomain s (nlev_s+1 each)
   std::vector<std::vector<int>> lvs;
   int pc_nlev = 0;
+  // space-time tiled schedules of the pc sweeps (forward, backward)
+  struct TileSched {
+    std::vector<int> launch_ptr;  // batches of launch m: [launch_ptr[m], launch_ptr[m+1])
+    int *bseg = nullptr, *seg = nullptr, *cells = nullptr;
+    int *brow = nullptr, *growp = nullptr, *xptr = nullptr, *xslot = nullptr;
+    unsigned short *lnbr = nullptr;
+    bool hex = false;  // every row has six entries
+    int nbatch = 0, nseg = 0;
+    size_t lds_bytes = 0;  // 0: batches too large for LDS, global-memory kernel
+  } tf, tb;
+  std::vector<int> launch_cells[2];  // cells updated by each launch (fwd, bwd)
+  bool tiled = true;
   // control
   GsCtl *gctl = nullptr;
   PcCtl *pctl = nullptr;
@@ -909,6 +1082,46 @@ static int pc_sweep(cfdl_ctx *c, const SysDesc &sys, bool forward, const int *ac
   return CFDL_OK;
 }
 
+static int pc_sweep_tiled(cfdl_ctx *c, const SysDesc &sys, bool forward, const int *active) {
+  const cfdl_ctx::TileSched &t = forward ? c->tf : c->tb;
+  const int nl = static_cast<int>(t.launch_ptr.size()) - 1;
+  for (int m = 0; m < nl; m++) {
+    const int b0 = t.launch_ptr[m], nb = t.launch_ptr[m + 1] - b0;
+    cfdl_ctx::Sample *smp = nullptr;
+    if (c->sample_every > 0 && c->pc_launches % c->sample_every == 0) {
+      if (c->nsamples == c->samples.size()) {
+        cfdl_ctx::Sample x{};
+        HIPCHK(hipEventCreate(&x.a));
+        HIPCHK(hipEventCreate(&x.b));
+        c->samples.push_back(x);
+      }
+      smp = &c->samples[c->nsamples++];
+      smp->cells = -1;  // filled from the host copy of the schedule below
+      HIPCHK(hipEventRecord(smp->a, c->st));
+    }
+    c->pc_launches++;
+    if (t.lds_bytes > 0 && t.hex)
+      hipLaunchKernelGGL(k_gs_tiles_lds<true>, dim3(nb), dim3(kLdsThreads), t.lds_bytes, c->st,
+                         sys, t.bseg, t.seg, t.cells, t.brow, t.growp, t.lnbr, t.xptr, t.xslot, b0,
+                         1.02, active);
+    else if (t.lds_bytes > 0)
+      hipLaunchKernelGGL(k_gs_tiles_lds<false>, dim3(nb), dim3(kLdsThreads), t.lds_bytes, c->st,
+                         sys, t.bseg, t.seg, t.cells, t.brow, t.growp, t.lnbr, t.xptr, t.xslot, b0,
+                         1.02, active);
+    else
+      hipLaunchKernelGGL(k_gs_tiles, dim3(nb), dim3(kTileThreads), 0, c->st, sys, t.bseg, t.seg,
+                         t.cells, b0, 1.02, active);
+    if (smp) {
+      HIPCHK(hipEventRecord(smp->b, c->st));
+      smp->cells = c->launch_cells[forward ? 0 : 1][m];
+    }
+    c->pc_cells += c->launch_cells[forward ? 0 : 1][m];
+    c->gs_launches++;
+  }
+  HIPCHK(hipGetLastError());
+  return CFDL_OK;
+}
+
 static int residual(cfdl_ctx *c, const SysDesc &sys, int nblk, int chunk, int absmax) {
   hipLaunchKernelGGL(k_resid_partial, dim3(nblk, sys.ny), dim3(kBlock), 0, c->st, sys, chunk,
                      nblk, absmax, c->part);
@@ -1000,8 +1213,13 @@ static int multi_subdomain_solver(cfdl_ctx *c) {
   const int nblock = (c->nit + 1) / 2;
   for (int blk = 1; blk <= nblock; blk++) {
     for (int rep = 0; rep < 2; rep++) {  // smoother_gs(..., 2): two symmetric iterations
-      CALL(pc_sweep(c, s1, true, act));
-      CALL(pc_sweep(c, s1, false, act));
+      if (c->tiled) {
+        CALL(pc_sweep_tiled(c, s1, true, act));
+        CALL(pc_sweep_tiled(c, s1, false, act));
+      } else {
+        CALL(pc_sweep(c, s1, true, act));
+        CALL(pc_sweep(c, s1, false, act));
+      }
     }
     if (c->nx > 0)
       hipLaunchKernelGGL(k_exchange, dim3(nblocks(c->nx)), dim3(kBlock), 0, c->st, c->nx, c->xdst,
@@ -1016,6 +1234,170 @@ static int multi_subdomain_solver(cfdl_ctx *c) {
                      c->phic_p, c->fd.phic);
   hipLaunchKernelGGL(k_log_pc, dim3(1), dim3(64), 0, c->st, c->pctl, c->dlog, c->dlog_n);
   HIPCHK(hipGetLastError());
+  return CFDL_OK;
+}
+
+// Space-time tiled schedule of one sweep direction over the pc system.
+// Cells are binned into spatial tiles (per subdomain, about tile^3 cells
+// each, from the cell centres); in the sweep's processing order each cell
+// gets launch = max over the neighbours it depends on of (their launch, +1
+// when the neighbour lies in another tile).  So a dependency between two
+// tiles always spans launches, and inside one launch a tile's cells only
+// depend on earlier cells of the same tile: the workgroup of that (launch,
+// tile) batch processes them level by level.  Cells of one level are
+// independent (no edge joins two cells of equal forward level).
+static int build_tile_sched(cfdl_ctx *c, const std::vector<int> &rowp, const std::vector<int> &nbrp,
+                            const std::vector<int> &lev, const std::vector<int64_t> &tile,
+                            bool forward, cfdl_ctx::TileSched *ts, std::vector<int> *lcells) {
+  const int ne = c->ne;
+  // processing order: forward = increasing forward level, backward = decreasing
+  std::vector<int> order(ne);
+  for (int e = 0; e < ne; e++) order[e] = e;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return forward ? lev[a] < lev[b] : lev[a] > lev[b];
+  });
+  // dense tile ids
+  std::unordered_map<int64_t, int> tid_of;
+  std::vector<int> tid(ne);
+  for (int e = 0; e < ne; e++) {
+    auto it = tid_of.find(tile[e]);
+    if (it == tid_of.end()) it = tid_of.emplace(tile[e], static_cast<int>(tid_of.size())).first;
+    tid[e] = it->second;
+  }
+  const int64_t ntile = static_cast<int64_t>(tid_of.size());
+  // a batch spans at most `span` levels: a cell that would stretch its
+  // batch further moves to the tile's batch in a later launch
+  const char *sp = std::getenv("CFDL_TILE_SPAN");
+  const int span = sp ? std::max(1, std::atoi(sp)) : 16;
+  std::unordered_map<int64_t, int> first_level;  // (launch, tile) -> first level
+  std::vector<int> launch(ne, 0);
+  int nl = 0;
+  for (int e : order) {
+    int L = 0;
+    for (int k = rowp[e]; k < rowp[e + 1]; k++) {
+      const int d = nbrp[k];
+      if (d >= ne) continue;  // halo: fixed during the sweep
+      // d is a predecessor when the sweep reaches it before e
+      const bool pred = forward ? lev[d] < lev[e] : lev[d] > lev[e];
+      if (!pred) continue;
+      const int v = launch[d] + (tid[d] != tid[e] ? 1 : 0);
+      if (v > L) L = v;
+    }
+    for (;;) {
+      auto it = first_level.find(static_cast<int64_t>(L) * ntile + tid[e]);
+      if (it == first_level.end()) {
+        first_level.emplace(static_cast<int64_t>(L) * ntile + tid[e], lev[e]);
+        break;
+      }
+      if (std::abs(lev[e] - it->second) < span) break;
+      L++;
+    }
+    launch[e] = L;
+    nl = std::max(nl, L + 1);
+  }
+  // group by (launch, tile, level in processing order)
+  std::vector<int> cells(order);
+  std::stable_sort(cells.begin(), cells.end(), [&](int a, int b) {
+    if (launch[a] != launch[b]) return launch[a] < launch[b];
+    if (tid[a] != tid[b]) return tid[a] < tid[b];
+    return forward ? lev[a] < lev[b] : lev[a] > lev[b];
+  });
+  std::vector<int> bseg, seg;
+  ts->launch_ptr.assign(1, 0);
+  lcells->assign(nl, 0);
+  int cur_launch = 0;
+  for (int i = 0; i < ne; i++) {
+    const int e = cells[i];
+    const bool new_batch = i == 0 || launch[e] != launch[cells[i - 1]] || tid[e] != tid[cells[i - 1]];
+    const bool new_seg = new_batch || lev[e] != lev[cells[i - 1]];
+    while (cur_launch < launch[e]) {
+      ts->launch_ptr.push_back(static_cast<int>(bseg.size()));
+      cur_launch++;
+    }
+    if (new_batch) bseg.push_back(static_cast<int>(seg.size()));
+    if (new_seg) seg.push_back(i);
+    (*lcells)[launch[e]]++;
+  }
+  ts->launch_ptr.push_back(static_cast<int>(bseg.size()));
+  bseg.push_back(static_cast<int>(seg.size()));
+  seg.push_back(ne);
+  ts->nbatch = static_cast<int>(bseg.size()) - 1;
+  ts->nseg = static_cast<int>(seg.size()) - 1;
+  // LDS tables: per batch, local slots for every CSR entry (batch cells
+  // first, then distinct outside neighbours)
+  std::vector<int> brow(ne + 1, 0), xptr(ts->nbatch + 1, 0), xslot;
+  std::vector<unsigned short> lnbr;
+  std::vector<int> pos_in_batch(ne, -1);
+  size_t lds_max = 0;
+  bool fits = true;
+  for (int i = 0; i < ne; i++) brow[i + 1] = brow[i] + (rowp[cells[i] + 1] - rowp[cells[i]]);
+  lnbr.resize(brow[ne]);
+  std::vector<int> ext_slot;  // halo/phi index -> local slot, reset per batch
+  std::unordered_map<int, int> ext;
+  for (int bi = 0; bi < ts->nbatch; bi++) {
+    const int c0 = seg[bseg[bi]], c1 = seg[bseg[bi + 1]];
+    for (int i = c0; i < c1; i++) pos_in_batch[cells[i]] = i - c0;
+    ext.clear();
+    const int nc = c1 - c0;
+    for (int i = c0; i < c1; i++) {
+      const int e = cells[i];
+      for (int k = rowp[e]; k < rowp[e + 1]; k++) {
+        const int d = nbrp[k];
+        int slot;
+        if (d < ne && pos_in_batch[d] >= 0) {
+          slot = pos_in_batch[d];
+        } else {
+          auto it = ext.find(d);
+          if (it == ext.end()) {
+            slot = nc + static_cast<int>(ext.size());
+            ext.emplace(d, slot);
+            xslot.push_back(d);
+          } else {
+            slot = it->second;
+          }
+        }
+        if (slot > 65535) fits = false;
+        lnbr[brow[i] + (k - rowp[e])] = static_cast<unsigned short>(slot & 0xffff);
+      }
+    }
+    for (int i = c0; i < c1; i++) pos_in_batch[cells[i]] = -1;
+    xptr[bi + 1] = static_cast<int>(xslot.size());
+    const size_t nent = brow[c1] - brow[c0], nq = bseg[bi + 1] - bseg[bi];
+    const size_t bytes = 8 * (static_cast<size_t>(nc) * 3 + ext.size() + nent) +
+                         4 * (2 * static_cast<size_t>(nc) + 1 + nq + 1) + 2 * nent;
+    lds_max = std::max(lds_max, bytes);
+  }
+  if (xslot.empty()) xslot.push_back(0);
+  if (trace_on()) {
+    int64_t sum_max_seg = 0;
+    for (int m = 0; m + 1 < static_cast<int>(ts->launch_ptr.size()); m++) {
+      int mx = 0;
+      for (int bi = ts->launch_ptr[m]; bi < ts->launch_ptr[m + 1]; bi++) mx = std::max(mx, bseg[bi + 1] - bseg[bi]);
+      sum_max_seg += mx;
+    }
+    std::fprintf(stderr, "cfdl: %s tiled sweep: %d launches, %d batches, %d segments, sum of max "
+                 "segments per launch %lld, largest batch LDS %zu bytes\n",
+                 forward ? "forward" : "backward", nl, ts->nbatch, ts->nseg,
+                 static_cast<long long>(sum_max_seg), lds_max);
+  }
+  const char *no_lds = std::getenv("CFDL_TILE_LDS");
+  const bool lds_ok = !(no_lds && no_lds[0] == '0');
+  ts->lds_bytes = (lds_ok && fits && lds_max <= 64 * 1024) ? lds_max : 0;
+  CALL(upload(c, &ts->bseg, bseg));
+  CALL(upload(c, &ts->seg, seg));
+  CALL(upload(c, &ts->cells, cells));
+  std::vector<int> growp(ne);
+  bool hex = true;
+  for (int i = 0; i < ne; i++) {
+    growp[i] = rowp[cells[i]];
+    hex = hex && (rowp[cells[i] + 1] - rowp[cells[i]] == 6);
+  }
+  ts->hex = hex;
+  CALL(upload(c, &ts->brow, brow));
+  CALL(upload(c, &ts->growp, growp));
+  CALL(upload(c, &ts->xptr, xptr));
+  CALL(upload(c, &ts->xslot, xslot));
+  CALL(upload(c, &ts->lnbr, lnbr));
   return CFDL_OK;
 }
 
@@ -1257,6 +1639,35 @@ int cfdl_ctx_create(int ne, int nf, int nbf, const int *ef2nb_idx, const int *ef
       }
     }
     for (auto &v : xsrc) v = o2n[v];
+    {
+      // tiles of about kTileEdge^3 cells from the cell centres, per subdomain
+      const char *te = std::getenv("CFDL_TILE");
+      const int edge = te ? std::max(1, std::atoi(te)) : 8;
+      const char *mode = std::getenv("CFDL_GS_MODE");
+      C->tiled = !(mode && std::string(mode) == "level");
+      std::vector<int> levn(ne);
+      std::vector<int64_t> tile(ne);
+      for (int s = 0; s < nsubd; s++) {
+        double lo[3] = {1e300, 1e300, 1e300}, vsum = 0.0;
+        for (int nq = C->co[s]; nq < C->co[s + 1]; nq++) {
+          const int g = l2gn[nq];
+          const double r[3] = {xc[g], yc[g], zc[g]};
+          for (int i = 0; i < 3; i++) lo[i] = std::min(lo[i], r[i]);
+          vsum += vol[g];
+        }
+        const double h = std::cbrt(vsum / (C->co[s + 1] - C->co[s])) * edge;
+        for (int nq = C->co[s]; nq < C->co[s + 1]; nq++) {
+          const int g = l2gn[nq];
+          const int64_t bx = static_cast<int64_t>((xc[g] - lo[0]) / h);
+          const int64_t by = static_cast<int64_t>((yc[g] - lo[1]) / h);
+          const int64_t bz = static_cast<int64_t>((zc[g] - lo[2]) / h);
+          tile[nq] = ((static_cast<int64_t>(s) * 4096 + bx) * 4096 + by) * 4096 + bz;
+          levn[nq] = lev[n2o[nq]];
+        }
+      }
+      CALL(build_tile_sched(C, rowp, nbrp, levn, tile, true, &C->tf, &C->launch_cells[0]));
+      CALL(build_tile_sched(C, rowp, nbrp, levn, tile, false, &C->tb, &C->launch_cells[1]));
+    }
     C->Lp = static_cast<int>(nbrp.size());
     C->nh = nh;
     C->nx = static_cast<int>(xdst.size());

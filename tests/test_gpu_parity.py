@@ -155,3 +155,20 @@ def test_set_get_roundtrip_and_errors():
         eq.set("u", x[:-1])
     with pytest.raises(cfdl.CfdlError):
         eq.set_relaxation(0.0, 1.0)
+
+
+@pytest.mark.parametrize("tile,mode,lds", [("2", "tiled", "1"), ("3", "tiled", "1"),
+                                           ("3", "tiled", "0"), ("8", "tiled", "1"),
+                                           ("8", "level", "1")])
+def test_sweep_schedules_bitwise(monkeypatch, tile, mode, lds):
+    """The pressure-correction sweeps run either as one launch per dependency
+    level or as space-time tiles; small tiles put many tile boundaries and
+    launches into small meshes.  Every schedule gives the oracle's bits."""
+    monkeypatch.setenv("CFDL_TILE", tile)
+    monkeypatch.setenv("CFDL_GS_MODE", mode)
+    monkeypatch.setenv("CFDL_TILE_LDS", lds)
+    eq, oc = make_pair(12, 0.15, 4)
+    eq.run(1, 2)
+    oc.run(1, 2)
+    assert_fields_equal(eq, oc, ["u", "v", "w", "p", "mip", "gp", "phic"])
+    assert_logs_match(eq.log(), oc.log())

@@ -16,7 +16,7 @@ BC_WALL, BC_LID, BC_SYMMETRY = 0, 1, 2
 
 FIELDS = {name: k for k, name in enumerate(
     ["u", "v", "w", "p", "u0", "v0", "w0", "gu", "gv", "gw", "gp", "gpc", "mip", "mip0",
-     "bu", "bv", "bw", "d", "dc", "ap", "b", "anb", "phic"])}
+     "bu", "bv", "bw", "d", "dc", "ap", "b", "anb", "phic", "work", "gwork"])}
 
 
 def _fid(name):
@@ -40,9 +40,9 @@ class UVWP:
     def _sizes(ne, nf, nbf):
         n, L = ne + nbf, 2 * nf - nbf
         s = {}
-        for k in ("u", "v", "w", "p", "u0", "v0", "w0", "phic"):
+        for k in ("u", "v", "w", "p", "u0", "v0", "w0", "phic", "work"):
             s[k] = n
-        for k in ("gu", "gv", "gw", "gp", "gpc"):
+        for k in ("gu", "gv", "gw", "gp", "gpc", "gwork"):
             s[k] = 3 * n
         for k in ("mip", "mip0"):
             s[k] = nf

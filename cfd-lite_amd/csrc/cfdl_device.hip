@@ -1534,7 +1534,7 @@ int cfdl_ctx_create(int ne, int nf, int nbf, const int *ef2nb_idx, const int *ef
   // --- fields ---
   const int64_t sz[CFDL_NFIELDS] = {n, n, n, n, n, n, n, 3 * int64_t{n}, 3 * int64_t{n},
                                     3 * int64_t{n}, 3 * int64_t{n}, 3 * int64_t{n}, nf, nf, ne,
-                                    ne, ne, ne, ne, ne, ne, L, n};
+                                    ne, ne, ne, ne, ne, ne, L, n, n, 3 * int64_t{n}};
   for (int f = 0; f < CFDL_NFIELDS; f++) {
     C->fsize[f] = sz[f];
     CALL(dalloc(C, &C->fields[f], sz[f]));
@@ -1735,6 +1735,42 @@ int cfdl_ctx_create_cavity(const cfdl_mesh *m, double rho, double mu, double dt,
                          m->vol.data(), m->aip.data(), m->rip.data(), m->nsec, m->esec.data(),
                          kind.data(), m->nsubd, m->gf2g.data(), m->g2gf.data(),
                          m->g2gf_idx.data(), r.data(), u.data(), dt, nit, out);
+}
+
+int cfdl_set_properties(cfdl_ctx *c, const double *rho, const double *mu) {
+  CALL(check_ctx(c));
+  if (!rho || !mu) {
+    set_error("cfdl_set_properties: null array");
+    return CFDL_EINVAL;
+  }
+  HIPCHK(hipMemcpyAsync(const_cast<double *>(c->md.rho), rho, 8 * size_t(c->ne), hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(const_cast<double *>(c->md.mu), mu, 8 * size_t(c->ne), hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return CFDL_OK;
+}
+
+int cfdl_set_dt(cfdl_ctx *c, double dt) {
+  CALL(check_ctx(c));
+  if (!(dt > 0.0)) {
+    set_error("cfdl_set_dt: dt must be positive");
+    return CFDL_EINVAL;
+  }
+  c->dt = dt;
+  return CFDL_OK;
+}
+
+int cfdl_set_bc_kind(cfdl_ctx *c, int first, int last, int kind) {
+  CALL(check_ctx(c));
+  if (first <= c->ne || last > c->ne + c->nbf || first > last ||
+      (kind != CFDL_BC_WALL && kind != CFDL_BC_LID && kind != CFDL_BC_SYMMETRY)) {
+    set_error("cfdl_set_bc_kind: halo range or kind out of range");
+    return CFDL_EINVAL;
+  }
+  std::vector<int> k(last - first + 1, kind);
+  HIPCHK(hipMemcpyAsync(const_cast<int *>(c->md.hkind) + (first - c->ne - 1), k.data(),
+                        sizeof(int) * k.size(), hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return CFDL_OK;
 }
 
 int cfdl_set_relaxation(cfdl_ctx *c, double alpha_u, double alpha_p) {

@@ -57,6 +57,8 @@ enum cfdl_field {
   CFDL_AP, CFDL_B,                                                /* ne       */
   CFDL_ANB,                                                       /* 2nf-nbf  */
   CFDL_PHIC,                                                      /* ne+nbf   */
+  CFDL_WORK,   /* ne+nbf scratch: any element field a caller passes in */
+  CFDL_GWORK,  /* 3(ne+nbf) scratch gradient */
   CFDL_NFIELDS
 };
 
@@ -129,6 +131,15 @@ void cfdl_ctx_destroy(cfdl_ctx *c);
    d and dc are formed, and p += alpha_p*pc, gp += alpha_p*gpc.  Both 1 (the
    default) is the reference algorithm bit for bit. */
 int cfdl_set_relaxation(cfdl_ctx *c, double alpha_u, double alpha_p);
+
+/* Host-side state a Fortran drop-in keeps authoritative (properties_t and
+   phys_t%dt, mod_properties.f90:9-51, mod_physics.f90:15; the BC type the
+   setters assign, mod_uvwp.f90:500,525,555: 'dirichlet' -> CFDL_BC_WALL,
+   'zero_flux' -> CFDL_BC_SYMMETRY) for the halos first..last (1-based
+   element numbers, ne < first <= last <= ne+nbf). */
+int cfdl_set_properties(cfdl_ctx *c, const double *rho, const double *mu);
+int cfdl_set_dt(cfdl_ctx *c, double dt);
+int cfdl_set_bc_kind(cfdl_ctx *c, int first, int last, int kind);
 
 /* host <-> device copies of one field (n = element count of that field) */
 int cfdl_set_field(cfdl_ctx *c, int field, const double *src, int64_t n);

@@ -29,14 +29,16 @@ def _unlimited_stack():
     resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
 
 
-def run_refsim(meshfile, outfile, nsubd, rho, mu, dt, ntstep, ncoef, nit=100, opt="O2", timeout=600):
-    args = [refsim_path(opt), meshfile, outfile, str(nsubd), repr(float(rho)), repr(float(mu)),
+def run_refsim(meshfile, outfile, nsubd, rho, mu, dt, ntstep, ncoef, nit=100, opt="O2", timeout=600,
+               binary=None):
+    args = [binary or refsim_path(opt), meshfile, outfile, str(nsubd), repr(float(rho)), repr(float(mu)),
             repr(float(dt)), str(ntstep), str(ncoef), str(nit)]
     p = subprocess.run(args, capture_output=True, text=True, timeout=timeout,
                        preexec_fn=_unlimited_stack)
     if p.returncode != 0:
         raise RuntimeError("refsim failed: %s\n%s" % (p.returncode, p.stderr[-2000:]))
     log, secs = [], None
+    run_refsim.last_stdout = p.stdout
     for line in p.stdout.splitlines():
         m = _LINE.match(line)
         if m and m.group(1) in ("u", "v", "w", "pc"):

@@ -13,9 +13,12 @@ the timed region starts.
 Prints one JSON line.  value = 2,097,152 * K / seconds for K timed steps.
 roofline: the pressure-correction Gauss-Seidel sweep kernel k_gs_tiles_lds (the dominant
 kernel), 108 algorithmic bytes per cell update (FP64 CSR sweep: b, ap, 6
-anb, 6 neighbour indices, neighbour/own phi read, phi write) times the cells
-of one launch, over the mean launch duration from HIP events on the context
-stream around every 97th launch of the timed region.
+anb, 6 neighbour indices, own phi read and write) times the cells of one
+launch, over the mean launch duration: HIP events on the context stream
+around every 7th sweep of the timed region (all of its launches, back to
+back), elapsed time divided by launches.  --traffic-kib adds the HBM bytes
+per launch measured with rocprofv3 (FETCH_SIZE + WRITE_SIZE, calibrated on
+the copy kernel of update_time).
 cpu_baseline: the reference Fortran (flang -O2 build in oracle/_ref) on one
 host core, 64^3 sample (see --cpu-*), or the oracle's C restatement when the
 reference build is absent.
@@ -118,9 +121,13 @@ def main():
     ap.add_argument("--alpha-u", type=float, default=0.7)
     ap.add_argument("--alpha-p", type=float, default=0.3)
     ap.add_argument("--cpu-n", type=int, default=64)
-    ap.add_argument("--cpu-iters", type=int, default=4)
+    ap.add_argument("--cpu-iters", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--sample-every", type=int, default=97)
+    ap.add_argument("--sample-every", type=int, default=7, help="time every k-th pc sweep")
+    ap.add_argument("--tstep-every", type=int, default=3,
+                    help="outer iterations per pseudo-time step (update_time after each group)")
+    ap.add_argument("--traffic-kib", type=float, default=None,
+                    help="measured HBM KiB per sweep launch (rocprofv3 FETCH_SIZE + WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
@@ -143,7 +150,7 @@ def main():
         eq.update_boundaries()
         eq.solve_uvwp()
         step_no[0] += 1
-        if step_no[0] % 3 == 0:
+        if step_no[0] % args.tstep_every == 0:
             eq.update_time()
 
     for _ in range(args.warmup):
@@ -156,22 +163,23 @@ def main():
     t = time.perf_counter()
     for k in range(args.steps):
         step()
-        if k == 0 or (k + 1) % 5 == 0:
-            print("bench: step %d queued" % (k + 1), file=sys.stderr, flush=True)
+        print("bench: step %d queued" % (k + 1), file=sys.stderr, flush=True)
     eq.synchronize()
     secs = time.perf_counter() - t
     eq.set_profile(0)
     st = eq.stats()
     log = eq.log()
     value = ne * args.steps / secs
-    kernel_ms = st["sample_ms"]
-    cells = st["sample_cells"]
+    kernel_ms = st["launch_ms"]
+    cells = st["launch_cells"]
     achieved = BYTES_PER_CELL_SWEEP * cells / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else 0.0
+    traffic = None if args.traffic_kib is None else round(args.traffic_kib * 1024.0)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "algorithmic_bytes_per_launch": round(BYTES_PER_CELL_SWEEP * cells),
                 "kernel": "k_gs_tiles_lds<true> (pressure-correction Gauss-Seidel, space-time tiles in LDS)",
                 "launch_ms": round(kernel_ms, 5), "cells_per_launch": round(cells, 1),
-                "samples": int(st["samples"])}
+                "sampled_launches": int(st["sampled_launches"])}
     base = None if args.no_cpu else cpu_baseline(args.cpu_n, args.cpu_iters, mu)
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "cell-iterations/s", "n_gpus": 1,

@@ -172,8 +172,8 @@ class UVWP:
     def stats(self):
         s = (C.c_double * 8)()
         check(self._lib.cfdl_stats(self._h, s, 8))
-        keys = ("gs_launches", "pc_launches", "pc_cells", "outer_iterations", "samples",
-                "sample_ms", "sample_cells", "pc_levels_per_sym_sweep")
+        keys = ("gs_launches", "pc_launches", "pc_cells", "outer_iterations", "sampled_launches",
+                "launch_ms", "launch_cells", "pc_levels_per_sym_sweep")
         return dict(zip(keys, list(s)))
 
     def reset_stats(self):

@@ -934,12 +934,14 @@ struct cfdl_ctx {
   int *dlog_n = nullptr;
   std::vector<LogLine> hlog;
   // launch counters and sampled timing of the pressure-correction sweep
-  // kernel (events around every sample_every-th level launch)
-  int64_t gs_launches = 0, pc_launches = 0, pc_cells = 0, outer = 0;
+  // kernel: events around every sample_every-th sweep (all its launches,
+  // back to back on the stream)
+  int64_t gs_launches = 0, pc_launches = 0, pc_cells = 0, outer = 0, pc_sweeps = 0;
   int sample_every = 0;
   struct Sample {
     hipEvent_t a, b;
-    int cells;
+    int64_t cells;
+    int launches;
   };
   std::vector<Sample> samples;
   size_t nsamples = 0;
@@ -1008,35 +1010,46 @@ static int make_sched(cfdl_ctx *c, const std::vector<int> &lev, int n, Sched *s)
 }
 
 static int sweep(cfdl_ctx *c, const Sched &s, const SysDesc &sys, int nc, double sor,
-                 const int *active, bool pc = false) {
+                 const int *active) {
   for (int l = 0; l < s.nlev; l++) {
     const int n = s.ptr[l + 1] - s.ptr[l];
-    cfdl_ctx::Sample *smp = nullptr;
-    if (pc) {
-      if (c->sample_every > 0 && c->pc_launches % c->sample_every == 0) {
-        if (c->nsamples == c->samples.size()) {
-          cfdl_ctx::Sample x{};
-          HIPCHK(hipEventCreate(&x.a));
-          HIPCHK(hipEventCreate(&x.b));
-          c->samples.push_back(x);
-        }
-        smp = &c->samples[c->nsamples++];
-        smp->cells = n;
-        HIPCHK(hipEventRecord(smp->a, c->st));
-      }
-      c->pc_launches++;
-      c->pc_cells += n;
-    }
     if (nc == 3)
       hipLaunchKernelGGL(k_gs_level<3>, dim3(nblocks(n)), dim3(kBlock), 0, c->st, sys,
                          s.cells + s.ptr[l], n, sor, active);
     else
       hipLaunchKernelGGL(k_gs_level<1>, dim3(nblocks(n)), dim3(kBlock), 0, c->st, sys,
                          s.cells + s.ptr[l], n, sor, active);
-    if (smp) HIPCHK(hipEventRecord(smp->b, c->st));
   }
   c->gs_launches += s.nlev;
   HIPCHK(hipGetLastError());
+  return CFDL_OK;
+}
+
+// Sampled timing of the pressure-correction sweeps: HIP events on the
+// context stream before the first and after the last launch of every
+// sample_every-th sweep.  The launches run back to back in between, so
+// elapsed / launches is the mean launch duration without the dispatch
+// latency that an event pair around a single launch adds.
+static int sample_begin(cfdl_ctx *c, cfdl_ctx::Sample **smp) {
+  *smp = nullptr;
+  const int64_t k = c->pc_sweeps++;
+  if (c->sample_every <= 0 || k % c->sample_every != 0) return CFDL_OK;
+  if (c->nsamples == c->samples.size()) {
+    cfdl_ctx::Sample x{};
+    HIPCHK(hipEventCreate(&x.a));
+    HIPCHK(hipEventCreate(&x.b));
+    c->samples.push_back(x);
+  }
+  *smp = &c->samples[c->nsamples++];
+  HIPCHK(hipEventRecord((*smp)->a, c->st));
+  return CFDL_OK;
+}
+
+static int sample_end(cfdl_ctx *c, cfdl_ctx::Sample *smp, int launches, int64_t cells) {
+  if (!smp) return CFDL_OK;
+  smp->launches = launches;
+  smp->cells = cells;
+  HIPCHK(hipEventRecord(smp->b, c->st));
   return CFDL_OK;
 }
 
@@ -1046,6 +1059,10 @@ static int sweep(cfdl_ctx *c, const Sched &s, const SysDesc &sys, int nc, double
 // backward schedule).
 static int pc_sweep(cfdl_ctx *c, const SysDesc &sys, bool forward, const int *active) {
   const int ns = c->nsubd;
+  cfdl_ctx::Sample *smp = nullptr;
+  CALL(sample_begin(c, &smp));
+  int launches = 0;
+  int64_t cells = 0;
   for (int i = 0; i < c->pc_nlev; i++) {
     const int l = forward ? i : c->pc_nlev - 1 - i;
     LevelRanges r{};
@@ -1060,46 +1077,27 @@ static int pc_sweep(cfdl_ctx *c, const SysDesc &sys, bool forward, const int *ac
     }
     const int n = r.pre[ns];
     if (n == 0) continue;
-    cfdl_ctx::Sample *smp = nullptr;
-    if (c->sample_every > 0 && c->pc_launches % c->sample_every == 0) {
-      if (c->nsamples == c->samples.size()) {
-        cfdl_ctx::Sample x{};
-        HIPCHK(hipEventCreate(&x.a));
-        HIPCHK(hipEventCreate(&x.b));
-        c->samples.push_back(x);
-      }
-      smp = &c->samples[c->nsamples++];
-      smp->cells = n;
-      HIPCHK(hipEventRecord(smp->a, c->st));
-    }
-    c->pc_launches++;
-    c->pc_cells += n;
     hipLaunchKernelGGL(k_gs_range, dim3(nblocks(n)), dim3(kBlock), 0, c->st, sys, r, 1.02, active);
-    if (smp) HIPCHK(hipEventRecord(smp->b, c->st));
-    c->gs_launches++;
+    launches++;
+    cells += n;
   }
+  CALL(sample_end(c, smp, launches, cells));
+  c->pc_launches += launches;
+  c->pc_cells += cells;
+  c->gs_launches += launches;
   HIPCHK(hipGetLastError());
   return CFDL_OK;
 }
 
 static int pc_sweep_tiled(cfdl_ctx *c, const SysDesc &sys, bool forward, const int *active) {
   const cfdl_ctx::TileSched &t = forward ? c->tf : c->tb;
+  const std::vector<int> &lc = c->launch_cells[forward ? 0 : 1];
   const int nl = static_cast<int>(t.launch_ptr.size()) - 1;
+  cfdl_ctx::Sample *smp = nullptr;
+  CALL(sample_begin(c, &smp));
+  int64_t cells = 0;
   for (int m = 0; m < nl; m++) {
     const int b0 = t.launch_ptr[m], nb = t.launch_ptr[m + 1] - b0;
-    cfdl_ctx::Sample *smp = nullptr;
-    if (c->sample_every > 0 && c->pc_launches % c->sample_every == 0) {
-      if (c->nsamples == c->samples.size()) {
-        cfdl_ctx::Sample x{};
-        HIPCHK(hipEventCreate(&x.a));
-        HIPCHK(hipEventCreate(&x.b));
-        c->samples.push_back(x);
-      }
-      smp = &c->samples[c->nsamples++];
-      smp->cells = -1;  // filled from the host copy of the schedule below
-      HIPCHK(hipEventRecord(smp->a, c->st));
-    }
-    c->pc_launches++;
     if (t.lds_bytes > 0 && t.hex)
       hipLaunchKernelGGL(k_gs_tiles_lds<true>, dim3(nb), dim3(kLdsThreads), t.lds_bytes, c->st,
                          sys, t.bseg, t.seg, t.cells, t.brow, t.growp, t.lnbr, t.xptr, t.xslot, b0,
@@ -1111,13 +1109,12 @@ static int pc_sweep_tiled(cfdl_ctx *c, const SysDesc &sys, bool forward, const i
     else
       hipLaunchKernelGGL(k_gs_tiles, dim3(nb), dim3(kTileThreads), 0, c->st, sys, t.bseg, t.seg,
                          t.cells, b0, 1.02, active);
-    if (smp) {
-      HIPCHK(hipEventRecord(smp->b, c->st));
-      smp->cells = c->launch_cells[forward ? 0 : 1][m];
-    }
-    c->pc_cells += c->launch_cells[forward ? 0 : 1][m];
-    c->gs_launches++;
+    cells += lc[m];
   }
+  CALL(sample_end(c, smp, nl, cells));
+  c->pc_launches += nl;
+  c->pc_cells += cells;
+  c->gs_launches += nl;
   HIPCHK(hipGetLastError());
   return CFDL_OK;
 }
@@ -1640,9 +1637,10 @@ int cfdl_ctx_create(int ne, int nf, int nbf, const int *ef2nb_idx, const int *ef
     }
     for (auto &v : xsrc) v = o2n[v];
     {
-      // tiles of about kTileEdge^3 cells from the cell centres, per subdomain
+      // tiles of about edge^3 cells from the cell centres, per subdomain
+      // (edge 7 measured fastest on the 128^3 cavity among 5..8)
       const char *te = std::getenv("CFDL_TILE");
-      const int edge = te ? std::max(1, std::atoi(te)) : 8;
+      const int edge = te ? std::max(1, std::atoi(te)) : 7;
       const char *mode = std::getenv("CFDL_GS_MODE");
       C->tiled = !(mode && std::string(mode) == "level");
       std::vector<int> levn(ne);
@@ -2021,17 +2019,17 @@ int cfdl_set_profile(cfdl_ctx *c, int sample_every) {
 int cfdl_stats(cfdl_ctx *c, double *stats, int n) {
   CALL(check_ctx(c));
   HIPCHK(hipStreamSynchronize(c->st));
-  double ms = 0.0, cells = 0.0;
+  double ms = 0.0, cells = 0.0, nl = 0.0;
   for (size_t k = 0; k < c->nsamples; k++) {
     float t = 0.f;
     HIPCHK(hipEventElapsedTime(&t, c->samples[k].a, c->samples[k].b));
     ms += t;
-    cells += c->samples[k].cells;
+    cells += static_cast<double>(c->samples[k].cells);
+    nl += c->samples[k].launches;
   }
-  const double ns = static_cast<double>(c->nsamples);
   const double v[8] = {static_cast<double>(c->gs_launches), static_cast<double>(c->pc_launches),
-                       static_cast<double>(c->pc_cells), static_cast<double>(c->outer), ns,
-                       ns > 0 ? ms / ns : 0.0, ns > 0 ? cells / ns : 0.0,
+                       static_cast<double>(c->pc_cells), static_cast<double>(c->outer), nl,
+                       nl > 0 ? ms / nl : 0.0, nl > 0 ? cells / nl : 0.0,
                        static_cast<double>(2 * c->pc_nlev)};
   for (int i = 0; i < n && i < 8; i++) stats[i] = v[i];
   return CFDL_OK;
@@ -2040,7 +2038,7 @@ int cfdl_stats(cfdl_ctx *c, double *stats, int n) {
 int cfdl_stats_reset(cfdl_ctx *c) {
   CALL(check_ctx(c));
   HIPCHK(hipStreamSynchronize(c->st));
-  c->gs_launches = c->pc_launches = c->pc_cells = c->outer = 0;
+  c->gs_launches = c->pc_launches = c->pc_cells = c->outer = c->pc_sweeps = 0;
   c->nsamples = 0;
   return CFDL_OK;
 }

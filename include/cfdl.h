@@ -179,12 +179,14 @@ int cfdl_log_clear(cfdl_ctx *c);
 
 /* Launch counters and sampled kernel timing (no counterpart in the
    reference, which has no timers).  cfdl_set_profile(c, k) brackets every
-   k-th level launch of the pressure-correction sweeps with HIP events on the
-   context stream (0 = off).  cfdl_stats synchronises and fills up to 8 values:
-   [0] Gauss-Seidel level launches (all solves), [1] pressure-correction level
-   launches, [2] cells updated by those launches, [3] outer iterations,
-   [4] sampled launches, [5] mean sampled launch duration (ms), [6] mean cells
-   per sampled launch, [7] levels per forward + backward pc sweep. */
+   k-th pressure-correction sweep (all of its launches, back to back) with
+   HIP events on the context stream (0 = off).  cfdl_stats synchronises and
+   fills up to 8 values: [0] Gauss-Seidel launches (all solves),
+   [1] pressure-correction sweep launches, [2] cells updated by those
+   launches, [3] outer iterations, [4] launches inside sampled sweeps,
+   [5] mean launch duration in sampled sweeps (ms: elapsed / launches),
+   [6] mean cells per launch in sampled sweeps, [7] dependency levels per
+   forward + backward pc sweep. */
 int cfdl_set_profile(cfdl_ctx *c, int sample_every);
 int cfdl_stats(cfdl_ctx *c, double *stats, int n);
 int cfdl_stats_reset(cfdl_ctx *c);

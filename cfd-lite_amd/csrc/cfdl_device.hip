@@ -505,48 +505,28 @@ __global__ void k_gs_level(SysDesc s, const int *__restrict__ cells, int n, doub
   }
 }
 
-// One dependency level of the pressure-correction sweeps.  The subdomain
-// systems are numbered subdomain-major and, inside a subdomain, by forward
-// level, so level l of every subdomain is one contiguous range; thread t
-// maps to range s with pre[s] <= t < pre[s+1].  Coefficient reads are
-// coalesced; only the neighbour values are gathered.
-struct LevelRanges {
-  int ns;
-  int lo[kMaxSys], pre[kMaxSys + 1];
-};
-__global__ void k_gs_range(SysDesc s, LevelRanges r, double sor, const int *__restrict__ active) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= r.pre[r.ns] || !active[0]) return;
-  int q = 0;
-  while (t >= r.pre[q + 1]) q++;
-  const int e = r.lo[q] + (t - r.pre[q]);
-  const int r0 = s.row[e], r1 = s.row[e + 1];
-  const double a = s.ap[e];
-  double *phi = s.phi[0];
-  double sum = s.b[0][e];
-  for (int k = r0; k < r1; k++) sum = sum + s.anb[k] * phi[s.nbr[k]];
-  phi[e] = (sum + (sor - 1.0) * a * phi[e]) / a / sor;
-}
-
-// Space-time tiled sweep: one workgroup per (launch, tile) batch walks the
-// batch's dependency-level segments in order, __syncthreads between them.
-// The host schedule puts every dependency that crosses tiles into a later
-// launch, so each workgroup only waits for its own earlier segments; the
-// sequential order of every cell's update is unchanged.
+// Space-time tiled sweeps of the pressure-correction system.  The host
+// numbers that system so that every (launch, tile) batch is one contiguous
+// cell range whose dependency-level segments are contiguous sub-ranges:
+// seg[bseg[b] .. bseg[b+1]) are the nq+1 segment bounds of batch b.  A
+// dependency between tiles always spans launches, so a workgroup only waits
+// for its own earlier segments (a barrier between segments).  The forward
+// sweep walks the launches and each batch's segments in order; the
+// backward sweep walks both in reverse, which visits every cell after all
+// of its backward predecessors (its neighbours on higher forward levels).
 constexpr int kTileThreads = 64;
 __global__ void __launch_bounds__(kTileThreads)
-k_gs_tiles(SysDesc s, const int *__restrict__ bseg, const int *__restrict__ seg,
-           const int *__restrict__ cells, int batch0, double sor,
-           const int *__restrict__ active) {
+k_gs_tiles(SysDesc s, const int *__restrict__ bseg, const int *__restrict__ seg, int batch0,
+           int backward, double sor, const int *__restrict__ active) {
   if (!active[0]) return;
   const int b = batch0 + blockIdx.x;
-  const int q0 = bseg[b], q1 = bseg[b + 1];
+  const int s0 = bseg[b], nq = bseg[b + 1] - s0 - 1;
   double *phi = s.phi[0];
   const double *rhs = s.b[0];
-  for (int q = q0; q < q1; q++) {
-    const int c1 = seg[q + 1];
-    for (int t = seg[q] + threadIdx.x; t < c1; t += kTileThreads) {
-      const int e = cells[t];
+  for (int j = 0; j < nq; j++) {
+    const int q = backward ? nq - 1 - j : j;
+    const int c1 = seg[s0 + q + 1];
+    for (int e = seg[s0 + q] + threadIdx.x; e < c1; e += kTileThreads) {
       const int r0 = s.row[e], r1 = s.row[e + 1];
       const double a = s.ap[e];
       double sum = rhs[e];
@@ -557,69 +537,68 @@ k_gs_tiles(SysDesc s, const int *__restrict__ bseg, const int *__restrict__ seg,
   }
 }
 
-// The same schedule with the batch resident in LDS: the workgroup gathers
-// its cells' coefficients and current values plus the values of every
-// outside neighbour (constant during the launch: a neighbour in another tile
-// is never updated in the same launch) into LDS, walks the level segments
-// on LDS only, and writes its cells back.  Per batch: cells [c0, c0+nc) of
-// the schedule's cell list, CSR entries [brow[c0], brow[c0+nc)) with local
-// slot indices lnbr (< nc: a batch cell, else outside slot nc + j whose
-// global index is xslot[xptr[b] + j]).
-constexpr int kLdsThreads = 64;  // one wave: segments average about a dozen cells
+// The same sweep with the batch resident in LDS.  One wave loads the
+// batch's values, coefficients and local slot table (contiguous ranges
+// starting at its first cell c0 and first CSR entry r0) plus the values of
+// its outside neighbours (xslot), walks the segments on LDS only and writes
+// the batch back.  Outside neighbours are constant during the launch: a
+// cell of another tile is never updated in the same launch.  lnbr gives,
+// per CSR entry, the LDS slot of the neighbour: < nc a batch cell, else
+// outside value nc + j.  Every load of a gather round is issued before the
+// first LDS store, so the gather costs two memory latencies (the outside
+// indices, then their values).  The segment count is at most 63 (host).
+constexpr int kLdsThreads = 64;
 template <bool HEX>
 __global__ void __launch_bounds__(kLdsThreads)
 k_gs_tiles_lds(SysDesc s, const int *__restrict__ bseg, const int *__restrict__ seg,
-               const int *__restrict__ cells, const int *__restrict__ brow,
-               const int *__restrict__ growp, const unsigned short *__restrict__ lnbr,
-               const int *__restrict__ xptr, const int *__restrict__ xslot, int batch0,
-               double sor, const int *__restrict__ active) {
+               const unsigned short *__restrict__ lnbr, const int *__restrict__ xptr,
+               const int *__restrict__ xslot, int batch0, int backward, double sor,
+               const int *__restrict__ active) {
   extern __shared__ double lds[];
   if (!active[0]) return;
   const int b = batch0 + blockIdx.x;
-  const int q0 = bseg[b], q1 = bseg[b + 1], nq = q1 - q0;
-  const int c0 = seg[q0], nc = seg[q1] - c0;
-  const int e0 = brow[c0], ne_b = brow[c0 + nc] - e0;
+  const int s0 = bseg[b], nq = bseg[b + 1] - s0 - 1;
+  const int c0 = seg[s0], nc = seg[s0 + nq] - c0;
+  const int r0 = s.row[c0], nent = s.row[c0 + nc] - r0;
   const int x0 = xptr[b], nx = xptr[b + 1] - x0;
   double *lphi = lds;              // nc + nx values
   double *lap = lphi + nc + nx;    // nc
   double *lb = lap + nc;           // nc
-  double *lanb = lb + nc;          // ne_b
-  int *lrow = reinterpret_cast<int *>(lanb + ne_b);  // nc + 1 local row starts
-  int *lseg = lrow + nc + 1;                         // nq + 1 local segment starts
-  int *lcell = lseg + nq + 1;                        // nc cell indices
-  unsigned short *lnb = reinterpret_cast<unsigned short *>(lcell + nc);  // ne_b slots
+  double *lanb = lb + nc;          // nent
+  int *lseg = reinterpret_cast<int *>(lanb + nent);  // nq + 1 bounds relative to c0
+  int *lrow = lseg + nq + 1;                          // nc + 1 row starts (general rows)
+  unsigned short *lnb = reinterpret_cast<unsigned short *>(lrow + (HEX ? 0 : nc + 1));
   double *phi = s.phi[0];
-  const double *rhs = s.b[0];
-  // gather: every load of a round is issued before the first LDS store so
-  // that the round costs two memory latencies (index, then value)
+  const double *pp = phi + c0, *pa = s.ap + c0, *pb = s.b[0] + c0, *pn = s.anb + r0;
+  const unsigned short *pl = lnbr + r0;
+  const int sg = threadIdx.x <= nq ? seg[s0 + threadIdx.x] - c0 : 0;
   constexpr int U = 8;
-  for (int base = 0; base < nc; base += U * kLdsThreads) {
-    int ee[U], gg[U], oo[U], nn[U];
+  const int nmax = nc > nx ? nc : nx;
+  for (int base = 0; base < nmax; base += U * kLdsThreads) {
+    int xs[U];
+    double vp[U], va[U], vb[U], vx[U], vn[U][6];
     unsigned short sl[U][6];
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int i = base + u * kLdsThreads + threadIdx.x;
-      const bool ok = i < nc;
-      ee[u] = ok ? cells[c0 + i] : 0;
-      gg[u] = ok ? growp[c0 + i] : 0;
-      oo[u] = ok ? brow[c0 + i] - e0 : 0;
-      nn[u] = ok ? brow[c0 + i + 1] - brow[c0 + i] : 0;
-      if (HEX) {
+      xs[u] = -1;
+      if (base + u * kLdsThreads < nx) xs[u] = i < nx ? xslot[x0 + i] : -1;
+      if (base + u * kLdsThreads < nc) {  // wave-uniform: rows of this round exist
+        const int ic = i < nc ? i : 0;    // lanes past the end reload row 0
+        vp[u] = pp[ic];
+        va[u] = pa[ic];
+        vb[u] = pb[ic];
+        if (HEX) {
 #pragma unroll
-        for (int k = 0; k < 6; k++) sl[u][k] = ok ? lnbr[e0 + 6 * i + k] : 0;
+          for (int k = 0; k < 6; k++) {
+            vn[u][k] = pn[6 * ic + k];
+            sl[u][k] = pl[6 * ic + k];
+          }
+        }
       }
     }
-    double vp[U], va[U], vb[U], vn[U][6];
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      vp[u] = phi[ee[u]];
-      va[u] = s.ap[ee[u]];
-      vb[u] = rhs[ee[u]];
-      if (HEX) {
-#pragma unroll
-        for (int k = 0; k < 6; k++) vn[u][k] = s.anb[gg[u] + k];
-      }
-    }
+    for (int u = 0; u < U; u++) vx[u] = xs[u] >= 0 ? phi[xs[u]] : 0.0;
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int i = base + u * kLdsThreads + threadIdx.x;
@@ -627,43 +606,28 @@ k_gs_tiles_lds(SysDesc s, const int *__restrict__ bseg, const int *__restrict__ 
         lphi[i] = vp[u];
         lap[i] = va[u];
         lb[i] = vb[u];
-        lrow[i] = oo[u];
-        lcell[i] = ee[u];
         if (HEX) {
 #pragma unroll
           for (int k = 0; k < 6; k++) {
-            lanb[oo[u] + k] = vn[u][k];
-            lnb[oo[u] + k] = sl[u][k];
-          }
-        } else {
-          for (int k = 0; k < nn[u]; k++) {
-            lanb[oo[u] + k] = s.anb[gg[u] + k];
-            lnb[oo[u] + k] = lnbr[e0 + oo[u] + k];
+            lanb[6 * i + k] = vn[u][k];
+            lnb[6 * i + k] = sl[u][k];
           }
         }
       }
+      if (i < nx) lphi[nc + i] = vx[u];
     }
   }
-  if (threadIdx.x == 0) lrow[nc] = ne_b;
-  for (int base = 0; base < nx; base += U * kLdsThreads) {
-    int xs[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int j = base + u * kLdsThreads + threadIdx.x;
-      xs[u] = j < nx ? xslot[x0 + j] : 0;
+  if (!HEX) {
+    for (int k = threadIdx.x; k < nent; k += kLdsThreads) {
+      lanb[k] = pn[k];
+      lnb[k] = pl[k];
     }
-    double vx[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) vx[u] = phi[xs[u]];
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int j = base + u * kLdsThreads + threadIdx.x;
-      if (j < nx) lphi[nc + j] = vx[u];
-    }
+    for (int i = threadIdx.x; i <= nc; i += kLdsThreads) lrow[i] = s.row[c0 + i] - r0;
   }
-  for (int q = threadIdx.x; q <= nq; q += kLdsThreads) lseg[q] = seg[q0 + q] - c0;
+  if (threadIdx.x <= nq) lseg[threadIdx.x] = sg;
   __syncthreads();
-  for (int q = 0; q < nq; q++) {
+  for (int j = 0; j < nq; j++) {
+    const int q = backward ? nq - 1 - j : j;
     const int i1 = lseg[q + 1];
     for (int i = lseg[q] + threadIdx.x; i < i1; i += kLdsThreads) {
       const double a = lap[i];
@@ -685,7 +649,7 @@ k_gs_tiles_lds(SysDesc s, const int *__restrict__ bseg, const int *__restrict__ 
     }
     __syncthreads();
   }
-  for (int i = threadIdx.x; i < nc; i += kLdsThreads) phi[lcell[i]] = lphi[i];
+  for (int i = threadIdx.x; i < nc; i += kLdsThreads) phi[c0 + i] = lphi[i];
 }
 
 // Residual partial sums: block (x, y) covers cells [beg_y + x*chunk, ...).
@@ -909,20 +873,19 @@ struct cfdl_ctx {
   int *xdst = nullptr, *xsrc = nullptr, *rowg = nullptr;
   double *ap_p = nullptr, *b_p = nullptr, *anb_p = nullptr, *phic_p = nullptr;
   std::vector<int> co;  // subdomain cell offsets (nsubd+1)
-  // lvs[s][l]: first cell of forward level l of subdomain s (nlev_s+1 each)
-  std::vector<std::vector<int>> lvs;
+  Sched pcl;            // pc system by forward level (CFDL_GS_MODE=level)
   int pc_nlev = 0;
-  // space-time tiled schedules of the pc sweeps (forward, backward)
+  // space-time tiled schedule of the pc sweeps (the backward sweep runs it
+  // in reverse)
   struct TileSched {
-    std::vector<int> launch_ptr;  // batches of launch m: [launch_ptr[m], launch_ptr[m+1])
-    int *bseg = nullptr, *seg = nullptr, *cells = nullptr;
-    int *brow = nullptr, *growp = nullptr, *xptr = nullptr, *xslot = nullptr;
+    std::vector<int> launch_ptr;   // batches of launch m: [launch_ptr[m], launch_ptr[m+1])
+    std::vector<int> launch_cells;  // cells updated by launch m
+    int *bseg = nullptr, *seg = nullptr, *xptr = nullptr, *xslot = nullptr;
     unsigned short *lnbr = nullptr;
     bool hex = false;  // every row has six entries
     int nbatch = 0, nseg = 0;
     size_t lds_bytes = 0;  // 0: batches too large for LDS, global-memory kernel
-  } tf, tb;
-  std::vector<int> launch_cells[2];  // cells updated by each launch (fwd, bwd)
+  } ts;
   bool tiled = true;
   // control
   GsCtl *gctl = nullptr;
@@ -1053,67 +1016,50 @@ static int sample_end(cfdl_ctx *c, cfdl_ctx::Sample *smp, int launches, int64_t 
   return CFDL_OK;
 }
 
-// A pressure-correction sweep: forward visits levels 0..L-1, backward the
-// same levels in reverse (a cell's neighbours on lower forward levels come
-// earlier in the sequential order, so reversing the level order is a valid
-// backward schedule).
+// A pressure-correction sweep, one launch per dependency level: forward
+// visits the levels in order, backward in reverse (a cell's neighbours on
+// lower forward levels come earlier in the sequential order, so reversing
+// the level order is a valid backward schedule).
 static int pc_sweep(cfdl_ctx *c, const SysDesc &sys, bool forward, const int *active) {
-  const int ns = c->nsubd;
+  const Sched &s = c->pcl;
   cfdl_ctx::Sample *smp = nullptr;
   CALL(sample_begin(c, &smp));
-  int launches = 0;
-  int64_t cells = 0;
-  for (int i = 0; i < c->pc_nlev; i++) {
-    const int l = forward ? i : c->pc_nlev - 1 - i;
-    LevelRanges r{};
-    r.ns = ns;
-    r.pre[0] = 0;
-    for (int q = 0; q < ns; q++) {
-      const std::vector<int> &v = c->lvs[q];
-      const int nl = static_cast<int>(v.size()) - 1;
-      const int lo = l < nl ? v[l] : 0, hi = l < nl ? v[l + 1] : 0;
-      r.lo[q] = lo;
-      r.pre[q + 1] = r.pre[q] + (hi - lo);
-    }
-    const int n = r.pre[ns];
-    if (n == 0) continue;
-    hipLaunchKernelGGL(k_gs_range, dim3(nblocks(n)), dim3(kBlock), 0, c->st, sys, r, 1.02, active);
-    launches++;
-    cells += n;
+  for (int i = 0; i < s.nlev; i++) {
+    const int l = forward ? i : s.nlev - 1 - i;
+    const int n = s.ptr[l + 1] - s.ptr[l];
+    hipLaunchKernelGGL(k_gs_level<1>, dim3(nblocks(n)), dim3(kBlock), 0, c->st, sys,
+                       s.cells + s.ptr[l], n, 1.02, active);
   }
-  CALL(sample_end(c, smp, launches, cells));
-  c->pc_launches += launches;
-  c->pc_cells += cells;
-  c->gs_launches += launches;
+  CALL(sample_end(c, smp, s.nlev, c->ne));
+  c->pc_launches += s.nlev;
+  c->pc_cells += c->ne;
+  c->gs_launches += s.nlev;
   HIPCHK(hipGetLastError());
   return CFDL_OK;
 }
 
 static int pc_sweep_tiled(cfdl_ctx *c, const SysDesc &sys, bool forward, const int *active) {
-  const cfdl_ctx::TileSched &t = forward ? c->tf : c->tb;
-  const std::vector<int> &lc = c->launch_cells[forward ? 0 : 1];
+  const cfdl_ctx::TileSched &t = c->ts;
   const int nl = static_cast<int>(t.launch_ptr.size()) - 1;
+  const int bwd = forward ? 0 : 1;
   cfdl_ctx::Sample *smp = nullptr;
   CALL(sample_begin(c, &smp));
-  int64_t cells = 0;
-  for (int m = 0; m < nl; m++) {
+  for (int i = 0; i < nl; i++) {
+    const int m = forward ? i : nl - 1 - i;
     const int b0 = t.launch_ptr[m], nb = t.launch_ptr[m + 1] - b0;
     if (t.lds_bytes > 0 && t.hex)
       hipLaunchKernelGGL(k_gs_tiles_lds<true>, dim3(nb), dim3(kLdsThreads), t.lds_bytes, c->st,
-                         sys, t.bseg, t.seg, t.cells, t.brow, t.growp, t.lnbr, t.xptr, t.xslot, b0,
-                         1.02, active);
+                         sys, t.bseg, t.seg, t.lnbr, t.xptr, t.xslot, b0, bwd, 1.02, active);
     else if (t.lds_bytes > 0)
       hipLaunchKernelGGL(k_gs_tiles_lds<false>, dim3(nb), dim3(kLdsThreads), t.lds_bytes, c->st,
-                         sys, t.bseg, t.seg, t.cells, t.brow, t.growp, t.lnbr, t.xptr, t.xslot, b0,
-                         1.02, active);
+                         sys, t.bseg, t.seg, t.lnbr, t.xptr, t.xslot, b0, bwd, 1.02, active);
     else
       hipLaunchKernelGGL(k_gs_tiles, dim3(nb), dim3(kTileThreads), 0, c->st, sys, t.bseg, t.seg,
-                         t.cells, b0, 1.02, active);
-    cells += lc[m];
+                         b0, bwd, 1.02, active);
   }
-  CALL(sample_end(c, smp, nl, cells));
+  CALL(sample_end(c, smp, nl, c->ne));
   c->pc_launches += nl;
-  c->pc_cells += cells;
+  c->pc_cells += c->ne;
   c->gs_launches += nl;
   HIPCHK(hipGetLastError());
   return CFDL_OK;
@@ -1234,167 +1180,142 @@ static int multi_subdomain_solver(cfdl_ctx *c) {
   return CFDL_OK;
 }
 
-// Space-time tiled schedule of one sweep direction over the pc system.
-// Cells are binned into spatial tiles (per subdomain, about tile^3 cells
-// each, from the cell centres); in the sweep's processing order each cell
-// gets launch = max over the neighbours it depends on of (their launch, +1
-// when the neighbour lies in another tile).  So a dependency between two
-// tiles always spans launches, and inside one launch a tile's cells only
-// depend on earlier cells of the same tile: the workgroup of that (launch,
-// tile) batch processes them level by level.  Cells of one level are
-// independent (no edge joins two cells of equal forward level).
-static int build_tile_sched(cfdl_ctx *c, const std::vector<int> &rowp, const std::vector<int> &nbrp,
-                            const std::vector<int> &lev, const std::vector<int64_t> &tile,
-                            bool forward, cfdl_ctx::TileSched *ts, std::vector<int> *lcells) {
-  const int ne = c->ne;
-  // processing order: forward = increasing forward level, backward = decreasing
+// Space-time tiles of the pressure-correction sweeps, on the pc system in
+// its set-up order (cells 0..ne-1, halo slots >= ne).  lev is each cell's
+// forward dependency level inside its subdomain and tid its spatial tile
+// (dense ids; a tile never crosses subdomains).  In the forward sweep's
+// processing order (increasing level) each cell gets launch = max over the
+// neighbours it depends on of (their launch, +1 when the neighbour lies in
+// another tile).  So a dependency between two tiles always spans launches,
+// and inside one launch a tile's cells only depend on earlier cells of the
+// same tile.  A (launch, tile) batch spans fewer than `span` levels: a cell
+// that would stretch its batch further moves to the tile's batch in a later
+// launch.
+static void tile_launches(int ne, const std::vector<int> &row, const std::vector<int> &nbr,
+                          const std::vector<int> &lev, const std::vector<int> &tid, int span,
+                          std::vector<int> *launch) {
   std::vector<int> order(ne);
   for (int e = 0; e < ne; e++) order[e] = e;
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-    return forward ? lev[a] < lev[b] : lev[a] > lev[b];
-  });
-  // dense tile ids
-  std::unordered_map<int64_t, int> tid_of;
-  std::vector<int> tid(ne);
-  for (int e = 0; e < ne; e++) {
-    auto it = tid_of.find(tile[e]);
-    if (it == tid_of.end()) it = tid_of.emplace(tile[e], static_cast<int>(tid_of.size())).first;
-    tid[e] = it->second;
-  }
-  const int64_t ntile = static_cast<int64_t>(tid_of.size());
-  // a batch spans at most `span` levels: a cell that would stretch its
-  // batch further moves to the tile's batch in a later launch
-  const char *sp = std::getenv("CFDL_TILE_SPAN");
-  const int span = sp ? std::max(1, std::atoi(sp)) : 16;
-  std::unordered_map<int64_t, int> first_level;  // (launch, tile) -> first level
-  std::vector<int> launch(ne, 0);
-  int nl = 0;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return lev[a] < lev[b]; });
+  int64_t ntile = 1;
+  for (int e = 0; e < ne; e++) ntile = std::max<int64_t>(ntile, tid[e] + 1);
+  std::unordered_map<int64_t, int> first_level;  // (launch, tile) -> its first level
+  launch->assign(ne, 0);
   for (int e : order) {
     int L = 0;
-    for (int k = rowp[e]; k < rowp[e + 1]; k++) {
-      const int d = nbrp[k];
-      if (d >= ne) continue;  // halo: fixed during the sweep
-      // d is a predecessor when the sweep reaches it before e
-      const bool pred = forward ? lev[d] < lev[e] : lev[d] > lev[e];
-      if (!pred) continue;
-      const int v = launch[d] + (tid[d] != tid[e] ? 1 : 0);
-      if (v > L) L = v;
+    for (int k = row[e]; k < row[e + 1]; k++) {
+      const int d = nbr[k];
+      if (d >= ne || lev[d] >= lev[e]) continue;  // halo, or visited after e
+      L = std::max(L, (*launch)[d] + (tid[d] != tid[e] ? 1 : 0));
     }
     for (;;) {
-      auto it = first_level.find(static_cast<int64_t>(L) * ntile + tid[e]);
+      const int64_t key = static_cast<int64_t>(L) * ntile + tid[e];
+      auto it = first_level.find(key);
       if (it == first_level.end()) {
-        first_level.emplace(static_cast<int64_t>(L) * ntile + tid[e], lev[e]);
+        first_level.emplace(key, lev[e]);
         break;
       }
-      if (std::abs(lev[e] - it->second) < span) break;
+      if (lev[e] - it->second < span) break;
       L++;
     }
-    launch[e] = L;
-    nl = std::max(nl, L + 1);
+    (*launch)[e] = L;
   }
-  // group by (launch, tile, level in processing order)
-  std::vector<int> cells(order);
-  std::stable_sort(cells.begin(), cells.end(), [&](int a, int b) {
-    if (launch[a] != launch[b]) return launch[a] < launch[b];
-    if (tid[a] != tid[b]) return tid[a] < tid[b];
-    return forward ? lev[a] < lev[b] : lev[a] > lev[b];
-  });
-  std::vector<int> bseg, seg;
-  ts->launch_ptr.assign(1, 0);
-  lcells->assign(nl, 0);
-  int cur_launch = 0;
-  for (int i = 0; i < ne; i++) {
-    const int e = cells[i];
-    const bool new_batch = i == 0 || launch[e] != launch[cells[i - 1]] || tid[e] != tid[cells[i - 1]];
-    const bool new_seg = new_batch || lev[e] != lev[cells[i - 1]];
-    while (cur_launch < launch[e]) {
-      ts->launch_ptr.push_back(static_cast<int>(bseg.size()));
-      cur_launch++;
-    }
-    if (new_batch) bseg.push_back(static_cast<int>(seg.size()));
-    if (new_seg) seg.push_back(i);
-    (*lcells)[launch[e]]++;
-  }
-  ts->launch_ptr.push_back(static_cast<int>(bseg.size()));
-  bseg.push_back(static_cast<int>(seg.size()));
-  seg.push_back(ne);
-  ts->nbatch = static_cast<int>(bseg.size()) - 1;
-  ts->nseg = static_cast<int>(seg.size()) - 1;
-  // LDS tables: per batch, local slots for every CSR entry (batch cells
-  // first, then distinct outside neighbours)
-  std::vector<int> brow(ne + 1, 0), xptr(ts->nbatch + 1, 0), xslot;
-  std::vector<unsigned short> lnbr;
-  std::vector<int> pos_in_batch(ne, -1);
-  size_t lds_max = 0;
-  bool fits = true;
-  for (int i = 0; i < ne; i++) brow[i + 1] = brow[i] + (rowp[cells[i] + 1] - rowp[cells[i]]);
-  lnbr.resize(brow[ne]);
-  std::vector<int> ext_slot;  // halo/phi index -> local slot, reset per batch
-  std::unordered_map<int, int> ext;
-  for (int bi = 0; bi < ts->nbatch; bi++) {
-    const int c0 = seg[bseg[bi]], c1 = seg[bseg[bi + 1]];
-    for (int i = c0; i < c1; i++) pos_in_batch[cells[i]] = i - c0;
-    ext.clear();
-    const int nc = c1 - c0;
-    for (int i = c0; i < c1; i++) {
-      const int e = cells[i];
-      for (int k = rowp[e]; k < rowp[e + 1]; k++) {
-        const int d = nbrp[k];
-        int slot;
-        if (d < ne && pos_in_batch[d] >= 0) {
-          slot = pos_in_batch[d];
-        } else {
-          auto it = ext.find(d);
-          if (it == ext.end()) {
-            slot = nc + static_cast<int>(ext.size());
-            ext.emplace(d, slot);
-            xslot.push_back(d);
-          } else {
-            slot = it->second;
-          }
-        }
-        if (slot > 65535) fits = false;
-        lnbr[brow[i] + (k - rowp[e])] = static_cast<unsigned short>(slot & 0xffff);
+}
+
+// Batch tables of the tiled sweeps on the renumbered pc system, whose cells
+// are ordered by (subdomain, launch, tile, level): each batch is the run of
+// cells with equal (subdomain, launch, tile), each segment the run of equal
+// level inside it.  Batches are listed launch by launch.  Per batch the LDS
+// kernel also needs the distinct outside neighbours it reads (xslot) and,
+// per CSR entry, the neighbour's LDS slot (lnbr).
+static int build_tile_tables(cfdl_ctx *c, const std::vector<int> &rowp,
+                             const std::vector<int> &nbrp, const std::vector<int> &sd,
+                             const std::vector<int> &launch, const std::vector<int> &tid,
+                             const std::vector<int> &lev) {
+  const int ne = c->ne;
+  cfdl_ctx::TileSched &t = c->ts;
+  struct Batch {
+    int launch, c0, c1, s0, s1;  // cells [c0, c1), level-segment starts sb[s0..s1)
+  };
+  std::vector<Batch> bl;
+  std::vector<int> sb;  // segment starts, batch by batch in cell order
+  for (int e = 0; e < ne; e++) {
+    const bool nb = e == 0 || sd[e] != sd[e - 1] || launch[e] != launch[e - 1] ||
+                    tid[e] != tid[e - 1];
+    if (nb) {
+      if (!bl.empty()) {
+        bl.back().c1 = e;
+        bl.back().s1 = static_cast<int>(sb.size());
       }
+      bl.push_back(Batch{launch[e], e, e, static_cast<int>(sb.size()), 0});
     }
-    for (int i = c0; i < c1; i++) pos_in_batch[cells[i]] = -1;
-    xptr[bi + 1] = static_cast<int>(xslot.size());
-    const size_t nent = brow[c1] - brow[c0], nq = bseg[bi + 1] - bseg[bi];
-    const size_t bytes = 8 * (static_cast<size_t>(nc) * 3 + ext.size() + nent) +
-                         4 * (2 * static_cast<size_t>(nc) + 1 + nq + 1) + 2 * nent;
+    if (nb || lev[e] != lev[e - 1]) sb.push_back(e);
+  }
+  bl.back().c1 = ne;
+  bl.back().s1 = static_cast<int>(sb.size());
+  std::stable_sort(bl.begin(), bl.end(),
+                   [](const Batch &x, const Batch &y) { return x.launch < y.launch; });
+  const int nl = bl.back().launch + 1;
+  t.nbatch = static_cast<int>(bl.size());
+  t.launch_ptr.assign(nl + 1, 0);
+  t.launch_cells.assign(nl, 0);
+  std::vector<int> bseg(t.nbatch + 1, 0), seg, xptr(t.nbatch + 1, 0), xslot;
+  std::vector<unsigned short> lnbr(rowp[ne]);
+  std::unordered_map<int, int> ext;
+  size_t lds_max = 0;
+  bool fits = true, hex = true;
+  for (int e = 0; e < ne; e++) hex = hex && rowp[e + 1] - rowp[e] == 6;
+  for (int i = 0; i < t.nbatch; i++) {
+    const Batch &B = bl[i];
+    t.launch_ptr[B.launch + 1]++;
+    t.launch_cells[B.launch] += B.c1 - B.c0;
+    bseg[i] = static_cast<int>(seg.size());
+    for (int q = B.s0; q < B.s1; q++) seg.push_back(sb[q]);
+    seg.push_back(B.c1);
+    const int nc = B.c1 - B.c0, nq = B.s1 - B.s0;
+    if (nq > 63) fits = false;  // the LDS kernel loads at most 64 bounds
+    ext.clear();
+    for (int k = rowp[B.c0]; k < rowp[B.c1]; k++) {
+      const int d = nbrp[k];
+      int slot;
+      if (d >= B.c0 && d < B.c1) {
+        slot = d - B.c0;
+      } else {
+        auto it = ext.find(d);
+        if (it == ext.end()) {
+          slot = nc + static_cast<int>(ext.size());
+          ext.emplace(d, slot);
+          xslot.push_back(d);
+        } else {
+          slot = it->second;
+        }
+      }
+      if (slot > 65535) fits = false;
+      lnbr[k] = static_cast<unsigned short>(slot & 0xffff);
+    }
+    xptr[i + 1] = static_cast<int>(xslot.size());
+    const size_t nent = rowp[B.c1] - rowp[B.c0];
+    const size_t bytes = 8 * (3 * static_cast<size_t>(nc) + ext.size() + nent) +
+                         4 * (nq + 1 + (hex ? 0 : nc + 1)) + 2 * nent;
     lds_max = std::max(lds_max, bytes);
   }
+  bseg[t.nbatch] = static_cast<int>(seg.size());
+  for (int m = 0; m < nl; m++) t.launch_ptr[m + 1] += t.launch_ptr[m];
+  t.nseg = static_cast<int>(seg.size()) - t.nbatch;
   if (xslot.empty()) xslot.push_back(0);
   if (trace_on()) {
-    int64_t sum_max_seg = 0;
-    for (int m = 0; m + 1 < static_cast<int>(ts->launch_ptr.size()); m++) {
-      int mx = 0;
-      for (int bi = ts->launch_ptr[m]; bi < ts->launch_ptr[m + 1]; bi++) mx = std::max(mx, bseg[bi + 1] - bseg[bi]);
-      sum_max_seg += mx;
-    }
-    std::fprintf(stderr, "cfdl: %s tiled sweep: %d launches, %d batches, %d segments, sum of max "
-                 "segments per launch %lld, largest batch LDS %zu bytes\n",
-                 forward ? "forward" : "backward", nl, ts->nbatch, ts->nseg,
-                 static_cast<long long>(sum_max_seg), lds_max);
+    std::fprintf(stderr, "cfdl: tiled sweep: %d launches, %d batches, %d segments, largest "
+                 "batch LDS %zu bytes\n", nl, t.nbatch, t.nseg, lds_max);
   }
   const char *no_lds = std::getenv("CFDL_TILE_LDS");
   const bool lds_ok = !(no_lds && no_lds[0] == '0');
-  ts->lds_bytes = (lds_ok && fits && lds_max <= 64 * 1024) ? lds_max : 0;
-  CALL(upload(c, &ts->bseg, bseg));
-  CALL(upload(c, &ts->seg, seg));
-  CALL(upload(c, &ts->cells, cells));
-  std::vector<int> growp(ne);
-  bool hex = true;
-  for (int i = 0; i < ne; i++) {
-    growp[i] = rowp[cells[i]];
-    hex = hex && (rowp[cells[i] + 1] - rowp[cells[i]] == 6);
-  }
-  ts->hex = hex;
-  CALL(upload(c, &ts->brow, brow));
-  CALL(upload(c, &ts->growp, growp));
-  CALL(upload(c, &ts->xptr, xptr));
-  CALL(upload(c, &ts->xslot, xslot));
-  CALL(upload(c, &ts->lnbr, lnbr));
+  t.lds_bytes = (lds_ok && fits && lds_max <= 64 * 1024) ? lds_max : 0;
+  t.hex = hex;
+  CALL(upload(c, &t.bseg, bseg));
+  CALL(upload(c, &t.seg, seg));
+  CALL(upload(c, &t.xptr, xptr));
+  CALL(upload(c, &t.xslot, xslot));
+  CALL(upload(c, &t.lnbr, lnbr));
   return CFDL_OK;
 }
 
@@ -1605,30 +1526,65 @@ int cfdl_ctx_create(int ne, int nf, int nbf, const int *ef2nb_idx, const int *ef
         }
       }
     }
-    std::vector<int> lev(ne), o2n(ne), n2o(ne);
-    C->lvs.assign(nsubd, std::vector<int>());
+    // forward dependency level of every cell inside its subdomain
+    std::vector<int> lev(ne);
     C->pc_nlev = 0;
-    for (int s = 0; s < nsubd; s++) {
-      levels(rowo, nbro, C->co[s], C->co[s + 1], true, lev);
-      int nl = 0;
-      for (int q = C->co[s]; q < C->co[s + 1]; q++) nl = std::max(nl, lev[q]);
-      std::vector<int> &st = C->lvs[s];
-      st.assign(nl + 1, 0);
-      for (int q = C->co[s]; q < C->co[s + 1]; q++) st[lev[q]]++;
-      st[0] = C->co[s];
-      for (int l = 1; l <= nl; l++) st[l] += st[l - 1];
-      std::vector<int> pos(st.begin(), st.end() - 1);
-      for (int q = C->co[s]; q < C->co[s + 1]; q++) {
-        o2n[q] = pos[lev[q] - 1]++;
-        n2o[o2n[q]] = q;
+    for (int s = 0; s < nsubd; s++) levels(rowo, nbro, C->co[s], C->co[s + 1], true, lev);
+    for (int q = 0; q < ne; q++) C->pc_nlev = std::max(C->pc_nlev, lev[q]);
+    // spatial tiles of about edge^3 cells from the cell centres, per
+    // subdomain (edge 7 measured fastest on the 128^3 cavity among 5..8)
+    const char *te = std::getenv("CFDL_TILE");
+    const int edge = te ? std::max(1, std::atoi(te)) : 7;
+    const char *sp = std::getenv("CFDL_TILE_SPAN");
+    const int span = sp ? std::min(63, std::max(1, std::atoi(sp))) : 16;
+    const char *mode = std::getenv("CFDL_GS_MODE");
+    C->tiled = !(mode && std::string(mode) == "level");
+    std::vector<int> tid(ne);
+    {
+      std::unordered_map<int64_t, int> tid_of;
+      for (int s = 0; s < nsubd; s++) {
+        double lo[3] = {1e300, 1e300, 1e300}, vsum = 0.0;
+        for (int q = C->co[s]; q < C->co[s + 1]; q++) {
+          const int g = l2g[q];
+          const double r[3] = {xc[g], yc[g], zc[g]};
+          for (int i = 0; i < 3; i++) lo[i] = std::min(lo[i], r[i]);
+          vsum += vol[g];
+        }
+        const double h = std::cbrt(vsum / (C->co[s + 1] - C->co[s])) * edge;
+        for (int q = C->co[s]; q < C->co[s + 1]; q++) {
+          const int g = l2g[q];
+          const int64_t bx = static_cast<int64_t>((xc[g] - lo[0]) / h);
+          const int64_t by = static_cast<int64_t>((yc[g] - lo[1]) / h);
+          const int64_t bz = static_cast<int64_t>((zc[g] - lo[2]) / h);
+          const int64_t key = ((static_cast<int64_t>(s) * 4096 + bx) * 4096 + by) * 4096 + bz;
+          auto it = tid_of.find(key);
+          if (it == tid_of.end()) it = tid_of.emplace(key, static_cast<int>(tid_of.size())).first;
+          tid[q] = it->second;
+        }
       }
-      C->pc_nlev = std::max(C->pc_nlev, nl);
     }
-    std::vector<int> rowp(ne + 1, 0), nbrp, l2gn(ne);
+    std::vector<int> launch;
+    tile_launches(ne, rowo, nbro, lev, tid, span, &launch);
+    // renumber the pc system by (subdomain, launch, tile, level): every
+    // (launch, tile) batch of the tiled sweeps becomes a contiguous range
+    std::vector<int> n2o(ne), o2n(ne);
+    for (int q = 0; q < ne; q++) n2o[q] = q;
+    std::stable_sort(n2o.begin(), n2o.end(), [&](int x, int y) {
+      if (sd_of[l2g[x]] != sd_of[l2g[y]]) return sd_of[l2g[x]] < sd_of[l2g[y]];
+      if (launch[x] != launch[y]) return launch[x] < launch[y];
+      if (tid[x] != tid[y]) return tid[x] < tid[y];
+      return lev[x] < lev[y];
+    });
+    for (int nq = 0; nq < ne; nq++) o2n[n2o[nq]] = nq;
+    std::vector<int> rowp(ne + 1, 0), nbrp, l2gn(ne), sdn(ne), launchn(ne), tidn(ne), levn(ne);
     nbrp.reserve(nbro.size());
     for (int nq = 0; nq < ne; nq++) {
       const int q = n2o[nq];
       l2gn[nq] = l2g[q];
+      sdn[nq] = sd_of[l2g[q]];
+      launchn[nq] = launch[q];
+      tidn[nq] = tid[q];
+      levn[nq] = lev[q];
       rowp[nq + 1] = rowp[nq] + (rowo[q + 1] - rowo[q]);
       for (int k = rowo[q]; k < rowo[q + 1]; k++) {
         const int v = nbro[k];
@@ -1636,36 +1592,8 @@ int cfdl_ctx_create(int ne, int nf, int nbf, const int *ef2nb_idx, const int *ef
       }
     }
     for (auto &v : xsrc) v = o2n[v];
-    {
-      // tiles of about edge^3 cells from the cell centres, per subdomain
-      // (edge 7 measured fastest on the 128^3 cavity among 5..8)
-      const char *te = std::getenv("CFDL_TILE");
-      const int edge = te ? std::max(1, std::atoi(te)) : 7;
-      const char *mode = std::getenv("CFDL_GS_MODE");
-      C->tiled = !(mode && std::string(mode) == "level");
-      std::vector<int> levn(ne);
-      std::vector<int64_t> tile(ne);
-      for (int s = 0; s < nsubd; s++) {
-        double lo[3] = {1e300, 1e300, 1e300}, vsum = 0.0;
-        for (int nq = C->co[s]; nq < C->co[s + 1]; nq++) {
-          const int g = l2gn[nq];
-          const double r[3] = {xc[g], yc[g], zc[g]};
-          for (int i = 0; i < 3; i++) lo[i] = std::min(lo[i], r[i]);
-          vsum += vol[g];
-        }
-        const double h = std::cbrt(vsum / (C->co[s + 1] - C->co[s])) * edge;
-        for (int nq = C->co[s]; nq < C->co[s + 1]; nq++) {
-          const int g = l2gn[nq];
-          const int64_t bx = static_cast<int64_t>((xc[g] - lo[0]) / h);
-          const int64_t by = static_cast<int64_t>((yc[g] - lo[1]) / h);
-          const int64_t bz = static_cast<int64_t>((zc[g] - lo[2]) / h);
-          tile[nq] = ((static_cast<int64_t>(s) * 4096 + bx) * 4096 + by) * 4096 + bz;
-          levn[nq] = lev[n2o[nq]];
-        }
-      }
-      CALL(build_tile_sched(C, rowp, nbrp, levn, tile, true, &C->tf, &C->launch_cells[0]));
-      CALL(build_tile_sched(C, rowp, nbrp, levn, tile, false, &C->tb, &C->launch_cells[1]));
-    }
+    CALL(build_tile_tables(C, rowp, nbrp, sdn, launchn, tidn, levn));
+    CALL(make_sched(C, levn, ne, &C->pcl));
     C->Lp = static_cast<int>(nbrp.size());
     C->nh = nh;
     C->nx = static_cast<int>(xdst.size());

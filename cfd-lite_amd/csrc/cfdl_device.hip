@@ -1309,7 +1309,18 @@ static int build_tile_tables(cfdl_ctx *c, const std::vector<int> &rowp,
   }
   const char *no_lds = std::getenv("CFDL_TILE_LDS");
   const bool lds_ok = !(no_lds && no_lds[0] == '0');
-  t.lds_bytes = (lds_ok && fits && lds_max <= 64 * 1024) ? lds_max : 0;
+  // a workgroup may use the device's whole per-block LDS (160 KB on gfx950)
+  // once the kernel's dynamic-LDS limit is raised to it
+  int lds_dev = 0;
+  HIPCHK(hipDeviceGetAttribute(&lds_dev, hipDeviceAttributeMaxSharedMemoryPerBlock, c->dev));
+  size_t lds_cap = 64 * 1024;
+  if (static_cast<size_t>(lds_dev) > lds_cap && lds_max > lds_cap &&
+      hipFuncSetAttribute(reinterpret_cast<const void *>(hex ? k_gs_tiles_lds<true>
+                                                             : k_gs_tiles_lds<false>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, lds_dev) == hipSuccess)
+    lds_cap = static_cast<size_t>(lds_dev);
+  (void)hipGetLastError();
+  t.lds_bytes = (lds_ok && fits && lds_max <= lds_cap) ? lds_max : 0;
   t.hex = hex;
   CALL(upload(c, &t.bseg, bseg));
   CALL(upload(c, &t.seg, seg));
@@ -1532,9 +1543,9 @@ int cfdl_ctx_create(int ne, int nf, int nbf, const int *ef2nb_idx, const int *ef
     for (int s = 0; s < nsubd; s++) levels(rowo, nbro, C->co[s], C->co[s + 1], true, lev);
     for (int q = 0; q < ne; q++) C->pc_nlev = std::max(C->pc_nlev, lev[q]);
     // spatial tiles of about edge^3 cells from the cell centres, per
-    // subdomain (edge 7 measured fastest on the 128^3 cavity among 5..8)
+    // subdomain (edge 8 measured fastest on the 128^3 cavity among 5..12)
     const char *te = std::getenv("CFDL_TILE");
-    const int edge = te ? std::max(1, std::atoi(te)) : 7;
+    const int edge = te ? std::max(1, std::atoi(te)) : 8;
     const char *sp = std::getenv("CFDL_TILE_SPAN");
     const int span = sp ? std::min(63, std::max(1, std::atoi(sp))) : 16;
     const char *mode = std::getenv("CFDL_GS_MODE");

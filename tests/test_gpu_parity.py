@@ -72,7 +72,9 @@ def test_relaxation_config_mode_bitwise():
     assert_logs_match(eq.log(), oc.log())
 
 
-def test_32cube_one_step():
+@pytest.mark.parametrize("mode", ["tiled", "level"])
+def test_32cube_one_step(monkeypatch, mode):
+    monkeypatch.setenv("CFDL_GS_MODE", mode)
     eq, oc = make_pair(32, 0.0, 4, mu=0.0025)
     eq.run(1, 2)
     oc.run(1, 2)
@@ -157,16 +159,19 @@ def test_set_get_roundtrip_and_errors():
         eq.set_relaxation(0.0, 1.0)
 
 
-@pytest.mark.parametrize("tile,mode,lds", [("2", "tiled", "1"), ("3", "tiled", "1"),
-                                           ("3", "tiled", "0"), ("8", "tiled", "1"),
-                                           ("8", "level", "1")])
-def test_sweep_schedules_bitwise(monkeypatch, tile, mode, lds):
-    """The pressure-correction sweeps run either as one launch per dependency
-    level or as space-time tiles; small tiles put many tile boundaries and
-    launches into small meshes.  Every schedule gives the oracle's bits."""
+@pytest.mark.parametrize("tile,mode,lds,span", [
+    ("2", "tiled", "1", "16"), ("3", "tiled", "1", "3"), ("3", "tiled", "0", "16"),
+    ("8", "tiled", "1", "16"), ("8", "level", "1", "16"), ("12", "tiled", "1", "32")])
+def test_sweep_schedules_bitwise(monkeypatch, tile, mode, lds, span):
+    """The pressure-correction sweeps run as one launch per dependency level
+    or as space-time tiles launched batch by batch (in LDS, or from global
+    memory with CFDL_TILE_LDS=0).  Small tiles and spans put many tile
+    boundaries into small meshes; tile 12 with span 32 needs more than 64 KB
+    of LDS per batch.  Every schedule gives the oracle's bits."""
     monkeypatch.setenv("CFDL_TILE", tile)
     monkeypatch.setenv("CFDL_GS_MODE", mode)
     monkeypatch.setenv("CFDL_TILE_LDS", lds)
+    monkeypatch.setenv("CFDL_TILE_SPAN", span)
     eq, oc = make_pair(12, 0.15, 4)
     eq.run(1, 2)
     oc.run(1, 2)

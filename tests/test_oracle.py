@@ -298,3 +298,78 @@ def test_relaxation_one_is_reference_mode():
     c.set_relaxation(0.7, 0.3)
     c.run(1, 2)
     assert not np.array_equal(ref.arr("u"), c.arr("u"))
+
+
+def _residuals(oc, names=("u", "v", "w"), rhs=("bu", "bv", "bw")):
+    """b - (ap*phi - sum anb*phi_nb) per cell, for each momentum component."""
+    c = oc.c
+    ne, L = c.ne, c.L
+    idx, ap, anb = oc.arr("ef2nb_idx"), oc.arr("ap"), oc.arr("anb")
+    nb = (oc.arr("ef2nb")[:L] >> 5) - 1
+    row = np.repeat(np.arange(ne), np.diff(idx))
+    out = []
+    for name, bn in zip(names, rhs):
+        phi = oc.arr(name)
+        s = np.zeros(ne)
+        np.add.at(s, row, anb * phi[nb])
+        out.append(oc.arr(bn) - (ap * phi[:ne] - s))
+    return out
+
+
+def test_momentum_relaxation_keeps_the_residual_of_the_current_iterate():
+    """Config mode, momentum: Patankar's implicit under-relaxation divides ap
+    by alpha_u and adds (ap/alpha_u - ap)*phi to b, so the residual of the
+    current iterate is unchanged (a converged field stays converged) while the
+    diagonal grows.  The neighbour coefficients are untouched, and d and dc
+    follow the relaxed ap (d = V/ap, dc = V/(ap - sum anb))."""
+    cases = []
+    for alpha in (1.0, 0.7):
+        oc = assembled(n=6, jit=0.15, steps=(1, 2))
+        lib = oracle_lib.lib()
+        lib.or_update_boundaries(oc.p)
+        oc.set_relaxation(alpha, 1.0)
+        lib.or_calc_coef_uvw(oc.p)
+        cases.append(oc)
+    a, b = cases
+    assert np.array_equal(a.arr("anb"), b.arr("anb"))
+    assert np.array_equal(b.arr("ap"), a.arr("ap") / 0.7)
+    ra, rb = _residuals(a), _residuals(b)
+    for x, y, bn in zip(ra, rb, ("bu", "bv", "bw")):
+        scale = np.abs(a.arr(bn)).max() + np.abs(a.arr("ap")).max() * 1e-3
+        assert np.allclose(x, y, rtol=0, atol=1e-12 * scale)
+    c = b.c
+    idx, anb, vol = b.arr("ef2nb_idx"), b.arr("anb"), b.arr("vol")
+    rows = np.add.reduceat(anb, idx[:-1] - 1)
+    assert np.allclose(b.arr("d") * b.arr("ap"), vol, rtol=1e-14)
+    assert np.allclose(b.arr("dc") * (b.arr("ap") - rows), vol, rtol=1e-12)
+    assert c.ne == len(vol)
+
+
+def test_pressure_relaxation_scales_the_correction():
+    """Config mode, pressure: update_uvwp adds alpha_p*pc to p and
+    alpha_p*gpc to gp, and corrects the face mass fluxes with the whole pc
+    (continuity is enforced in full; only the pressure is relaxed)."""
+    rng = np.random.default_rng(7)
+    cases = []
+    pc = None
+    for alpha in (1.0, 0.3):
+        oc = assembled(n=5, jit=0.1, steps=(1, 2))
+        c = oc.c
+        n = c.ne + c.nbf
+        if pc is None:
+            pc = rng.standard_normal(n)
+            gpc = rng.standard_normal(3 * n)
+        oc.arr("gpc")[:] = gpc
+        p0, gp0 = oc.arr("p").copy(), oc.arr("gp").copy()
+        oc.set_relaxation(1.0, alpha)
+        oracle_lib.lib().or_update_uvwp(oc.p, pc.ctypes.data_as(oracle_lib.PD))
+        cases.append((oc, p0, gp0))
+    (a, pa, ga), (b, pb, gb) = cases
+    ne = a.c.ne
+    assert np.array_equal(pa, pb) and np.array_equal(ga, gb)
+    assert np.allclose(b.arr("p")[:ne] - pb[:ne], 0.3 * (a.arr("p")[:ne] - pa[:ne]),
+                       rtol=1e-12, atol=1e-14)
+    assert np.allclose(b.arr("gp")[:3 * ne] - gb[:3 * ne],
+                       0.3 * (a.arr("gp")[:3 * ne] - ga[:3 * ne]), rtol=1e-12, atol=1e-14)
+    assert np.array_equal(a.arr("mip"), b.arr("mip"))
+    assert np.array_equal(a.arr("p")[ne:], pa[ne:])  # halos are not touched

@@ -548,7 +548,7 @@ k_gs_tiles(SysDesc s, const int *__restrict__ bseg, const int *__restrict__ seg,
 // first LDS store, so the gather costs two memory latencies (the outside
 // indices, then their values).  The segment count is at most 63 (host).
 constexpr int kLdsThreads = 64;
-template <bool HEX>
+template <bool HEX, bool PIPE>
 __global__ void __launch_bounds__(kLdsThreads)
 k_gs_tiles_lds(SysDesc s, const int *__restrict__ bseg, const int *__restrict__ seg,
                const unsigned short *__restrict__ lnbr, const int *__restrict__ xptr,
@@ -626,6 +626,68 @@ k_gs_tiles_lds(SysDesc s, const int *__restrict__ bseg, const int *__restrict__ 
   }
   if (threadIdx.x <= nq) lseg[threadIdx.x] = sg;
   __syncthreads();
+  if (PIPE) {
+    // Every segment has at most 64 cells: lane l updates cell lseg[q] + l of
+    // segment q.  The operands that do not change during the sweep (ap, b,
+    // anb, slots) and the cell's own old value are loaded for the next
+    // segment while the current one computes, so after each barrier only
+    // the neighbour values are read from LDS.  Lanes without a cell load
+    // row 0 and store nothing.  Segment bounds come from lane q of sg.
+    const int lane = threadIdx.x;
+    const double sm1 = sor - 1.0;
+    int q = backward ? nq - 1 : 0;
+    int i = __builtin_amdgcn_readlane(sg, q) + lane;
+    bool on = i < __builtin_amdgcn_readlane(sg, q + 1);
+    int ic = on ? i : 0;
+    double a = lap[ic], bv = lb[ic], t = sm1 * a * lphi[ic], an[6];
+    int sl[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      sl[k] = lnb[6 * ic + k];
+      an[k] = lanb[6 * ic + k];
+    }
+    for (int j = 0; j < nq; j++) {
+      double v[6];
+#pragma unroll
+      for (int k = 0; k < 6; k++) v[k] = lphi[sl[k]];
+      // the next segment's operands (its cells are not written this level)
+      const int qn = backward ? q - 1 : q + 1;
+      int in = 0;
+      bool onn = false;
+      if (j + 1 < nq) {
+        in = __builtin_amdgcn_readlane(sg, qn) + lane;
+        onn = in < __builtin_amdgcn_readlane(sg, qn + 1);
+      }
+      const int icn = onn ? in : 0;
+      const double a2 = lap[icn], b2 = lb[icn], p2 = lphi[icn];
+      double an2[6];
+      int sl2[6];
+#pragma unroll
+      for (int k = 0; k < 6; k++) {
+        sl2[k] = lnb[6 * icn + k];
+        an2[k] = lanb[6 * icn + k];
+      }
+      double sum = bv;
+#pragma unroll
+      for (int k = 0; k < 6; k++) sum = sum + an[k] * v[k];
+      const double nv = (sum + t) / a / sor;
+      if (on) lphi[i] = nv;
+      __syncthreads();
+      q = qn;
+      i = in;
+      on = onn;
+      a = a2;
+      bv = b2;
+      t = sm1 * a2 * p2;
+#pragma unroll
+      for (int k = 0; k < 6; k++) {
+        sl[k] = sl2[k];
+        an[k] = an2[k];
+      }
+    }
+    for (int r = threadIdx.x; r < nc; r += kLdsThreads) phi[c0 + r] = lphi[r];
+    return;
+  }
   for (int j = 0; j < nq; j++) {
     const int q = backward ? nq - 1 - j : j;
     const int i1 = lseg[q + 1];
@@ -882,7 +944,8 @@ struct cfdl_ctx {
     std::vector<int> launch_cells;  // cells updated by launch m
     int *bseg = nullptr, *seg = nullptr, *xptr = nullptr, *xslot = nullptr;
     unsigned short *lnbr = nullptr;
-    bool hex = false;  // every row has six entries
+    bool hex = false;   // every row has six entries
+    bool pipe = false;  // hex, and no level segment has more than 64 cells
     int nbatch = 0, nseg = 0;
     size_t lds_bytes = 0;  // 0: batches too large for LDS, global-memory kernel
   } ts;
@@ -1047,12 +1110,18 @@ static int pc_sweep_tiled(cfdl_ctx *c, const SysDesc &sys, bool forward, const i
   for (int i = 0; i < nl; i++) {
     const int m = forward ? i : nl - 1 - i;
     const int b0 = t.launch_ptr[m], nb = t.launch_ptr[m + 1] - b0;
-    if (t.lds_bytes > 0 && t.hex)
-      hipLaunchKernelGGL(k_gs_tiles_lds<true>, dim3(nb), dim3(kLdsThreads), t.lds_bytes, c->st,
-                         sys, t.bseg, t.seg, t.lnbr, t.xptr, t.xslot, b0, bwd, 1.02, active);
+    if (t.lds_bytes > 0 && t.pipe)
+      hipLaunchKernelGGL((k_gs_tiles_lds<true, true>), dim3(nb), dim3(kLdsThreads), t.lds_bytes,
+                         c->st, sys, t.bseg, t.seg, t.lnbr, t.xptr, t.xslot, b0, bwd, 1.02,
+                         active);
+    else if (t.lds_bytes > 0 && t.hex)
+      hipLaunchKernelGGL((k_gs_tiles_lds<true, false>), dim3(nb), dim3(kLdsThreads), t.lds_bytes,
+                         c->st, sys, t.bseg, t.seg, t.lnbr, t.xptr, t.xslot, b0, bwd, 1.02,
+                         active);
     else if (t.lds_bytes > 0)
-      hipLaunchKernelGGL(k_gs_tiles_lds<false>, dim3(nb), dim3(kLdsThreads), t.lds_bytes, c->st,
-                         sys, t.bseg, t.seg, t.lnbr, t.xptr, t.xslot, b0, bwd, 1.02, active);
+      hipLaunchKernelGGL((k_gs_tiles_lds<false, false>), dim3(nb), dim3(kLdsThreads), t.lds_bytes,
+                         c->st, sys, t.bseg, t.seg, t.lnbr, t.xptr, t.xslot, b0, bwd, 1.02,
+                         active);
     else
       hipLaunchKernelGGL(k_gs_tiles, dim3(nb), dim3(kTileThreads), 0, c->st, sys, t.bseg, t.seg,
                          b0, bwd, 1.02, active);
@@ -1263,6 +1332,7 @@ static int build_tile_tables(cfdl_ctx *c, const std::vector<int> &rowp,
   std::vector<unsigned short> lnbr(rowp[ne]);
   std::unordered_map<int, int> ext;
   size_t lds_max = 0;
+  int seg_max = 0;
   bool fits = true, hex = true;
   for (int e = 0; e < ne; e++) hex = hex && rowp[e + 1] - rowp[e] == 6;
   for (int i = 0; i < t.nbatch; i++) {
@@ -1270,7 +1340,10 @@ static int build_tile_tables(cfdl_ctx *c, const std::vector<int> &rowp,
     t.launch_ptr[B.launch + 1]++;
     t.launch_cells[B.launch] += B.c1 - B.c0;
     bseg[i] = static_cast<int>(seg.size());
-    for (int q = B.s0; q < B.s1; q++) seg.push_back(sb[q]);
+    for (int q = B.s0; q < B.s1; q++) {
+      seg.push_back(sb[q]);
+      seg_max = std::max(seg_max, (q + 1 < B.s1 ? sb[q + 1] : B.c1) - sb[q]);
+    }
     seg.push_back(B.c1);
     const int nc = B.c1 - B.c0, nq = B.s1 - B.s0;
     if (nq > 63) fits = false;  // the LDS kernel loads at most 64 bounds
@@ -1304,8 +1377,9 @@ static int build_tile_tables(cfdl_ctx *c, const std::vector<int> &rowp,
   t.nseg = static_cast<int>(seg.size()) - t.nbatch;
   if (xslot.empty()) xslot.push_back(0);
   if (trace_on()) {
-    std::fprintf(stderr, "cfdl: tiled sweep: %d launches, %d batches, %d segments, largest "
-                 "batch LDS %zu bytes\n", nl, t.nbatch, t.nseg, lds_max);
+    std::fprintf(stderr, "cfdl: tiled sweep: %d launches, %d batches, %d segments (largest "
+                 "%d cells), largest batch LDS %zu bytes\n", nl, t.nbatch, t.nseg, seg_max,
+                 lds_max);
   }
   const char *no_lds = std::getenv("CFDL_TILE_LDS");
   const bool lds_ok = !(no_lds && no_lds[0] == '0');
@@ -1315,13 +1389,18 @@ static int build_tile_tables(cfdl_ctx *c, const std::vector<int> &rowp,
   HIPCHK(hipDeviceGetAttribute(&lds_dev, hipDeviceAttributeMaxSharedMemoryPerBlock, c->dev));
   size_t lds_cap = 64 * 1024;
   if (static_cast<size_t>(lds_dev) > lds_cap && lds_max > lds_cap &&
-      hipFuncSetAttribute(reinterpret_cast<const void *>(hex ? k_gs_tiles_lds<true>
-                                                             : k_gs_tiles_lds<false>),
+      hipFuncSetAttribute(reinterpret_cast<const void *>(&k_gs_tiles_lds<true, true>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, lds_dev) == hipSuccess &&
+      hipFuncSetAttribute(reinterpret_cast<const void *>(&k_gs_tiles_lds<true, false>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, lds_dev) == hipSuccess &&
+      hipFuncSetAttribute(reinterpret_cast<const void *>(&k_gs_tiles_lds<false, false>),
                           hipFuncAttributeMaxDynamicSharedMemorySize, lds_dev) == hipSuccess)
     lds_cap = static_cast<size_t>(lds_dev);
   (void)hipGetLastError();
   t.lds_bytes = (lds_ok && fits && lds_max <= lds_cap) ? lds_max : 0;
   t.hex = hex;
+  const char *pp = std::getenv("CFDL_TILE_PIPE");
+  t.pipe = hex && seg_max <= kLdsThreads && !(pp && pp[0] == '0');
   CALL(upload(c, &t.bseg, bseg));
   CALL(upload(c, &t.seg, seg));
   CALL(upload(c, &t.xptr, xptr));

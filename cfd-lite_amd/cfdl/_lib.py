@@ -32,6 +32,9 @@ _SIGS = {
     "cfdl_mesh_get_i32": (C.c_int, [VP, C.c_char_p, PI, C.c_int64]),
     "cfdl_mesh_get_f64": (C.c_int, [VP, C.c_char_p, PD, C.c_int64]),
     "cfdl_mesh_free": (None, [VP]),
+    "cfdl_mesh_set_section_names": (C.c_int, [VP, C.c_int, C.POINTER(C.c_char_p)]),
+    "cfdl_mesh_write_vtu": (C.c_int, [VP, C.c_char_p, C.c_int, C.POINTER(C.c_char_p), PI,
+                                      C.POINTER(PD)]),
     "cfdl_ctx_create": (C.c_int, [C.c_int, C.c_int, C.c_int, PI, PI, PI, PI, PD, PD, PD, PD, PD, PD,
                                   C.c_int, PI, PI, C.c_int, PI, PI, PI, PD, PD, C.c_double, C.c_int,
                                   C.POINTER(VP)]),
@@ -58,6 +61,13 @@ _SIGS = {
     "cfdl_update_uvwp": (C.c_int, [VP]),
     "cfdl_solve_uvwp": (C.c_int, [VP]),
     "cfdl_update_time": (C.c_int, [VP]),
+    "cfdl_calc_mass_imbalance": (C.c_int, [VP, C.c_int]),
+    "cfdl_set_thermal": (C.c_int, [VP, PD, PD]),
+    "cfdl_construct_energy": (C.c_int, [VP]),
+    "cfdl_update_boundaries_energy": (C.c_int, [VP]),
+    "cfdl_calc_coef_energy": (C.c_int, [VP]),
+    "cfdl_solve_energy": (C.c_int, [VP]),
+    "cfdl_run_energy": (C.c_int, [VP, C.c_int, C.c_int]),
     "cfdl_run": (C.c_int, [VP, C.c_int, C.c_int]),
     "cfdl_log_count": (C.c_int, [VP, PI]),
     "cfdl_log_get": (C.c_int, [VP, C.c_int, C.c_char_p, PI, PD, PD, PD]),

@@ -46,6 +46,9 @@ class Mesh:
         check(lib.cfdl_mesh_from_elements(len(x), x.ctypes.data_as(PD), y.ctypes.data_as(PD),
                                           z.ctypes.data_as(PD), len(et), et.ctypes.data_as(PI),
                                           es.ctypes.data_as(PI), ev.ctypes.data_as(PI), C.byref(h)))
+        if names is not None:
+            arr = (C.c_char_p * len(names))(*[n.encode() for n in names])
+            check(lib.cfdl_mesh_set_section_names(h, len(names), arr))
         return cls(h, names)
 
     def partition(self, nsubd):
@@ -67,6 +70,27 @@ class Mesh:
                 "g2gf": ne if nsubd else 0, "g2gf_idx": nsubd + 1 if nsubd else 0,
                 "xc": ne + nbf, "yc": ne + nbf, "zc": ne + nbf, "vol": ne, "aip": 3 * nf,
                 "rip": 3 * nf, "x": nvx, "y": nvx, "z": nvx}[name]
+
+    def write_vtu(self, path, arrays):
+        """VTU file of cell arrays in the reference's write_vtubin layout
+        (src/modules/mod_vtu_output.f90:6-326).  arrays: [(name, values)] or
+        [(name, values, ncomp)], values in cell order with at least ne*ncomp
+        entries (ncomp 1 by default; extra halo entries are not written)."""
+        ne = self.info()["ne"]
+        keep, names, ncomp = [], [], []
+        for item in arrays:
+            name, a = item[0], np.ascontiguousarray(item[1], dtype=np.float64)
+            k = item[2] if len(item) > 2 else 1
+            if a.size < k * ne:
+                raise ValueError("%s: %d values for %d cells" % (name, a.size, ne))
+            keep.append(a)
+            names.append(name.encode())
+            ncomp.append(k)
+        nv = len(keep)
+        cn = (C.c_char_p * max(nv, 1))(*names)
+        nc = (C.c_int * max(nv, 1))(*ncomp)
+        dp = (PD * max(nv, 1))(*[a.ctypes.data_as(PD) for a in keep])
+        check(self._lib.cfdl_mesh_write_vtu(self._h, str(path).encode(), nv, cn, nc, dp))
 
     def get(self, name):
         n = self._size(name)

@@ -16,7 +16,8 @@ BC_WALL, BC_LID, BC_SYMMETRY = 0, 1, 2
 
 FIELDS = {name: k for k, name in enumerate(
     ["u", "v", "w", "p", "u0", "v0", "w0", "gu", "gv", "gw", "gp", "gpc", "mip", "mip0",
-     "bu", "bv", "bw", "d", "dc", "ap", "b", "anb", "phic", "work", "gwork"])}
+     "bu", "bv", "bw", "d", "dc", "ap", "b", "anb", "phic", "work", "gwork",
+     "t", "gt", "h", "h0", "gh"])}
 
 
 def _fid(name):
@@ -40,9 +41,9 @@ class UVWP:
     def _sizes(ne, nf, nbf):
         n, L = ne + nbf, 2 * nf - nbf
         s = {}
-        for k in ("u", "v", "w", "p", "u0", "v0", "w0", "phic", "work"):
+        for k in ("u", "v", "w", "p", "u0", "v0", "w0", "phic", "work", "t", "h", "h0"):
             s[k] = n
-        for k in ("gu", "gv", "gw", "gp", "gpc", "gwork"):
+        for k in ("gu", "gv", "gw", "gp", "gpc", "gwork", "gt", "gh"):
             s[k] = 3 * n
         for k in ("mip", "mip0"):
             s[k] = nf
@@ -142,8 +143,54 @@ class UVWP:
     def update_time(self):
         check(self._lib.cfdl_update_time(self._h))
 
-    def run(self, ntstep, ncoef):
-        check(self._lib.cfdl_run(self._h, ntstep, ncoef))
+    def mass_imbalance(self):
+        """Net outward mass flux per cell (ne values), computed on the GPU."""
+        check(self._lib.cfdl_calc_mass_imbalance(self._h, FIELDS["work"]))
+        return self.get("work")[:self.sizes["ap"]]
+
+    def write_vtu(self, mesh, project, step):
+        """The reference's end-of-run output of the uvwp equation
+        (src/main.f90:89 -> write_vtubin): project/VTK/output-<step>/uvwp.vtu
+        with u, v, w, p, gpc and the cell mass imbalance as 'mip'.  Returns
+        the file path."""
+        import os
+        d = os.path.join(project, "VTK", "output-%d" % step)
+        os.makedirs(d, exist_ok=True)
+        path = os.path.join(d, "uvwp.vtu")
+        ne = self.sizes["ap"]
+        mesh.write_vtu(path, [("u", self.get("u")[:ne]), ("v", self.get("v")[:ne]),
+                              ("w", self.get("w")[:ne]), ("p", self.get("p")[:ne]),
+                              ("gpc", self.get("gpc")[:3 * ne], 3),
+                              ("mip", self.mass_imbalance())])
+        return path
+
+    def run(self, ntstep, ncoef, energy=False):
+        """ntstep x ncoef outer iterations (src/main.f90:50-63); energy=True
+        also solves the energy equation after every solve_uvwp."""
+        if energy:
+            check(self._lib.cfdl_run_energy(self._h, ntstep, ncoef))
+        else:
+            check(self._lib.cfdl_run(self._h, ntstep, ncoef))
+
+    # energy equation, src/equations/mod_energy.f90
+    def set_thermal(self, cp, tc):
+        cp = np.ascontiguousarray(cp, np.float64)
+        tc = np.ascontiguousarray(tc, np.float64)
+        if cp.size != self.sizes["ap"] or tc.size != self.sizes["ap"]:
+            raise CfdlError("cp and tc need one value per cell")
+        check(self._lib.cfdl_set_thermal(self._h, cp.ctypes.data_as(PD), tc.ctypes.data_as(PD)))
+
+    def construct_energy(self):
+        check(self._lib.cfdl_construct_energy(self._h))
+
+    def update_boundaries_energy(self):
+        check(self._lib.cfdl_update_boundaries_energy(self._h))
+
+    def calc_coef_energy(self):
+        check(self._lib.cfdl_calc_coef_energy(self._h))
+
+    def solve_energy(self):
+        check(self._lib.cfdl_solve_energy(self._h))
 
     def synchronize(self):
         check(self._lib.cfdl_synchronize(self._h))

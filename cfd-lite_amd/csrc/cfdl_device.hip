@@ -95,6 +95,13 @@ struct MeshDev {
   int ne, nf, nbf, L;
   const int *row, *nbr, *fsg, *own, *fcell, *bcrow, *bcslot, *hcell, *hslot, *hkind;
   const double *xc, *yc, *zc, *vol, *aip, *rip, *rho, *mu;
+  const double *cp, *tc;  // thermal properties of the energy equation
+};
+
+// energy equation state (energy_t, mod_energy.f90:4-10): temperature t and
+// its gradient gt, the enthalpy h it solves for (phi), h0, gradient gh
+struct EnergyDev {
+  double *t, *gt, *h, *h0, *gh;
 };
 
 struct FieldsDev {
@@ -132,7 +139,37 @@ struct SysDesc {
   const double *b[kMaxSys];
   const int *row, *nbr;
   const double *ap, *anb;
+  const double *rap;  // 1/ap per cell (tiled LDS sweeps), correctly rounded
 };
+
+// x / a, correctly rounded, from y = 1/a correctly rounded: q0 = x*y is
+// corrected once by the exact remainder, q0 + (x - a*q0)*y (Markstein's
+// theorem), with the sign of q0 (a zero quotient keeps its sign).  This
+// takes the division off the Gauss-Seidel critical path, since y is known
+// before the sweep.  The result is exact while q0, the remainder and r*y
+// stay normal: |q0| in [2^-960, 2^960], or q0 = 0 (then r = 0 and the sign
+// comes from q0).  Callers keep the operands inside that range (rcp_of and
+// kRcpExp) and fall back to the hardware division outside it.
+__device__ inline double div_rcp(double x, double a, double y) {
+  const double q0 = x * y;
+  const double r = fma(-a, q0, x);
+  return __builtin_copysign(fma(r, y, q0), q0);
+}
+
+// Operand ranges of the reciprocal division: divisors (ap, sor) with
+// |a| in [2^-64, 2^64] and dividends x = 0 or with frexp exponent in
+// [-kRcpExp, kRcpExp] put every quotient estimate of two chained divisions
+// (x/ap, then /sor) within [2^-880, 2^880] or at zero.
+constexpr int kRcpExp = 750;
+constexpr double kRcpDivMin = 0x1p-64, kRcpDivMax = 0x1p64;
+
+// 1/a for div_rcp, NaN for a divisor outside [2^-64, 2^64] (zero, huge, NaN
+// included): every quotient that uses it is then NaN, which the sweep
+// detects and recomputes with the hardware division
+__device__ inline double rcp_of(double a) {
+  const double m = fabs(a);
+  return (m >= kRcpDivMin && m <= kRcpDivMax) ? 1.0 / a : __builtin_nan("");
+}
 
 __device__ inline double dot3(const double *a, const double *b) {
   return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
@@ -302,6 +339,99 @@ __global__ void k_coef_uvw(MeshDev m, FieldsDev f, double dt, double alpha_u) {
   double dc = ap;
   for (int k = r0; k < r1; k++) dc = dc - f.anb[k];
   f.dc[e] = vol / dc;
+}
+
+// ---------------------------------------------------------------------------
+// Energy equation (src/equations/mod_energy.f90; never solved by the
+// reference's main loop, whose update_boundaries still applies its BCs).
+// ---------------------------------------------------------------------------
+// construct_energy, :30-38: t = 273 K, enthalpy h = cp*t on the cells, h0 = h,
+// zero gradients (the halos of h are set by the BC setters before any use)
+__global__ void k_energy_init(MeshDev m, EnergyDev en) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = m.ne + m.nbf;
+  if (i >= n) return;
+  en.t[i] = 273.0;
+  en.h[i] = i < m.ne ? 273.0 * m.cp[i] : 0.0;
+  en.h0[i] = en.h[i];
+  for (int q = 0; q < 3; q++) {
+    en.gt[3 * i + q] = 0.0;
+    en.gh[3 * i + q] = 0.0;
+  }
+}
+
+// BC setters lid (t = 373) and dirichlet0 (t = 273), :182-212, per halo;
+// construct_energy binds 'top' to lid and every other section to
+// dirichlet0 (:41-47), i.e. the lid-driven cavity's uvwp binding
+__global__ void k_bc_energy(MeshDev m, EnergyDev en) {
+  const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= m.nbf) return;
+  const int H = m.ne + h, e = m.hcell[h];
+  en.t[H] = m.hkind[h] == CFDL_BC_LID ? 373.0 : 273.0;
+  en.h[H] = m.cp[e] * en.t[H];
+}
+
+// calc_coef_energy, :82-169: upwind convection with the face mass fluxes of
+// uvwp, diffusion tc/cp with a deferred non-orthogonal correction from the
+// gradients of t and h, then the Dirichlet boundary faces of the cell in
+// halo order (f = 0, d = tc*A/ds/cp)
+__global__ void k_coef_energy(MeshDev m, FieldsDev f, EnergyDev en, double dt) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m.ne) return;
+  double rp[3];
+  centre(m, e, rp);
+  double ap = 0.0, sumf = 0.0, sumdefc = 0.0;
+  for (int k = m.row[e]; k < m.row[e + 1]; k++) {
+    f.anb[k] = 0.0;
+    const int nb = m.nbr[k];
+    if (nb >= m.ne) continue;
+    const int fs = m.fsg[k];
+    const double sg = fs >= 0 ? 1.0 : -1.0;
+    const int fg = abs(fs) - 1;
+    const double *a = m.aip + 3 * fg, *rip = m.rip + 3 * fg;
+    const double area = area_of(a);
+    double nrm[3], rpnb[3], dr[3];
+    for (int i = 0; i < 3; i++) nrm[i] = sg * a[i] / area;
+    centre(m, nb, rpnb);
+    for (int i = 0; i < 3; i++) dr[i] = rpnb[i] - rp[i];
+    const double wt = vec_weight(rip, rp, rpnb);
+    const double fl = -(sg * f.mip[fg]);
+    const double fnb = fmax(fl, 0.0);
+    sumf = sumf + fl;
+    const double tci = (1.0 - wt) * m.tc[e] + wt * m.tc[nb];
+    const double cpi = (1.0 - wt) * m.cp[e] + wt * m.cp[nb];
+    const double d = tci / cpi / dot3(dr, nrm) * area;
+    double ghi[3], gti[3];
+    for (int i = 0; i < 3; i++) {
+      ghi[i] = (1.0 - wt) * en.gh[3 * e + i] + wt * en.gh[3 * nb + i];
+      gti[i] = (1.0 - wt) * en.gt[3 * e + i] + wt * en.gt[3 * nb + i];
+    }
+    sumdefc = sumdefc + tci * area * (dot3(gti, nrm) - dot3(ghi, nrm) / cpi);
+    f.anb[k] = d + fnb;
+    ap = ap + d + fnb;
+  }
+  const double ap0 = m.rho[e] * m.vol[e] / dt;
+  ap = ap + ap0;
+  f.b[e] = ap0 * en.h0[e] + sumf * en.h[e] + sumdefc;
+  for (int q = m.bcrow[e]; q < m.bcrow[e + 1]; q++) {
+    const int k = m.bcslot[q], H = m.nbr[k];
+    const int fg = m.fsg[k] - 1;  // boundary faces are always owner side
+    const double *a = m.aip + 3 * fg;
+    const double area = area_of(a);
+    const double nrm[3] = {a[0] / area, a[1] / area, a[2] / area};
+    const double dr[3] = {m.xc[H] - rp[0], m.yc[H] - rp[1], m.zc[H] - rp[2]};
+    const double ds = dot3(dr, nrm);
+    const double fz = 0.0, d = m.tc[e] * area / ds / m.cp[e];
+    ap = ap + d + fz;
+    f.anb[k] = f.anb[k] + d + fz;
+  }
+  f.ap[e] = ap;
+}
+
+// calc_temperature, mod_properties.f90:214-222
+__global__ void k_temperature(MeshDev m, EnergyDev en) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < m.ne) en.t[e] = en.h[e] / m.cp[e];
 }
 
 // ---------------------------------------------------------------------------
@@ -482,6 +612,20 @@ __global__ void k_copy(double *dst, const double *src, int64_t n) {
   if (i < n) dst[i] = src[i];
 }
 
+// Net outward mass flux of each cell, sum over its CSR entries of
+// mip(face) * sign in row order: the quantity the reference's VTU output
+// stores in dc and writes as 'mip' (src/modules/mod_vtu_output.f90:113-123).
+__global__ void k_mass_imbalance(MeshDev m, const double *__restrict__ mip, double *out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m.ne) return;
+  double s = 0.0;
+  for (int k = m.row[e]; k < m.row[e + 1]; k++) {
+    const int f = m.fsg[k];
+    s = s + mip[(f > 0 ? f : -f) - 1] * (f > 0 ? 1.0 : -1.0);
+  }
+  out[e] = s;
+}
+
 // ---------------------------------------------------------------------------
 // Gauss-Seidel: one dependency level of a sweep, up to NC right-hand sides
 // sharing ap/anb (smoother_gs / solve_gs inner loop, mod_solver.f90:205-224,
@@ -548,46 +692,71 @@ k_gs_tiles(SysDesc s, const int *__restrict__ bseg, const int *__restrict__ seg,
 // first LDS store, so the gather costs two memory latencies (the outside
 // indices, then their values).  The segment count is at most 63 (host).
 constexpr int kLdsThreads = 64;
-template <bool HEX, bool PIPE>
+template <bool HEX, bool PIPE, int NC>
 __global__ void __launch_bounds__(kLdsThreads)
 k_gs_tiles_lds(SysDesc s, const int *__restrict__ bseg, const int *__restrict__ seg,
                const unsigned short *__restrict__ lnbr, const int *__restrict__ xptr,
                const int *__restrict__ xslot, int batch0, int backward, double sor,
-               const int *__restrict__ active) {
+               double rsor, const int *__restrict__ active) {
   extern __shared__ double lds[];
-  if (!active[0]) return;
+  // NC right-hand sides share ap/anb (u, v, w of a momentum solve); a
+  // right-hand side whose solve has stopped is neither updated nor written
+  bool act[NC];
+  bool any = false;
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    act[c] = active[c] != 0;
+    any = any || act[c];
+  }
+  if (!any) return;
   const int b = batch0 + blockIdx.x;
   const int s0 = bseg[b], nq = bseg[b + 1] - s0 - 1;
   const int c0 = seg[s0], nc = seg[s0 + nq] - c0;
   const int r0 = s.row[c0], nent = s.row[c0 + nc] - r0;
   const int x0 = xptr[b], nx = xptr[b + 1] - x0;
-  double *lphi = lds;              // nc + nx values
-  double *lap = lphi + nc + nx;    // nc
-  double *lb = lap + nc;           // nc
-  double *lanb = lb + nc;          // nent
+  const int np = nc + nx;
+  double *lphi = lds;               // NC x (nc + nx) values
+  double *lap = lphi + NC * np;     // nc
+  double *lra = lap + nc;           // nc reciprocals 1/ap
+  double *lb = lra + nc;            // NC x nc
+  double *lanb = lb + NC * nc;      // nent
   int *lseg = reinterpret_cast<int *>(lanb + nent);  // nq + 1 bounds relative to c0
   int *lrow = lseg + nq + 1;                          // nc + 1 row starts (general rows)
   unsigned short *lnb = reinterpret_cast<unsigned short *>(lrow + (HEX ? 0 : nc + 1));
-  double *phi = s.phi[0];
-  const double *pp = phi + c0, *pa = s.ap + c0, *pb = s.b[0] + c0, *pn = s.anb + r0;
+  const double *pa = s.ap + c0, *pr = s.rap + c0, *pn = s.anb + r0;
   const unsigned short *pl = lnbr + r0;
   const int sg = threadIdx.x <= nq ? seg[s0 + threadIdx.x] - c0 : 0;
-  constexpr int U = 8;
-  const int nmax = nc > nx ? nc : nx;
-  for (int base = 0; base < nmax; base += U * kLdsThreads) {
-    int xs[U];
-    double vp[U], va[U], vb[U], vx[U], vn[U][6];
+  // gather: each round covers U*64 cells (values, coefficients, slots) and
+  // 512 outside neighbours; every load of a round is issued before its LDS
+  // stores, so a round costs two memory latencies (the outside indices, then
+  // their values)
+  constexpr int U = NC == 1 ? 8 : 4;
+  const int nrc = (nc + U * kLdsThreads - 1) / (U * kLdsThreads);
+  const int nrx = (nx + 8 * kLdsThreads - 1) / (8 * kLdsThreads);
+  const int nr = nrc > nrx ? nrc : nrx;
+  for (int r = 0; r < nr; r++) {
+    const int cb = r * U * kLdsThreads, xb = r * 8 * kLdsThreads;
+    int xs[8];
+    double va[U], vr[U], vn[U][6], vp[NC][U], vb[NC][U], vx[NC][8];
     unsigned short sl[U][6];
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int i = base + u * kLdsThreads + threadIdx.x;
+    for (int u = 0; u < 8; u++) {
+      const int j = xb + u * kLdsThreads + threadIdx.x;
       xs[u] = -1;
-      if (base + u * kLdsThreads < nx) xs[u] = i < nx ? xslot[x0 + i] : -1;
-      if (base + u * kLdsThreads < nc) {  // wave-uniform: rows of this round exist
-        const int ic = i < nc ? i : 0;    // lanes past the end reload row 0
-        vp[u] = pp[ic];
+      if (xb + u * kLdsThreads < nx) xs[u] = j < nx ? xslot[x0 + j] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int i = cb + u * kLdsThreads + threadIdx.x;
+      if (cb + u * kLdsThreads < nc) {  // wave-uniform: rows of this round exist
+        const int ic = i < nc ? i : 0;  // lanes past the end reload row 0
         va[u] = pa[ic];
-        vb[u] = pb[ic];
+        vr[u] = pr[ic];
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+          vp[c][u] = s.phi[c][c0 + ic];
+          vb[c][u] = s.b[c][c0 + ic];
+        }
         if (HEX) {
 #pragma unroll
           for (int k = 0; k < 6; k++) {
@@ -598,14 +767,20 @@ k_gs_tiles_lds(SysDesc s, const int *__restrict__ bseg, const int *__restrict__ 
       }
     }
 #pragma unroll
-    for (int u = 0; u < U; u++) vx[u] = xs[u] >= 0 ? phi[xs[u]] : 0.0;
+    for (int u = 0; u < 8; u++)
+#pragma unroll
+      for (int c = 0; c < NC; c++) vx[c][u] = xs[u] >= 0 ? s.phi[c][xs[u]] : 0.0;
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      const int i = base + u * kLdsThreads + threadIdx.x;
+      const int i = cb + u * kLdsThreads + threadIdx.x;
       if (i < nc) {
-        lphi[i] = vp[u];
         lap[i] = va[u];
-        lb[i] = vb[u];
+        lra[i] = vr[u];
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+          lphi[c * np + i] = vp[c][u];
+          lb[c * nc + i] = vb[c][u];
+        }
         if (HEX) {
 #pragma unroll
           for (int k = 0; k < 6; k++) {
@@ -614,7 +789,13 @@ k_gs_tiles_lds(SysDesc s, const int *__restrict__ bseg, const int *__restrict__ 
           }
         }
       }
-      if (i < nx) lphi[nc + i] = vx[u];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int j = xb + u * kLdsThreads + threadIdx.x;
+      if (j < nx)
+#pragma unroll
+        for (int c = 0; c < NC; c++) lphi[c * np + nc + j] = vx[c][u];
     }
   }
   if (!HEX) {
@@ -628,90 +809,151 @@ k_gs_tiles_lds(SysDesc s, const int *__restrict__ bseg, const int *__restrict__ 
   __syncthreads();
   if (PIPE) {
     // Every segment has at most 64 cells: lane l updates cell lseg[q] + l of
-    // segment q.  The operands that do not change during the sweep (ap, b,
-    // anb, slots) and the cell's own old value are loaded for the next
+    // segment q.  The operands that do not change during the sweep (ap, 1/ap,
+    // b, anb, slots) and the cell's own old values are loaded for the next
     // segment while the current one computes, so after each barrier only
     // the neighbour values are read from LDS.  Lanes without a cell load
     // row 0 and store nothing.  Segment bounds come from lane q of sg.
+    // RCP: the divisions by ap and sor use div_rcp.  Without a branch in the
+    // walk, each lane keeps the least and greatest frexp exponent of its
+    // dividends x and a NaN detector (0*nv is NaN for an infinite or NaN
+    // result, which an out-of-range 1/ap or an overflow produces); if any
+    // lane saw a dividend outside [-kRcpExp, kRcpExp] or a non-finite
+    // result, the batch is walked again with hardware division from its
+    // original values, which global memory still holds.
     const int lane = threadIdx.x;
     const double sm1 = sor - 1.0;
-    int q = backward ? nq - 1 : 0;
-    int i = __builtin_amdgcn_readlane(sg, q) + lane;
-    bool on = i < __builtin_amdgcn_readlane(sg, q + 1);
-    int ic = on ? i : 0;
-    double a = lap[ic], bv = lb[ic], t = sm1 * a * lphi[ic], an[6];
-    int sl[6];
+    auto walk = [&](bool rcp) -> bool {
+      int emin = 0, emax = 0;
+      double nan_seen = 0.0;
+      int q = backward ? nq - 1 : 0;
+      int i = __builtin_amdgcn_readlane(sg, q) + lane;
+      bool on = i < __builtin_amdgcn_readlane(sg, q + 1);
+      int ic = on ? i : 0;
+      double a = lap[ic], ya = lra[ic], an[6], bv[NC], t[NC];
+      int sl[6];
 #pragma unroll
-    for (int k = 0; k < 6; k++) {
-      sl[k] = lnb[6 * ic + k];
-      an[k] = lanb[6 * ic + k];
+      for (int c = 0; c < NC; c++) {
+        bv[c] = lb[c * nc + ic];
+        t[c] = sm1 * a * lphi[c * np + ic];
+      }
+#pragma unroll
+      for (int k = 0; k < 6; k++) {
+        sl[k] = lnb[6 * ic + k];
+        an[k] = lanb[6 * ic + k];
+      }
+      for (int j = 0; j < nq; j++) {
+        double v[NC][6];
+#pragma unroll
+        for (int c = 0; c < NC; c++)
+#pragma unroll
+          for (int k = 0; k < 6; k++) v[c][k] = lphi[c * np + sl[k]];
+        // the next segment's operands (its cells are not written this level)
+        const int qn = backward ? q - 1 : q + 1;
+        int in = 0;
+        bool onn = false;
+        if (j + 1 < nq) {
+          in = __builtin_amdgcn_readlane(sg, qn) + lane;
+          onn = in < __builtin_amdgcn_readlane(sg, qn + 1);
+        }
+        const int icn = onn ? in : 0;
+        const double a2 = lap[icn], ya2 = lra[icn];
+        double an2[6], b2[NC], p2[NC];
+        int sl2[6];
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+          b2[c] = lb[c * nc + icn];
+          p2[c] = lphi[c * np + icn];
+        }
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+          sl2[k] = lnb[6 * icn + k];
+          an2[k] = lanb[6 * icn + k];
+        }
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+          if (!act[c]) continue;
+          double sum = bv[c];
+#pragma unroll
+          for (int k = 0; k < 6; k++) sum = sum + an[k] * v[c][k];
+          const double x = sum + t[c];
+          double nv;
+          if (rcp) {
+            const int ex = __builtin_amdgcn_frexp_exp(x);  // 0 for x = 0
+            emin = min(emin, ex);
+            emax = max(emax, ex);
+            nv = div_rcp(div_rcp(x, a, ya), sor, rsor);
+            nan_seen = fma(0.0, nv, nan_seen);
+          } else {
+            nv = x / a / sor;
+          }
+          if (on) lphi[c * np + i] = nv;
+        }
+        __syncthreads();
+        q = qn;
+        i = in;
+        on = onn;
+        a = a2;
+        ya = ya2;
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+          bv[c] = b2[c];
+          t[c] = sm1 * a2 * p2[c];
+        }
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+          sl[k] = sl2[k];
+          an[k] = an2[k];
+        }
+      }
+      return emin < -kRcpExp || emax > kRcpExp || nan_seen != nan_seen;
+    };
+    if (rsor > 0.0) {
+      if (__builtin_amdgcn_ballot_w64(walk(true))) {
+        for (int r = threadIdx.x; r < nc; r += kLdsThreads)
+#pragma unroll
+          for (int c = 0; c < NC; c++) lphi[c * np + r] = s.phi[c][c0 + r];
+        __syncthreads();
+        walk(false);
+      }
+    } else {
+      walk(false);
     }
+  } else {
     for (int j = 0; j < nq; j++) {
-      double v[6];
+      const int q = backward ? nq - 1 - j : j;
+      const int i1 = lseg[q + 1];
+      for (int i = lseg[q] + threadIdx.x; i < i1; i += kLdsThreads) {
+        const double a = lap[i];
 #pragma unroll
-      for (int k = 0; k < 6; k++) v[k] = lphi[sl[k]];
-      // the next segment's operands (its cells are not written this level)
-      const int qn = backward ? q - 1 : q + 1;
-      int in = 0;
-      bool onn = false;
-      if (j + 1 < nq) {
-        in = __builtin_amdgcn_readlane(sg, qn) + lane;
-        onn = in < __builtin_amdgcn_readlane(sg, qn + 1);
+        for (int c = 0; c < NC; c++) {
+          if (!act[c]) continue;
+          double *lp = lphi + c * np;
+          double sum = lb[c * nc + i];
+          if (HEX) {  // six faces: gather all slots first, then sum in row order
+            const int k0 = 6 * i;
+            int sl[6];
+            double v[6];
+#pragma unroll
+            for (int k = 0; k < 6; k++) sl[k] = lnb[k0 + k];
+#pragma unroll
+            for (int k = 0; k < 6; k++) v[k] = lp[sl[k]];
+#pragma unroll
+            for (int k = 0; k < 6; k++) sum = sum + lanb[k0 + k] * v[k];
+          } else {
+            for (int k = lrow[i]; k < lrow[i + 1]; k++) sum = sum + lanb[k] * lp[lnb[k]];
+          }
+          lp[i] = (sum + (sor - 1.0) * a * lp[i]) / a / sor;
+        }
       }
-      const int icn = onn ? in : 0;
-      const double a2 = lap[icn], b2 = lb[icn], p2 = lphi[icn];
-      double an2[6];
-      int sl2[6];
-#pragma unroll
-      for (int k = 0; k < 6; k++) {
-        sl2[k] = lnb[6 * icn + k];
-        an2[k] = lanb[6 * icn + k];
-      }
-      double sum = bv;
-#pragma unroll
-      for (int k = 0; k < 6; k++) sum = sum + an[k] * v[k];
-      const double nv = (sum + t) / a / sor;
-      if (on) lphi[i] = nv;
       __syncthreads();
-      q = qn;
-      i = in;
-      on = onn;
-      a = a2;
-      bv = b2;
-      t = sm1 * a2 * p2;
-#pragma unroll
-      for (int k = 0; k < 6; k++) {
-        sl[k] = sl2[k];
-        an[k] = an2[k];
-      }
     }
-    for (int r = threadIdx.x; r < nc; r += kLdsThreads) phi[c0 + r] = lphi[r];
-    return;
   }
-  for (int j = 0; j < nq; j++) {
-    const int q = backward ? nq - 1 - j : j;
-    const int i1 = lseg[q + 1];
-    for (int i = lseg[q] + threadIdx.x; i < i1; i += kLdsThreads) {
-      const double a = lap[i];
-      double sum = lb[i];
-      if (HEX) {  // six faces: gather all slots first, then sum in row order
-        const int k0 = 6 * i;
-        int sl[6];
-        double v[6];
 #pragma unroll
-        for (int k = 0; k < 6; k++) sl[k] = lnb[k0 + k];
-#pragma unroll
-        for (int k = 0; k < 6; k++) v[k] = lphi[sl[k]];
-#pragma unroll
-        for (int k = 0; k < 6; k++) sum = sum + lanb[k0 + k] * v[k];
-      } else {
-        for (int k = lrow[i]; k < lrow[i + 1]; k++) sum = sum + lanb[k] * lphi[lnb[k]];
-      }
-      lphi[i] = (sum + (sor - 1.0) * a * lphi[i]) / a / sor;
-    }
-    __syncthreads();
+  for (int c = 0; c < NC; c++) {
+    if (!act[c]) continue;
+    for (int r = threadIdx.x; r < nc; r += kLdsThreads) s.phi[c][c0 + r] = lphi[c * np + r];
   }
-  for (int i = threadIdx.x; i < nc; i += kLdsThreads) phi[c0 + i] = lphi[i];
 }
 
 // Residual partial sums: block (x, y) covers cells [beg_y + x*chunk, ...).
@@ -869,11 +1111,12 @@ __global__ void k_log_pc(const PcCtl *c, LogLine *log, int *nlog) {
 __global__ void k_pc_assemble(int ne, const int *l2g_cell, const int *rowp, const int *rowg,
                               const double *ap, const double *b, const double *anb,
                               const double *phi, double *ap_p, double *b_p, double *anb_p,
-                              double *phic_p) {
+                              double *phic_p, double *rap_p) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ne) return;
   const int g = l2g_cell[c];
   ap_p[c] = ap[g];
+  rap_p[c] = rcp_of(ap[g]);
   b_p[c] = b[g];
   phic_p[c] = phi[g];
   const int kg = rowg[g];
@@ -891,6 +1134,43 @@ __global__ void k_exchange(int n, const int *dst, const int *src, double *phic,
 __global__ void k_scatter(int n, const int *idx, const double *src, double *dst) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dst[idx[i]] = src[i];
+}
+
+// The global system in tile numbering for a tiled solve_gs: gather the
+// shared coefficients, the right-hand sides and the values (boundary halos
+// keep their slots ne..ne+nbf-1; rows keep their entry order) before the
+// solve, scatter the values back after it.
+struct TileMap {
+  int ne, nbf, ny;
+  const int *l2g, *rowm, *rowg;
+  const double *ap, *anb;
+  double *ap_m, *anb_m, *rap_m;
+  const double *b[3];
+  double *b_m[3];
+  double *phi[3];
+  double *phi_m[3];
+};
+__global__ void k_tile_gather(TileMap t) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < t.ne) {
+    const int g = t.l2g[i];
+    t.ap_m[i] = t.ap[g];
+    t.rap_m[i] = rcp_of(t.ap[g]);
+    const int r = t.rowm[i], rg = t.rowg[g], n = t.rowg[g + 1] - rg;
+    for (int k = 0; k < n; k++) t.anb_m[r + k] = t.anb[rg + k];
+    for (int y = 0; y < t.ny; y++) {
+      t.b_m[y][i] = t.b[y][g];
+      t.phi_m[y][i] = t.phi[y][g];
+    }
+  } else if (i < t.ne + t.nbf) {
+    for (int y = 0; y < t.ny; y++) t.phi_m[y][i] = t.phi[y][i];
+  }
+}
+__global__ void k_tile_scatter(TileMap t) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= t.ne) return;
+  const int g = t.l2g[i];
+  for (int y = 0; y < t.ny; y++) t.phi[y][g] = t.phi_m[y][i];
 }
 
 }  // namespace cfdl
@@ -926,6 +1206,7 @@ struct cfdl_ctx {
   DevBuf mem;
   MeshDev md{};
   FieldsDev fd{};
+  EnergyDev en{};
   double *fields[CFDL_NFIELDS] = {};
   int64_t fsize[CFDL_NFIELDS] = {};
   Sched gf, gb;  // global system, forward / backward sweeps
@@ -933,7 +1214,7 @@ struct cfdl_ctx {
   int Lp = 0, nh = 0, nx = 0;
   int *rowp = nullptr, *nbrp = nullptr, *l2g_cell = nullptr, *halo_src = nullptr;
   int *xdst = nullptr, *xsrc = nullptr, *rowg = nullptr;
-  double *ap_p = nullptr, *b_p = nullptr, *anb_p = nullptr, *phic_p = nullptr;
+  double *ap_p = nullptr, *b_p = nullptr, *anb_p = nullptr, *phic_p = nullptr, *rap_p = nullptr;
   std::vector<int> co;  // subdomain cell offsets (nsubd+1)
   Sched pcl;            // pc system by forward level (CFDL_GS_MODE=level)
   int pc_nlev = 0;
@@ -948,8 +1229,14 @@ struct cfdl_ctx {
     bool pipe = false;  // hex, and no level segment has more than 64 cells
     int nbatch = 0, nseg = 0;
     size_t lds_bytes = 0;  // 0: batches too large for LDS, global-memory kernel
-  } ts;
+  } ts, tm;
+  // the global system in tile numbering (tiled solve_gs): new -> old cell,
+  // CSR, and copies of the shared coefficients, right-hand sides and values
+  bool mtiled = false;
+  int *l2g_m = nullptr, *rowm = nullptr, *nbrm = nullptr;
+  double *ap_m = nullptr, *anb_m = nullptr, *rap_m = nullptr, *b_m[3] = {}, *phi_m[3] = {};
   bool tiled = true;
+  bool rcp = true;  // reciprocal-based division in the pipelined walk
   // control
   GsCtl *gctl = nullptr;
   PcCtl *pctl = nullptr;
@@ -1101,30 +1388,50 @@ static int pc_sweep(cfdl_ctx *c, const SysDesc &sys, bool forward, const int *ac
   return CFDL_OK;
 }
 
-static int pc_sweep_tiled(cfdl_ctx *c, const SysDesc &sys, bool forward, const int *active) {
-  const cfdl_ctx::TileSched &t = c->ts;
+// One tiled sweep of a system in its tile numbering: the launches of the
+// schedule in order (forward) or in reverse (backward), NC right-hand sides.
+template <int NC>
+static void tiled_launches(cfdl_ctx *c, const cfdl_ctx::TileSched &t, const SysDesc &sys,
+                           bool forward, double sor, const int *active) {
   const int nl = static_cast<int>(t.launch_ptr.size()) - 1;
   const int bwd = forward ? 0 : 1;
-  cfdl_ctx::Sample *smp = nullptr;
-  CALL(sample_begin(c, &smp));
+  // 1/sor correctly rounded, as div_rcp requires; 0 selects the hardware
+  // division in the kernel (CFDL_TILE_RCP=0, or sor outside div_rcp's
+  // divisor range)
+  const bool rcp_ok = c->rcp && std::fabs(sor) >= kRcpDivMin && std::fabs(sor) <= kRcpDivMax;
+  const double rsor = rcp_ok ? 1.0 / sor : 0.0;
   for (int i = 0; i < nl; i++) {
     const int m = forward ? i : nl - 1 - i;
     const int b0 = t.launch_ptr[m], nb = t.launch_ptr[m + 1] - b0;
-    if (t.lds_bytes > 0 && t.pipe)
-      hipLaunchKernelGGL((k_gs_tiles_lds<true, true>), dim3(nb), dim3(kLdsThreads), t.lds_bytes,
-                         c->st, sys, t.bseg, t.seg, t.lnbr, t.xptr, t.xslot, b0, bwd, 1.02,
-                         active);
-    else if (t.lds_bytes > 0 && t.hex)
-      hipLaunchKernelGGL((k_gs_tiles_lds<true, false>), dim3(nb), dim3(kLdsThreads), t.lds_bytes,
-                         c->st, sys, t.bseg, t.seg, t.lnbr, t.xptr, t.xslot, b0, bwd, 1.02,
-                         active);
-    else if (t.lds_bytes > 0)
-      hipLaunchKernelGGL((k_gs_tiles_lds<false, false>), dim3(nb), dim3(kLdsThreads), t.lds_bytes,
-                         c->st, sys, t.bseg, t.seg, t.lnbr, t.xptr, t.xslot, b0, bwd, 1.02,
-                         active);
+    if (t.pipe)
+      hipLaunchKernelGGL((k_gs_tiles_lds<true, true, NC>), dim3(nb), dim3(kLdsThreads),
+                         t.lds_bytes, c->st, sys, t.bseg, t.seg, t.lnbr, t.xptr, t.xslot, b0, bwd,
+                         sor, rsor, active);
+    else if (t.hex)
+      hipLaunchKernelGGL((k_gs_tiles_lds<true, false, NC>), dim3(nb), dim3(kLdsThreads),
+                         t.lds_bytes, c->st, sys, t.bseg, t.seg, t.lnbr, t.xptr, t.xslot, b0, bwd,
+                         sor, rsor, active);
     else
+      hipLaunchKernelGGL((k_gs_tiles_lds<false, false, NC>), dim3(nb), dim3(kLdsThreads),
+                         t.lds_bytes, c->st, sys, t.bseg, t.seg, t.lnbr, t.xptr, t.xslot, b0, bwd,
+                         sor, rsor, active);
+  }
+}
+
+static int pc_sweep_tiled(cfdl_ctx *c, const SysDesc &sys, bool forward, const int *active) {
+  const cfdl_ctx::TileSched &t = c->ts;
+  const int nl = static_cast<int>(t.launch_ptr.size()) - 1;
+  cfdl_ctx::Sample *smp = nullptr;
+  CALL(sample_begin(c, &smp));
+  if (t.lds_bytes > 0) {
+    tiled_launches<1>(c, t, sys, forward, 1.02, active);
+  } else {
+    for (int i = 0; i < nl; i++) {
+      const int m = forward ? i : nl - 1 - i;
+      const int b0 = t.launch_ptr[m], nb = t.launch_ptr[m + 1] - b0;
       hipLaunchKernelGGL(k_gs_tiles, dim3(nb), dim3(kTileThreads), 0, c->st, sys, t.bseg, t.seg,
-                         b0, bwd, 1.02, active);
+                         b0, forward ? 0 : 1, 1.02, active);
+    }
   }
   CALL(sample_end(c, smp, nl, c->ne));
   c->pc_launches += nl;
@@ -1159,25 +1466,64 @@ static int solve_gs_n(cfdl_ctx *c, int ny, double **phi, const double **b,
                       const char *const *names, double sor) {
   Names nm{};
   for (int y = 0; y < ny; y++) std::strncpy(nm.n[y], names[y], 7);
+  // tiled: the system and its right-hand sides in tile numbering (one or
+  // three right-hand sides; solve_gs_uvw passes three)
+  const bool tiled = c->mtiled && (ny == 1 || ny == 3);
+  TileMap tmap{};
+  if (tiled) {
+    tmap.ne = c->ne;
+    tmap.nbf = c->nbf;
+    tmap.ny = ny;
+    tmap.l2g = c->l2g_m;
+    tmap.rowm = c->rowm;
+    tmap.rowg = c->md.row;
+    tmap.ap = c->fd.ap;
+    tmap.anb = c->fd.anb;
+    tmap.ap_m = c->ap_m;
+    tmap.anb_m = c->anb_m;
+    tmap.rap_m = c->rap_m;
+    for (int y = 0; y < ny; y++) {
+      tmap.b[y] = b[y];
+      tmap.b_m[y] = c->b_m[y];
+      tmap.phi[y] = phi[y];
+      tmap.phi_m[y] = c->phi_m[y];
+    }
+    hipLaunchKernelGGL(k_tile_gather, dim3(nblocks(int64_t{c->ne} + c->nbf)), dim3(kBlock), 0,
+                       c->st, tmap);
+  }
   SysDesc s{};
   s.ny = ny;
   for (int y = 0; y < ny; y++) {
     s.beg[y] = 0;
     s.end[y] = c->ne;
-    s.phi[y] = phi[y];
-    s.b[y] = b[y];
+    s.phi[y] = tiled ? c->phi_m[y] : phi[y];
+    s.b[y] = tiled ? c->b_m[y] : b[y];
   }
-  s.row = c->md.row;
-  s.nbr = c->md.nbr;
-  s.ap = c->fd.ap;
-  s.anb = c->fd.anb;
+  s.row = tiled ? c->rowm : c->md.row;
+  s.nbr = tiled ? c->nbrm : c->md.nbr;
+  s.ap = tiled ? c->ap_m : c->fd.ap;
+  s.anb = tiled ? c->anb_m : c->fd.anb;
+  s.rap = tiled ? c->rap_m : nullptr;
   CALL(residual(c, s, c->nblk_g, c->chunk_g, 1));
   hipLaunchKernelGGL(k_gs_init, dim3(1), dim3(64), 0, c->st, c->gctl, c->red, ny,
                      static_cast<double>(c->ne), c->nit);
   // one SysDesc per sweep kernel with its RHS subset in slots 0..ny-1
   for (int it = 0; it < c->nit; it++) {
-    CALL(sweep(c, c->gf, s, ny, sor, gs_active(c)));
-    CALL(sweep(c, c->gb, s, ny, sor, gs_active(c)));
+    if (tiled) {
+      const int nl = static_cast<int>(c->tm.launch_ptr.size()) - 1;
+      if (ny == 3) {
+        tiled_launches<3>(c, c->tm, s, true, sor, gs_active(c));
+        tiled_launches<3>(c, c->tm, s, false, sor, gs_active(c));
+      } else {
+        tiled_launches<1>(c, c->tm, s, true, sor, gs_active(c));
+        tiled_launches<1>(c, c->tm, s, false, sor, gs_active(c));
+      }
+      c->gs_launches += 2 * nl;
+      HIPCHK(hipGetLastError());
+    } else {
+      CALL(sweep(c, c->gf, s, ny, sor, gs_active(c)));
+      CALL(sweep(c, c->gb, s, ny, sor, gs_active(c)));
+    }
     CALL(residual(c, s, c->nblk_g, c->chunk_g, 1));
     hipLaunchKernelGGL(k_gs_iter_end, dim3(1), dim3(64), 0, c->st, c->gctl, c->red, ny,
                        static_cast<double>(c->ne), c->nit);
@@ -1188,6 +1534,8 @@ static int solve_gs_n(cfdl_ctx *c, int ny, double **phi, const double **b,
     for (int y = 0; y < ny; y++) any = any || c->host_flags[y];
     if (!any) break;
   }
+  if (tiled)
+    hipLaunchKernelGGL(k_tile_scatter, dim3(nblocks(c->ne)), dim3(kBlock), 0, c->st, tmap);
   hipLaunchKernelGGL(k_log_gs, dim3(1), dim3(64), 0, c->st, c->gctl, ny, nm, c->dlog, c->dlog_n);
   HIPCHK(hipGetLastError());
   return CFDL_OK;
@@ -1198,7 +1546,7 @@ static int multi_subdomain_solver(cfdl_ctx *c) {
   // assemble_coef, mod_subdomains.f90:162-189
   hipLaunchKernelGGL(k_pc_assemble, dim3(nblocks(ne)), dim3(kBlock), 0, c->st, ne, c->l2g_cell,
                      c->rowp, c->md.row, c->fd.ap, c->fd.b, c->fd.anb, c->fd.phic, c->ap_p,
-                     c->b_p, c->anb_p, c->phic_p);
+                     c->b_p, c->anb_p, c->phic_p, c->rap_p);
   if (c->nh > 0)
     hipLaunchKernelGGL(k_gather, dim3(nblocks(c->nh)), dim3(kBlock), 0, c->st, c->nh, c->halo_src,
                        c->fd.phic, c->phic_p + ne);
@@ -1217,6 +1565,7 @@ static int multi_subdomain_solver(cfdl_ctx *c) {
   s.nbr = c->nbrp;
   s.ap = c->ap_p;
   s.anb = c->anb_p;
+  s.rap = c->rap_p;
   CALL(residual(c, s, c->nblk_p, c->chunk_p, 0));
   hipLaunchKernelGGL(k_pc_init, dim3(1), dim3(64), 0, c->st, c->pctl, c->red, sc, c->nit);
   SysDesc s1 = s;  // the sweeps see the whole concatenated system as one RHS
@@ -1247,6 +1596,37 @@ static int multi_subdomain_solver(cfdl_ctx *c) {
   hipLaunchKernelGGL(k_log_pc, dim3(1), dim3(64), 0, c->st, c->pctl, c->dlog, c->dlog_n);
   HIPCHK(hipGetLastError());
   return CFDL_OK;
+}
+
+// Dense ids of spatial tiles of about edge^3 cells: bins of the cell centres
+// in the bounding box of each group of positions [grp[s], grp[s+1]) (the
+// subdomains of the pc system, or all cells); tiles never cross groups.
+// cell[q] is the mesh cell at position q.
+static void bin_tiles(const std::vector<int> &grp, const std::vector<int> &cell, const double *xc,
+                      const double *yc, const double *zc, const double *vol, int edge,
+                      std::vector<int> *tid) {
+  std::unordered_map<int64_t, int> tid_of;
+  tid->assign(cell.size(), 0);
+  for (size_t s = 0; s + 1 < grp.size(); s++) {
+    double lo[3] = {1e300, 1e300, 1e300}, vsum = 0.0;
+    for (int q = grp[s]; q < grp[s + 1]; q++) {
+      const int g = cell[q];
+      const double r[3] = {xc[g], yc[g], zc[g]};
+      for (int i = 0; i < 3; i++) lo[i] = std::min(lo[i], r[i]);
+      vsum += vol[g];
+    }
+    const double h = std::cbrt(vsum / (grp[s + 1] - grp[s])) * edge;
+    for (int q = grp[s]; q < grp[s + 1]; q++) {
+      const int g = cell[q];
+      const int64_t bx = static_cast<int64_t>((xc[g] - lo[0]) / h);
+      const int64_t by = static_cast<int64_t>((yc[g] - lo[1]) / h);
+      const int64_t bz = static_cast<int64_t>((zc[g] - lo[2]) / h);
+      const int64_t key = ((static_cast<int64_t>(s) * 4096 + bx) * 4096 + by) * 4096 + bz;
+      auto it = tid_of.find(key);
+      if (it == tid_of.end()) it = tid_of.emplace(key, static_cast<int>(tid_of.size())).first;
+      (*tid)[q] = it->second;
+    }
+  }
 }
 
 // Space-time tiles of the pressure-correction sweeps, on the pc system in
@@ -1300,9 +1680,9 @@ static void tile_launches(int ne, const std::vector<int> &row, const std::vector
 static int build_tile_tables(cfdl_ctx *c, const std::vector<int> &rowp,
                              const std::vector<int> &nbrp, const std::vector<int> &sd,
                              const std::vector<int> &launch, const std::vector<int> &tid,
-                             const std::vector<int> &lev) {
+                             const std::vector<int> &lev, int nrhs, cfdl_ctx::TileSched *tp) {
   const int ne = c->ne;
-  cfdl_ctx::TileSched &t = c->ts;
+  cfdl_ctx::TileSched &t = *tp;
   struct Batch {
     int launch, c0, c1, s0, s1;  // cells [c0, c1), level-segment starts sb[s0..s1)
   };
@@ -1368,7 +1748,7 @@ static int build_tile_tables(cfdl_ctx *c, const std::vector<int> &rowp,
     }
     xptr[i + 1] = static_cast<int>(xslot.size());
     const size_t nent = rowp[B.c1] - rowp[B.c0];
-    const size_t bytes = 8 * (3 * static_cast<size_t>(nc) + ext.size() + nent) +
+    const size_t bytes = 8 * ((2 * nrhs + 2) * static_cast<size_t>(nc) + nrhs * ext.size() + nent) +
                          4 * (nq + 1 + (hex ? 0 : nc + 1)) + 2 * nent;
     lds_max = std::max(lds_max, bytes);
   }
@@ -1388,14 +1768,19 @@ static int build_tile_tables(cfdl_ctx *c, const std::vector<int> &rowp,
   int lds_dev = 0;
   HIPCHK(hipDeviceGetAttribute(&lds_dev, hipDeviceAttributeMaxSharedMemoryPerBlock, c->dev));
   size_t lds_cap = 64 * 1024;
-  if (static_cast<size_t>(lds_dev) > lds_cap && lds_max > lds_cap &&
-      hipFuncSetAttribute(reinterpret_cast<const void *>(&k_gs_tiles_lds<true, true>),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, lds_dev) == hipSuccess &&
-      hipFuncSetAttribute(reinterpret_cast<const void *>(&k_gs_tiles_lds<true, false>),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, lds_dev) == hipSuccess &&
-      hipFuncSetAttribute(reinterpret_cast<const void *>(&k_gs_tiles_lds<false, false>),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, lds_dev) == hipSuccess)
-    lds_cap = static_cast<size_t>(lds_dev);
+  if (static_cast<size_t>(lds_dev) > lds_cap && lds_max > lds_cap) {
+    const void *kern[] = {reinterpret_cast<const void *>(&k_gs_tiles_lds<true, true, 1>),
+                          reinterpret_cast<const void *>(&k_gs_tiles_lds<true, false, 1>),
+                          reinterpret_cast<const void *>(&k_gs_tiles_lds<false, false, 1>),
+                          reinterpret_cast<const void *>(&k_gs_tiles_lds<true, true, 3>),
+                          reinterpret_cast<const void *>(&k_gs_tiles_lds<true, false, 3>),
+                          reinterpret_cast<const void *>(&k_gs_tiles_lds<false, false, 3>)};
+    bool ok = true;
+    for (const void *k : kern)
+      ok = ok && hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, lds_dev) ==
+                     hipSuccess;
+    if (ok) lds_cap = static_cast<size_t>(lds_dev);
+  }
   (void)hipGetLastError();
   t.lds_bytes = (lds_ok && fits && lds_max <= lds_cap) ? lds_max : 0;
   t.hex = hex;
@@ -1527,7 +1912,7 @@ int cfdl_ctx_create(int ne, int nf, int nbf, const int *ef2nb_idx, const int *ef
     HIPCHK(hipMemcpy(*p, src, 8 * cnt, hipMemcpyHostToDevice));
     return CFDL_OK;
   };
-  double *d_xc, *d_yc, *d_zc, *d_vol, *d_aip, *d_rip, *d_rho, *d_mu;
+  double *d_xc, *d_yc, *d_zc, *d_vol, *d_aip, *d_rip, *d_rho, *d_mu, *d_cp, *d_tc;
   CALL(up64(&d_xc, xc, n));
   CALL(up64(&d_yc, yc, n));
   CALL(up64(&d_zc, zc, n));
@@ -1536,13 +1921,18 @@ int cfdl_ctx_create(int ne, int nf, int nbf, const int *ef2nb_idx, const int *ef
   CALL(up64(&d_rip, rip, 3 * int64_t{nf}));
   CALL(up64(&d_rho, rho, ne));
   CALL(up64(&d_mu, mu, ne));
+  // thermal properties as init_properties sets them, mod_properties.f90:88-89
+  CALL(up64(&d_cp, std::vector<double>(ne, 1000.0).data(), ne));
+  CALL(up64(&d_tc, std::vector<double>(ne, 5.0).data(), ne));
   C->md = MeshDev{ne, nf, nbf, L, d_row, d_nbr, d_fsg, d_own, d_fcell, d_bcrow, d_bcslot,
-                  d_hcell, d_hslot, d_hkind, d_xc, d_yc, d_zc, d_vol, d_aip, d_rip, d_rho, d_mu};
+                  d_hcell, d_hslot, d_hkind, d_xc, d_yc, d_zc, d_vol, d_aip, d_rip, d_rho, d_mu,
+                  d_cp, d_tc};
   trace("ctx_create: mesh uploaded");
   // --- fields ---
   const int64_t sz[CFDL_NFIELDS] = {n, n, n, n, n, n, n, 3 * int64_t{n}, 3 * int64_t{n},
                                     3 * int64_t{n}, 3 * int64_t{n}, 3 * int64_t{n}, nf, nf, ne,
-                                    ne, ne, ne, ne, ne, ne, L, n, n, 3 * int64_t{n}};
+                                    ne, ne, ne, ne, ne, ne, L, n, n, 3 * int64_t{n},
+                                    n, 3 * int64_t{n}, n, n, 3 * int64_t{n}};
   for (int f = 0; f < CFDL_NFIELDS; f++) {
     C->fsize[f] = sz[f];
     CALL(dalloc(C, &C->fields[f], sz[f]));
@@ -1553,12 +1943,63 @@ int cfdl_ctx_create(int ne, int nf, int nbf, const int *ef2nb_idx, const int *ef
                     F[CFDL_W0], F[CFDL_GU], F[CFDL_GV], F[CFDL_GW], F[CFDL_GP], F[CFDL_GPC],
                     F[CFDL_MIP], F[CFDL_MIP0], F[CFDL_BU], F[CFDL_BV], F[CFDL_BW], F[CFDL_D],
                     F[CFDL_DC], F[CFDL_AP], F[CFDL_B], F[CFDL_ANB], F[CFDL_PHIC]};
+  C->en = EnergyDev{F[CFDL_T], F[CFDL_GT], F[CFDL_ENTH], F[CFDL_ENTH0], F[CFDL_GENTH]};
+  hipLaunchKernelGGL(k_energy_init, dim3(nblocks(n)), dim3(kBlock), 0, C->st, C->md, C->en);
   trace("ctx_create: fields");
+  // sweep schedules: spatial tiles of about edge^3 cells (edge 8 measured
+  // fastest on the 128^3 cavity among 5..12), batches spanning fewer than
+  // `span` levels; CFDL_GS_MODE=level keeps one launch per level
+  const char *te = std::getenv("CFDL_TILE");
+  const int edge = te ? std::max(1, std::atoi(te)) : 8;
+  const char *sp = std::getenv("CFDL_TILE_SPAN");
+  const int span = sp ? std::min(63, std::max(1, std::atoi(sp))) : 16;
+  const char *mode = std::getenv("CFDL_GS_MODE");
+  C->tiled = !(mode && std::string(mode) == "level");
+  const char *rc = std::getenv("CFDL_TILE_RCP");
+  C->rcp = !(rc && rc[0] == '0');
   // --- global Gauss-Seidel schedules ---
   {
     std::vector<int> lev(ne);
     levels(row, nbr, 0, ne, true, lev);
     CALL(make_sched(C, lev, ne, &C->gf));
+    if (C->tiled) {
+      // the global system in tile numbering, for the tiled solve_gs
+      std::vector<int> all(ne), grp{0, ne}, tid, launch;
+      for (int e = 0; e < ne; e++) all[e] = e;
+      bin_tiles(grp, all, xc, yc, zc, vol, edge, &tid);
+      tile_launches(ne, row, nbr, lev, tid, span, &launch);
+      std::vector<int> n2o(all), o2n(ne);
+      std::stable_sort(n2o.begin(), n2o.end(), [&](int x, int y) {
+        if (launch[x] != launch[y]) return launch[x] < launch[y];
+        if (tid[x] != tid[y]) return tid[x] < tid[y];
+        return lev[x] < lev[y];
+      });
+      for (int i = 0; i < ne; i++) o2n[n2o[i]] = i;
+      std::vector<int> rowm(ne + 1, 0), nbrm, zero(ne, 0), launchm(ne), tidm(ne), levm(ne);
+      nbrm.reserve(L);
+      for (int i = 0; i < ne; i++) {
+        const int g = n2o[i];
+        launchm[i] = launch[g];
+        tidm[i] = tid[g];
+        levm[i] = lev[g];
+        rowm[i + 1] = rowm[i] + (row[g + 1] - row[g]);
+        for (int k = row[g]; k < row[g + 1]; k++) nbrm.push_back(nbr[k] < ne ? o2n[nbr[k]] : nbr[k]);
+      }
+      CALL(build_tile_tables(C, rowm, nbrm, zero, launchm, tidm, levm, 3, &C->tm));
+      C->mtiled = C->tm.lds_bytes > 0;
+      if (C->mtiled) {
+        CALL(upload(C, &C->l2g_m, n2o));
+        CALL(upload(C, &C->rowm, rowm));
+        CALL(upload(C, &C->nbrm, nbrm));
+        CALL(dalloc(C, &C->ap_m, ne));
+        CALL(dalloc(C, &C->rap_m, ne));
+        CALL(dalloc(C, &C->anb_m, L));
+        for (int y = 0; y < 3; y++) {
+          CALL(dalloc(C, &C->b_m[y], ne));
+          CALL(dalloc(C, &C->phi_m[y], int64_t{ne} + nbf));
+        }
+      }
+    }
     levels(row, nbr, 0, ne, false, lev);
     CALL(make_sched(C, lev, ne, &C->gb));
   }
@@ -1621,38 +2062,8 @@ int cfdl_ctx_create(int ne, int nf, int nbf, const int *ef2nb_idx, const int *ef
     C->pc_nlev = 0;
     for (int s = 0; s < nsubd; s++) levels(rowo, nbro, C->co[s], C->co[s + 1], true, lev);
     for (int q = 0; q < ne; q++) C->pc_nlev = std::max(C->pc_nlev, lev[q]);
-    // spatial tiles of about edge^3 cells from the cell centres, per
-    // subdomain (edge 8 measured fastest on the 128^3 cavity among 5..12)
-    const char *te = std::getenv("CFDL_TILE");
-    const int edge = te ? std::max(1, std::atoi(te)) : 8;
-    const char *sp = std::getenv("CFDL_TILE_SPAN");
-    const int span = sp ? std::min(63, std::max(1, std::atoi(sp))) : 16;
-    const char *mode = std::getenv("CFDL_GS_MODE");
-    C->tiled = !(mode && std::string(mode) == "level");
-    std::vector<int> tid(ne);
-    {
-      std::unordered_map<int64_t, int> tid_of;
-      for (int s = 0; s < nsubd; s++) {
-        double lo[3] = {1e300, 1e300, 1e300}, vsum = 0.0;
-        for (int q = C->co[s]; q < C->co[s + 1]; q++) {
-          const int g = l2g[q];
-          const double r[3] = {xc[g], yc[g], zc[g]};
-          for (int i = 0; i < 3; i++) lo[i] = std::min(lo[i], r[i]);
-          vsum += vol[g];
-        }
-        const double h = std::cbrt(vsum / (C->co[s + 1] - C->co[s])) * edge;
-        for (int q = C->co[s]; q < C->co[s + 1]; q++) {
-          const int g = l2g[q];
-          const int64_t bx = static_cast<int64_t>((xc[g] - lo[0]) / h);
-          const int64_t by = static_cast<int64_t>((yc[g] - lo[1]) / h);
-          const int64_t bz = static_cast<int64_t>((zc[g] - lo[2]) / h);
-          const int64_t key = ((static_cast<int64_t>(s) * 4096 + bx) * 4096 + by) * 4096 + bz;
-          auto it = tid_of.find(key);
-          if (it == tid_of.end()) it = tid_of.emplace(key, static_cast<int>(tid_of.size())).first;
-          tid[q] = it->second;
-        }
-      }
-    }
+    std::vector<int> tid;
+    bin_tiles(C->co, l2g, xc, yc, zc, vol, edge, &tid);
     std::vector<int> launch;
     tile_launches(ne, rowo, nbro, lev, tid, span, &launch);
     // renumber the pc system by (subdomain, launch, tile, level): every
@@ -1682,7 +2093,7 @@ int cfdl_ctx_create(int ne, int nf, int nbf, const int *ef2nb_idx, const int *ef
       }
     }
     for (auto &v : xsrc) v = o2n[v];
-    CALL(build_tile_tables(C, rowp, nbrp, sdn, launchn, tidn, levn));
+    CALL(build_tile_tables(C, rowp, nbrp, sdn, launchn, tidn, levn, 1, &C->ts));
     CALL(make_sched(C, levn, ne, &C->pcl));
     C->Lp = static_cast<int>(nbrp.size());
     C->nh = nh;
@@ -1694,6 +2105,7 @@ int cfdl_ctx_create(int ne, int nf, int nbf, const int *ef2nb_idx, const int *ef
     CALL(upload(C, &C->xdst, xdst));
     CALL(upload(C, &C->xsrc, xsrc));
     CALL(dalloc(C, &C->ap_p, ne));
+    CALL(dalloc(C, &C->rap_p, ne));
     CALL(dalloc(C, &C->b_p, ne));
     CALL(dalloc(C, &C->anb_p, C->Lp));
     CALL(dalloc(C, &C->phic_p, int64_t{ne} + nh));
@@ -1742,9 +2154,20 @@ int cfdl_ctx_create_cavity(const cfdl_mesh *m, double rho, double mu, double dt,
     set_error("cfdl_ctx_create_cavity: mesh is not partitioned (call cfdl_mesh_partition)");
     return CFDL_ESTATE;
   }
+  // BCs bind by section name, as make_bc does (mod_eqn_setup.f90:46-67): a
+  // mesh without a 'top' section has no lid, which the reference stops on
   std::vector<int> kind(m->nsec, CFDL_BC_WALL);
+  bool lid = false;
   for (int s = 0; s < m->nsec; s++)
-    if (s < static_cast<int>(m->secname.size()) && m->secname[s] == "top") kind[s] = CFDL_BC_LID;
+    if (s < static_cast<int>(m->secname.size()) && m->secname[s] == "top") {
+      kind[s] = CFDL_BC_LID;
+      lid = true;
+    }
+  if (!lid) {
+    set_error("cfdl_ctx_create_cavity: BC top not found (name the sections with "
+              "cfdl_mesh_set_section_names)");
+    return CFDL_EINVAL;
+  }
   std::vector<double> r(m->ne, rho), u(m->ne, mu);
   return cfdl_ctx_create(m->ne, m->nf, m->nbf, m->ef2nb_idx.data(), m->ef2nb.data(),
                          m->s2g.data(), m->bs.data(), m->xc.data(), m->yc.data(), m->zc.data(),
@@ -1832,8 +2255,62 @@ int cfdl_synchronize(cfdl_ctx *c) {
 
 int cfdl_update_boundaries(cfdl_ctx *c) {
   CALL(check_ctx(c));
-  if (c->nbf > 0)
+  if (c->nbf > 0) {
     hipLaunchKernelGGL(k_bc_update, dim3(nblocks(c->nbf)), dim3(kBlock), 0, c->st, c->md, c->fd);
+    hipLaunchKernelGGL(k_bc_energy, dim3(nblocks(c->nbf)), dim3(kBlock), 0, c->st, c->md, c->en);
+  }
+  HIPCHK(hipGetLastError());
+  return CFDL_OK;
+}
+
+int cfdl_update_boundaries_energy(cfdl_ctx *c) {
+  CALL(check_ctx(c));
+  if (c->nbf > 0)
+    hipLaunchKernelGGL(k_bc_energy, dim3(nblocks(c->nbf)), dim3(kBlock), 0, c->st, c->md, c->en);
+  HIPCHK(hipGetLastError());
+  return CFDL_OK;
+}
+
+int cfdl_set_thermal(cfdl_ctx *c, const double *cp, const double *tc) {
+  CALL(check_ctx(c));
+  if (!cp || !tc) {
+    set_error("cfdl_set_thermal: null array");
+    return CFDL_EINVAL;
+  }
+  HIPCHK(hipMemcpyAsync(const_cast<double *>(c->md.cp), cp, 8 * size_t(c->ne), hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(const_cast<double *>(c->md.tc), tc, 8 * size_t(c->ne), hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return CFDL_OK;
+}
+
+int cfdl_construct_energy(cfdl_ctx *c) {
+  CALL(check_ctx(c));
+  hipLaunchKernelGGL(k_energy_init, dim3(nblocks(int64_t{c->ne} + c->nbf)), dim3(kBlock), 0, c->st,
+                     c->md, c->en);
+  HIPCHK(hipGetLastError());
+  return CFDL_OK;
+}
+
+int cfdl_calc_coef_energy(cfdl_ctx *c) {
+  CALL(check_ctx(c));
+  hipLaunchKernelGGL(k_coef_energy, dim3(nblocks(c->ne)), dim3(kBlock), 0, c->st, c->md, c->fd,
+                     c->en, c->dt);
+  HIPCHK(hipGetLastError());
+  return CFDL_OK;
+}
+
+// solve_energy, mod_energy.f90:58-80: gradients of t and h, coefficients,
+// solve_gs('e') on the enthalpy (SOR factor 1), temperature from enthalpy
+int cfdl_solve_energy(cfdl_ctx *c) {
+  CALL(check_ctx(c));
+  hipLaunchKernelGGL(k_grad, dim3(nblocks(c->ne)), dim3(kBlock), 0, c->st, c->md, c->en.t, c->en.gt);
+  hipLaunchKernelGGL(k_grad, dim3(nblocks(c->ne)), dim3(kBlock), 0, c->st, c->md, c->en.h, c->en.gh);
+  CALL(cfdl_calc_coef_energy(c));
+  double *phi = c->en.h;
+  const double *b = c->fd.b;
+  const char *name = "e";
+  CALL(solve_gs_n(c, 1, &phi, &b, &name, sor_of(name)));
+  hipLaunchKernelGGL(k_temperature, dim3(nblocks(c->ne)), dim3(kBlock), 0, c->st, c->md, c->en);
   HIPCHK(hipGetLastError());
   return CFDL_OK;
 }
@@ -1956,6 +2433,19 @@ int cfdl_solve_uvwp(cfdl_ctx *c) {
   return CFDL_OK;
 }
 
+int cfdl_calc_mass_imbalance(cfdl_ctx *c, int dst) {
+  CALL(check_ctx(c));
+  double *out = field_ptr(c, dst);
+  if (!out || c->fsize[dst] < c->ne) {
+    set_error("cfdl_calc_mass_imbalance: destination must be a field of at least ne values");
+    return CFDL_EINVAL;
+  }
+  hipLaunchKernelGGL(k_mass_imbalance, dim3(nblocks(c->ne)), dim3(kBlock), 0, c->st, c->md,
+                     c->fd.mip, out);
+  HIPCHK(hipGetLastError());
+  return CFDL_OK;
+}
+
 int cfdl_update_time(cfdl_ctx *c) {
   CALL(check_ctx(c));
   const int64_t n = c->ne + c->nbf;
@@ -1964,21 +2454,27 @@ int cfdl_update_time(cfdl_ctx *c) {
   hipLaunchKernelGGL(k_copy, dim3(nblocks(n)), dim3(kBlock), 0, c->st, c->fd.w0, c->fd.w, n);
   hipLaunchKernelGGL(k_copy, dim3(nblocks(c->nf)), dim3(kBlock), 0, c->st, c->fd.mip0, c->fd.mip,
                      int64_t{c->nf});
+  hipLaunchKernelGGL(k_copy, dim3(nblocks(n)), dim3(kBlock), 0, c->st, c->en.h0, c->en.h, n);
   HIPCHK(hipGetLastError());
   return CFDL_OK;
 }
 
-int cfdl_run(cfdl_ctx *c, int ntstep, int ncoef) {
+static int run_loop(cfdl_ctx *c, int ntstep, int ncoef, bool energy) {
   CALL(check_ctx(c));
   for (int t = 0; t < ntstep; t++) {
     for (int k = 0; k < ncoef; k++) {
       CALL(cfdl_update_boundaries(c));
       CALL(cfdl_solve_uvwp(c));
+      if (energy) CALL(cfdl_solve_energy(c));
     }
     CALL(cfdl_update_time(c));
   }
   return CFDL_OK;
 }
+
+int cfdl_run(cfdl_ctx *c, int ntstep, int ncoef) { return run_loop(c, ntstep, ncoef, false); }
+
+int cfdl_run_energy(cfdl_ctx *c, int ntstep, int ncoef) { return run_loop(c, ntstep, ncoef, true); }
 
 static int drain_log(cfdl_ctx *c) {
   HIPCHK(hipStreamSynchronize(c->st));

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <numeric>
@@ -866,3 +867,149 @@ extern "C" int cfdl_mesh_get_f64(const cfdl_mesh *m, const char *name, double *d
 }
 
 extern "C" void cfdl_mesh_free(cfdl_mesh *m) { delete m; }
+
+extern "C" int cfdl_mesh_set_section_names(cfdl_mesh *m, int nsec, const char *const *names) {
+  if (!m || !names || nsec != m->nsec) {
+    cfdl::set_error("cfdl_mesh_set_section_names: one name per section is needed");
+    return CFDL_EINVAL;
+  }
+  m->secname.clear();
+  for (int s = 0; s < nsec; s++) {
+    if (!names[s]) {
+      cfdl::set_error("cfdl_mesh_set_section_names: null name");
+      return CFDL_EINVAL;
+    }
+    m->secname.emplace_back(names[s]);
+  }
+  return CFDL_OK;
+}
+
+// VTU output of cell fields, in the file layout of the reference's
+// write_vtubin / vtu_data (src/modules/mod_vtu_output.f90:6-326): an XML
+// header that lists every array with its byte offset into the appended
+// block, then '_' and the raw little-endian data, each array preceded by its
+// byte count as a 64-bit integer -- the cell arrays (values of the 3D
+// sections, ncomp per cell), the vertex coordinates (x,y,z interleaved), the
+// 0-based connectivity, the running vertex offsets and the VTK cell types
+// (12 = hexahedron), all Int32.
+extern "C" int cfdl_mesh_write_vtu(const cfdl_mesh *m, const char *path, int nvar,
+                                   const char *const *names, const int *ncomp,
+                                   const double *const *data) {
+  using namespace cfdl;
+  if (!m || !path || nvar < 0 || (nvar > 0 && (!names || !ncomp || !data))) {
+    set_error("cfdl_mesh_write_vtu: bad arguments");
+    return CFDL_EINVAL;
+  }
+  std::vector<int> cells;  // 3D sections
+  for (int s = 0; s < m->nsec; s++) {
+    if (m->etype[s] == kQuad4) continue;
+    if (m->etype[s] != kHexa8) {
+      set_error("cfdl_mesh_write_vtu: only HEXA_8 cell sections are supported");
+      return CFDL_EINVAL;
+    }
+    cells.push_back(s);
+  }
+  for (int v = 0; v < nvar; v++)
+    if (!names[v] || !data[v] || ncomp[v] < 1 || ncomp[v] > 9) {
+      set_error("cfdl_mesh_write_vtu: bad array " + std::to_string(v));
+      return CFDL_EINVAL;
+    }
+  int64_t ncell = 0;
+  for (int s : cells) ncell += m->esec[2 * s + 1] - m->esec[2 * s] + 1;
+  // offsets are printed as the reference's I12 edit descriptor prints them:
+  // right-justified in 12 characters, leading blanks kept
+  auto i12 = [](int64_t v) {
+    char buf[32];
+    std::snprintf(buf, sizeof buf, "%12lld", static_cast<long long>(v));
+    return std::string(buf);
+  };
+  std::string h;
+  const char lf = '\n';
+  h += std::string("<?xml version=\"1.0\"?>") + lf;
+  h += std::string("<VTKFile type=\"UnstructuredGrid\" version=\"1.0\" byte_order=\"LittleEndian\" "
+                   "header_type=\"UInt64\">") + lf;
+  h += std::string(" <UnstructuredGrid>") + lf;
+  h += "  <Piece NumberOfPoints=\"" + std::to_string(m->nvx) + "\" NumberOfCells=\"" +
+       std::to_string(m->ne) + "\">" + lf;
+  h += std::string("   <CellData Scalars=\"scalars\">") + lf;
+  int64_t off = 0;
+  std::vector<int64_t> nbytes(nvar);
+  for (int v = 0; v < nvar; v++) {
+    h += "    <DataArray type=\"Float64\" Name=\"" + std::string(names[v]) +
+         "\" NumberOfComponents=\"" + std::to_string(ncomp[v]) +
+         "\" format=\"appended\" offset=\"" + i12(off) + "\" />" + lf;
+    nbytes[v] = static_cast<int64_t>(m->ne) * ncomp[v] * 8;
+    off += 8 + nbytes[v];
+  }
+  h += std::string("   </CellData>") + lf;
+  h += std::string("   <Points>") + lf;
+  h += "    <DataArray type=\"Float64\" NumberOfComponents=\"3\" format=\"appended\" offset=\"" +
+       i12(off) + "\" />" + lf;
+  const int64_t nb_pts = 3 * static_cast<int64_t>(m->nvx) * 8;
+  off += 8 + nb_pts;
+  h += std::string("   </Points>") + lf;
+  h += std::string("   <Cells>") + lf;
+  h += "    <DataArray type=\"Int32\" Name=\"connectivity\" format=\"appended\" offset=\"" +
+       i12(off) + "\" />" + lf;
+  const int64_t nb_conn = ncell * 8 * 4;
+  off += 8 + nb_conn;
+  h += "    <DataArray type=\"Int32\" Name=\"offsets\" format=\"appended\" offset=\"" +
+       i12(off) + "\" />" + lf;
+  const int64_t nb_offs = ncell * 4;
+  off += 8 + nb_offs;
+  h += "    <DataArray type=\"Int32\" Name=\"types\" format=\"appended\" offset=\"" +
+       i12(off) + "\" />" + lf;
+  const int64_t nb_types = static_cast<int64_t>(m->ne) * 4;
+  h += std::string("   </Cells>") + lf;
+  h += std::string("  </Piece>") + lf;
+  h += std::string(" </UnstructuredGrid>") + lf;
+  h += std::string(" <AppendedData encoding=\"raw\">") + lf;
+  h += "_";
+  FILE *f = std::fopen(path, "wb");
+  if (!f) {
+    set_error(std::string("cfdl_mesh_write_vtu: cannot open ") + path);
+    return CFDL_EINVAL;
+  }
+  bool ok = std::fwrite(h.data(), 1, h.size(), f) == h.size();
+  auto put = [&](const void *p, size_t n) { ok = ok && std::fwrite(p, 1, n, f) == n; };
+  for (int v = 0; v < nvar; v++) {
+    put(&nbytes[v], 8);
+    for (int s : cells) {
+      const int64_t e0 = m->esec[2 * s] - 1, e1 = m->esec[2 * s + 1];
+      put(data[v] + e0 * ncomp[v], static_cast<size_t>((e1 - e0) * ncomp[v]) * 8);
+    }
+  }
+  put(&nb_pts, 8);
+  for (int j = 0; j < m->nvx; j++) {
+    const double xyz[3] = {m->x[j], m->y[j], m->z[j]};
+    put(xyz, 24);
+  }
+  put(&nb_conn, 8);
+  for (int s : cells)
+    for (int e = m->esec[2 * s]; e <= m->esec[2 * s + 1]; e++) {
+      int32_t v[8];
+      for (int k = 0; k < 8; k++) v[k] = m->e2vx[8 * static_cast<size_t>(e - 1) + k] - 1;
+      put(v, 32);
+    }
+  put(&nb_offs, 8);
+  int32_t running = 0;
+  for (int s : cells)
+    for (int e = m->esec[2 * s]; e <= m->esec[2 * s + 1]; e++) {
+      running += 8;
+      put(&running, 4);
+    }
+  put(&nb_types, 8);
+  for (int s : cells)
+    for (int e = m->esec[2 * s]; e <= m->esec[2 * s + 1]; e++) {
+      const int32_t t = 12;
+      put(&t, 4);
+    }
+  const std::string tail = std::string(1, lf) + " </AppendedData>" + lf + "</VTKFile>" + lf;
+  put(tail.data(), tail.size());
+  ok = (std::fclose(f) == 0) && ok;
+  if (!ok) {
+    set_error(std::string("cfdl_mesh_write_vtu: write failed: ") + path);
+    return CFDL_EINVAL;
+  }
+  return CFDL_OK;
+}

@@ -14,7 +14,8 @@ module cfdl_c
        CFDL_U0 = 4, CFDL_V0 = 5, CFDL_W0 = 6, CFDL_GU = 7, CFDL_GV = 8, CFDL_GW = 9, &
        CFDL_GP = 10, CFDL_GPC = 11, CFDL_MIP = 12, CFDL_MIP0 = 13, CFDL_BU = 14, &
        CFDL_BV = 15, CFDL_BW = 16, CFDL_D = 17, CFDL_DC = 18, CFDL_AP = 19, CFDL_B = 20, &
-       CFDL_ANB = 21, CFDL_PHIC = 22, CFDL_WORK = 23, CFDL_GWORK = 24
+       CFDL_ANB = 21, CFDL_PHIC = 22, CFDL_WORK = 23, CFDL_GWORK = 24, &
+       CFDL_T = 25, CFDL_GT = 26, CFDL_ENTH = 27, CFDL_ENTH0 = 28, CFDL_GENTH = 29
 
   interface
     function cfdl_last_error() bind(C, name="cfdl_last_error")
@@ -43,6 +44,15 @@ module cfdl_c
       integer(c_int), intent(in) :: etype(*), esec(*), e2vx(*)
       type(c_ptr), intent(out) :: out
       integer(c_int) :: cfdl_mesh_from_elements
+    end function
+    function cfdl_mesh_write_vtu(m, path, nvar, names, ncomp, data) bind(C, name="cfdl_mesh_write_vtu")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: m
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: nvar
+      type(c_ptr), intent(in) :: names(*), data(*)
+      integer(c_int), intent(in) :: ncomp(*)
+      integer(c_int) :: cfdl_mesh_write_vtu
     end function
     function cfdl_mesh_partition(m, nsubd) bind(C, name="cfdl_mesh_partition")
       import :: c_int, c_ptr
@@ -219,6 +229,46 @@ module cfdl_c
       import :: c_int, c_ptr
       type(c_ptr), value :: c
       integer(c_int) :: cfdl_update_time
+    end function
+    function cfdl_calc_mass_imbalance(c, dst) bind(C, name="cfdl_calc_mass_imbalance")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: c
+      integer(c_int), value :: dst
+      integer(c_int) :: cfdl_calc_mass_imbalance
+    end function
+
+    ! ---- energy equation (src/equations/mod_energy.f90) ----
+    function cfdl_set_thermal(c, cp, tc) bind(C, name="cfdl_set_thermal")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: c
+      real(c_double), intent(in) :: cp(*), tc(*)
+      integer(c_int) :: cfdl_set_thermal
+    end function
+    function cfdl_construct_energy(c) bind(C, name="cfdl_construct_energy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: c
+      integer(c_int) :: cfdl_construct_energy
+    end function
+    function cfdl_update_boundaries_energy(c) bind(C, name="cfdl_update_boundaries_energy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: c
+      integer(c_int) :: cfdl_update_boundaries_energy
+    end function
+    function cfdl_calc_coef_energy(c) bind(C, name="cfdl_calc_coef_energy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: c
+      integer(c_int) :: cfdl_calc_coef_energy
+    end function
+    function cfdl_solve_energy(c) bind(C, name="cfdl_solve_energy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: c
+      integer(c_int) :: cfdl_solve_energy
+    end function
+    function cfdl_run_energy(c, ntstep, ncoef) bind(C, name="cfdl_run_energy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: c
+      integer(c_int), value :: ntstep, ncoef
+      integer(c_int) :: cfdl_run_energy
     end function
     function cfdl_run(c, ntstep, ncoef) bind(C, name="cfdl_run")
       import :: c_int, c_ptr

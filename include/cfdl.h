@@ -59,6 +59,9 @@ enum cfdl_field {
   CFDL_PHIC,                                                      /* ne+nbf   */
   CFDL_WORK,   /* ne+nbf scratch: any element field a caller passes in */
   CFDL_GWORK,  /* 3(ne+nbf) scratch gradient */
+  /* energy equation (energy_t, mod_energy.f90:4-10): temperature t, its
+     gradient gt, enthalpy (the equation's phi), phi0, grad */
+  CFDL_T, CFDL_GT, CFDL_ENTH, CFDL_ENTH0, CFDL_GENTH, /* ne+nbf, 3(ne+nbf), ne+nbf x2, 3(ne+nbf) */
   CFDL_NFIELDS
 };
 
@@ -85,6 +88,10 @@ int cfdl_mesh_box(int nx, int ny, int nz, double lx, double ly, double lz, doubl
 int cfdl_mesh_from_elements(int nvx, const double *x, const double *y, const double *z,
                             int nsec, const int *etype, const int *esec, const int *e2vx,
                             cfdl_mesh **out);
+/* Section names, which bind the boundary conditions (the reference's
+   make_bc matches them, mod_eqn_setup.f90:46-67; cfdl_ctx_create_cavity
+   needs a section named "top").  cfdl_mesh_box names its sections itself. */
+int cfdl_mesh_set_section_names(cfdl_mesh *m, int nsec, const char *const *names);
 /* Oct-tree bisection into nsubd subdomains and the subdomain-local order
    (nsubd = 1: no partition, the pc solve uses solve_gs). */
 int cfdl_mesh_partition(cfdl_mesh *m, int nsubd);
@@ -97,6 +104,15 @@ int cfdl_mesh_info(const cfdl_mesh *m, int *ne, int *nf, int *nbf, int *nsec, in
 int cfdl_mesh_get_i32(const cfdl_mesh *m, const char *name, int *dst, int64_t n);
 int cfdl_mesh_get_f64(const cfdl_mesh *m, const char *name, double *dst, int64_t n);
 void cfdl_mesh_free(cfdl_mesh *m);
+/* VTU file of cell arrays on the mesh, byte for byte the layout the
+   reference's write_vtubin / vtu_data writes (src/modules/mod_vtu_output.f90:
+   6-326): nvar Float64 cell arrays (names[v], ncomp[v] components, ne*ncomp
+   values in cell order), then points, connectivity, offsets and types in
+   one appended raw block.  For the uvwp equation the reference writes u, v,
+   w, p, gpc (3 components) and the cell mass imbalance as 'mip'
+   (cfdl_calc_mass_imbalance). */
+int cfdl_mesh_write_vtu(const cfdl_mesh *m, const char *path, int nvar, const char *const *names,
+                        const int *ncomp, const double *const *data);
 
 /* ------------------------------------------------------------------------ */
 /* Device context: the drop-in for the hot path.                             */
@@ -147,7 +163,7 @@ int cfdl_get_field(cfdl_ctx *c, int field, double *dst, int64_t n);
 int cfdl_synchronize(cfdl_ctx *c);
 
 /* One entry point per reference routine, all on device-resident state.    */
-int cfdl_update_boundaries(cfdl_ctx *c);   /* mod_physics.f90:38-50 (uvwp BCs) */
+int cfdl_update_boundaries(cfdl_ctx *c);   /* mod_physics.f90:38-50 (uvwp, energy BCs) */
 int cfdl_calc_coef_uvw(cfdl_ctx *c);       /* mod_uvwp.f90:161-286             */
 /* solve_gs (mod_solver.f90:255-327) on the shared ap/anb for phi = field
    (CFDL_U/V/W or CFDL_PHIC) with right-hand side rhs (CFDL_BU/BV/BW or
@@ -165,7 +181,26 @@ int cfdl_solve(cfdl_ctx *c, const char *name);
 int cfdl_adjust_pc(cfdl_ctx *c);   /* pref = phic(1); mod_uvwp.f90:129-130,136-158 */
 int cfdl_update_uvwp(cfdl_ctx *c); /* mod_uvwp.f90:370-436 */
 int cfdl_solve_uvwp(cfdl_ctx *c);  /* mod_uvwp.f90:95-134, fused */
-int cfdl_update_time(cfdl_ctx *c); /* mod_physics.f90:101-112 (uvwp part) */
+int cfdl_update_time(cfdl_ctx *c); /* mod_physics.f90:101-112 (uvwp, energy) */
+/* Energy equation (src/equations/mod_energy.f90), which the reference's main
+   loop never solves but whose BC setters its update_boundaries applies
+   (mod_physics.f90:44-48).  cfdl_update_boundaries and cfdl_update_time act
+   on it as the reference's update_boundaries / update_time do.  Thermal
+   properties start as init_properties sets them (cp = 1000, tc = 5,
+   mod_properties.f90:86-89) and the state as construct_energy leaves it
+   (t = 273, enthalpy = cp*t, mod_energy.f90:30-38). */
+int cfdl_set_thermal(cfdl_ctx *c, const double *cp, const double *tc); /* ne each */
+int cfdl_construct_energy(cfdl_ctx *c);          /* mod_energy.f90:30-38 */
+int cfdl_update_boundaries_energy(cfdl_ctx *c);  /* lid / dirichlet0, :182-212 */
+int cfdl_calc_coef_energy(cfdl_ctx *c);          /* :82-169, into ap/anb/b */
+int cfdl_solve_energy(cfdl_ctx *c);              /* solve_energy, :58-80 */
+/* cfdl_run with solve_energy after every solve_uvwp */
+int cfdl_run_energy(cfdl_ctx *c, int ntstep, int ncoef);
+
+/* Net outward mass flux of each cell into field dst (ne values): the array
+   the reference's VTU output computes into dc and writes as 'mip'
+   (mod_vtu_output.f90:113-123); dc itself is left alone. */
+int cfdl_calc_mass_imbalance(cfdl_ctx *c, int dst);
 /* ntstep x ncoef outer iterations with the loop nesting of src/main.f90:50-63 */
 int cfdl_run(cfdl_ctx *c, int ntstep, int ncoef);
 

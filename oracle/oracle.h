@@ -81,6 +81,11 @@ typedef struct or_case {
   /* residual log */
   int nlog, caplog;
   or_logline *log;
+  /* energy equation (energy_t, mod_energy.f90:4-10): temperature t and its
+     gradient gt, enthalpy h (the equation's phi), h0, gradient gh; thermal
+     properties cp, tc (properties_t, mod_properties.f90:76-90) */
+  double *cp, *tc;
+  double *t, *gt, *h, *h0, *gh;
 } or_case;
 
 /* ---- setup ---- */
@@ -127,6 +132,16 @@ void or_construct_uvwp(or_case *c); /* zero state, calc_mip(.false.), mip0=mip *
 
 /* run ntstep x ncoef outer iterations exactly as src/main.f90:50-63 */
 void or_run(or_case *c, int ntstep, int ncoef);
+
+/* ---- energy equation (mirror mod_energy.f90; never solved by the
+   reference's main loop, whose update_boundaries still applies its BCs) ---- */
+void or_construct_energy(or_case *c);          /* mod_energy.f90:14-48 */
+void or_update_boundaries_energy(or_case *c);  /* lid / dirichlet0, :182-212 */
+void or_calc_coef_energy(or_case *c);          /* :82-169 */
+void or_solve_energy(or_case *c);              /* :58-80 */
+void or_update_time_energy(or_case *c);        /* mod_physics.f90:111 */
+/* or_run with solve_energy after every solve_uvwp */
+void or_run_energy(or_case *c, int ntstep, int ncoef);
 
 #ifdef __cplusplus
 }

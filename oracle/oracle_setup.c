@@ -673,6 +673,11 @@ void or_init_physics(or_case *c, double rho, double mu, double dt, int nit) {
   double **f0[] = {&c->bu, &c->bv, &c->bw, &c->d, &c->dc, &c->ap, &c->b};
   for (int k = 0; k < 7; k++) *f0[k] = xcalloc(ne, 8);
   c->anb = xcalloc(c->L, 8);
+  /* thermal properties of init_properties, mod_properties.f90:86-89 */
+  c->cp = xcalloc(ne, 8); c->tc = xcalloc(ne, 8);
+  for (int e = 0; e < ne; e++) { c->cp[e] = 1000.0; c->tc[e] = 5.0; }
+  c->t = xcalloc(n, 8); c->h = xcalloc(n, 8); c->h0 = xcalloc(n, 8);
+  c->gt = xcalloc(3 * (size_t)n, 8); c->gh = xcalloc(3 * (size_t)n, 8);
 }
 
 void or_case_free(or_case *c) {
@@ -681,7 +686,8 @@ void or_case_free(or_case *c) {
                   c->ef2nb, c->s2g, c->bs, c->xc, c->yc, c->zc, c->vol, c->aip, c->rip,
                   c->gf2g, c->g2gf, c->g2gf_idx, c->rho, c->mu, c->u, c->v, c->w, c->p,
                   c->u0, c->v0, c->w0, c->gu, c->gv, c->gw, c->gp, c->gpc, c->mip, c->mip0,
-                  c->bu, c->bv, c->bw, c->d, c->dc, c->ap, c->anb, c->b, c->phic, c->log};
+                  c->bu, c->bv, c->bw, c->d, c->dc, c->ap, c->anb, c->b, c->phic, c->log,
+                  c->cp, c->tc, c->t, c->gt, c->h, c->h0, c->gh};
   for (size_t k = 0; k < sizeof(ptrs) / sizeof(ptrs[0]); k++) free(ptrs[k]);
   if (c->sd) {
     for (int k = 0; k < c->nsubd; k++) {

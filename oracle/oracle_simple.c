@@ -574,13 +574,137 @@ void or_construct_uvwp(or_case *c) {
   memcpy(c->mip0, c->mip, 8 * (size_t)c->nf);
 }
 
-/* main loop nesting, src/main.f90:50-63 */
+/* main loop nesting, src/main.f90:50-63; update_boundaries (mod_physics.f90:
+   38-50) applies the uvwp and the energy BC setters */
 void or_run(or_case *c, int ntstep, int ncoef) {
   for (int t = 0; t < ntstep; t++) {
     for (int k = 0; k < ncoef; k++) {
       or_update_boundaries(c);
+      or_update_boundaries_energy(c);
       or_solve_uvwp(c);
     }
     or_update_time(c);
+    or_update_time_energy(c);
+  }
+}
+
+/* ---------------------------------------------------------------- energy */
+
+/* construct_energy, src/equations/mod_energy.f90:14-48: t = 273 K, the
+   enthalpy phi = cp*t on the cells (the halos are set by the BC setters
+   before any use), phi0 = phi, zero gradients */
+void or_construct_energy(or_case *c) {
+  int n = c->ne + c->nbf;
+  for (int e = 0; e < n; e++) c->t[e] = 273.0;
+  for (int e = 0; e < c->ne; e++) c->h[e] = c->t[e] * c->cp[e];
+  memcpy(c->h0, c->h, 8 * (size_t)n);
+  memset(c->gt, 0, 24 * (size_t)n);
+  memset(c->gh, 0, 24 * (size_t)n);
+}
+
+/* energy BC setters lid (t = 373) and dirichlet0 (t = 273), mod_energy.f90:
+   182-212, phi = cp(owner)*t; construct_energy binds 'top' to lid and every
+   other boundary section to dirichlet0 (mod_energy.f90:41-47) */
+void or_update_boundaries_energy(or_case *c) {
+  for (int s = 0; s < c->nsec; s++) {
+    if (c->etype[s] >= 10) continue;
+    double tb = c->bckind[s] == OR_BC_LID ? 373.0 : 273.0;
+    for (int e = c->esec[2 * s]; e <= c->esec[2 * s + 1]; e++) {
+      int enb = PK_E(abs(c->bs[e - c->ne - 1]));
+      c->t[e - 1] = tb;
+      c->h[e - 1] = c->cp[enb - 1] * c->t[e - 1];
+    }
+  }
+}
+
+/* calc_coef_energy, src/equations/mod_energy.f90:82-169 */
+void or_calc_coef_energy(or_case *c) {
+  int ne = c->ne;
+  const int *g1 = c->ef2nb, *g2 = c->ef2nb + c->L;
+  for (int e = 1; e <= ne; e++) {
+    c->ap[e - 1] = 0.0;
+    double sumf = 0.0, sumdefc = 0.0;
+    double rp[3] = {c->xc[e - 1], c->yc[e - 1], c->zc[e - 1]};
+    for (int idx = c->ef2nb_idx[e - 1]; idx <= c->ef2nb_idx[e] - 1; idx++) {
+      c->anb[idx - 1] = 0.0;
+      int enb = PK_E(g1[idx - 1]), lfnb = PK_F(g1[idx - 1]);
+      if (lfnb == 0) continue;
+      int fg = g2[idx - 1];
+      double sg = fg >= 0 ? 1.0 : -1.0;
+      fg = abs(fg);
+      const double *a = c->aip + 3 * (fg - 1), *rip = c->rip + 3 * (fg - 1);
+      double area = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+      double nrm[3], rpnb[3] = {c->xc[enb - 1], c->yc[enb - 1], c->zc[enb - 1]}, dr[3];
+      for (int i = 0; i < 3; i++) nrm[i] = sg * a[i] / area;
+      for (int i = 0; i < 3; i++) dr[i] = rpnb[i] - rp[i];
+      double wt = or_vec_weight(rip, rp, rpnb);
+      double f = -(sg * c->mip[fg - 1]);
+      double fnb = f > 0.0 ? f : 0.0;
+      sumf = sumf + f;
+      double tci = (1.0 - wt) * c->tc[e - 1] + wt * c->tc[enb - 1];
+      double cpi = (1.0 - wt) * c->cp[e - 1] + wt * c->cp[enb - 1];
+      double d = tci / cpi / dot3(dr, nrm) * area;
+      double ghi[3], gti[3];
+      for (int i = 0; i < 3; i++) {
+        ghi[i] = (1.0 - wt) * c->gh[3 * (e - 1) + i] + wt * c->gh[3 * (enb - 1) + i];
+        gti[i] = (1.0 - wt) * c->gt[3 * (e - 1) + i] + wt * c->gt[3 * (enb - 1) + i];
+      }
+      sumdefc = sumdefc + tci * area * (dot3(gti, nrm) - dot3(ghi, nrm) / cpi);
+      c->anb[idx - 1] = d + fnb;
+      c->ap[e - 1] = c->ap[e - 1] + d + fnb;
+    }
+    double ap0 = c->rho[e - 1] * c->vol[e - 1] / c->dt;
+    c->ap[e - 1] = c->ap[e - 1] + ap0;
+    c->b[e - 1] = ap0 * c->h0[e - 1] + sumf * c->h[e - 1] + sumdefc;
+  }
+  /* boundary loop: interfaces (sections) in order, halos in order; both
+     setters are 'dirichlet' (f = 0, d = tc*A/ds/cp) */
+  for (int s = 0; s < c->nsec; s++) {
+    if (c->etype[s] >= 10) continue;
+    for (int enb = c->esec[2 * s]; enb <= c->esec[2 * s + 1]; enb++) {
+      int pk = abs(c->bs[enb - c->ne - 1]);
+      int e = PK_E(pk), lf = PK_F(pk);
+      int idx = c->ef2nb_idx[e - 1] + lf - 1;
+      int fg = g2[idx - 1];
+      const double *a = c->aip + 3 * (fg - 1);
+      double area = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+      double nrm[3] = {a[0] / area, a[1] / area, a[2] / area};
+      double dr[3] = {c->xc[enb - 1] - c->xc[e - 1], c->yc[enb - 1] - c->yc[e - 1],
+                      c->zc[enb - 1] - c->zc[e - 1]};
+      double ds = dot3(dr, nrm);
+      double f = 0.0;
+      double d = c->tc[e - 1] * area / ds / c->cp[e - 1];
+      c->ap[e - 1] = c->ap[e - 1] + d + f;
+      c->anb[idx - 1] = c->anb[idx - 1] + d + f;
+    }
+  }
+}
+
+/* solve_energy, src/equations/mod_energy.f90:58-80 (calc_temperature,
+   mod_properties.f90:214-222) */
+void or_solve_energy(or_case *c) {
+  int ne = c->ne, nf = c->nf, nbf = c->nbf;
+  or_calc_grad(c->t, c->gt, c->xc, c->yc, c->zc, c->ef2nb_idx, c->ef2nb, ne, nf, nbf);
+  or_calc_grad(c->h, c->gh, c->xc, c->yc, c->zc, c->ef2nb_idx, c->ef2nb, ne, nf, nbf);
+  or_calc_coef_energy(c);
+  or_solve_gs(c, "e", c->h, c->ap, c->anb, c->b, c->ef2nb_idx, c->ef2nb, ne, nf, nbf, c->nit);
+  for (int e = 0; e < ne; e++) c->t[e] = c->h[e] / c->cp[e];
+}
+
+/* update_time, src/modules/mod_physics.f90:111 */
+void or_update_time_energy(or_case *c) {
+  memcpy(c->h0, c->h, 8 * (size_t)(c->ne + c->nbf));
+}
+
+void or_run_energy(or_case *c, int ntstep, int ncoef) {
+  for (int t = 0; t < ntstep; t++) {
+    for (int k = 0; k < ncoef; k++) {
+      or_update_boundaries(c);
+      or_update_boundaries_energy(c);
+      or_solve_uvwp(c);
+      or_solve_energy(c);
+    }
+    or_update_time(c);
+    or_update_time_energy(c);
   }
 }

@@ -6,18 +6,27 @@
 ! src/main.f90:50-63 through the reference's own mod_physics / mod_uvwp,
 ! times the outer iterations and dumps setup arrays and final fields.
 !
-! usage: refsim mesh.bin out.bin nsubd rho mu dt ntstep ncoef nit
+! usage: refsim mesh.bin out.bin nsubd rho mu dt ntstep ncoef nit [project/]
+! With the optional project path the run ends like src/main.f90:89, with the
+! reference's write_vtubin of the uvwp fields into project/VTK/output-N/.
+! With REFSIM_ENERGY=1 in the environment every outer iteration also calls
+! the reference's solve_energy (src/equations/mod_energy.f90:58-80), which
+! src/main.f90 never calls, and the energy fields are appended to the dump.
 program refsim
   use mod_cell
   use mod_physics
   use mod_solver
+  use mod_vtu_output
   implicit none
   type(geometry_t) :: geom
   type(phys_t) :: phys
   character(len=512) :: meshfile, outfile, arg
+  character(len=180) :: projpath
   integer :: nsubd, ntstep, ncoef, nit, tstep, icoef, u, c, cnb, ne, nbf
   real :: rho, mu, dt
   integer(8) :: t0, t1, rate
+  character(len=8) :: envval
+  logical :: energy
 
   call get_command_argument(1, meshfile)
   call get_command_argument(2, outfile)
@@ -29,6 +38,8 @@ program refsim
   call get_command_argument(8, arg); read(arg,*) ncoef
   call get_command_argument(9, arg); read(arg,*) nit
 
+  call get_environment_variable('REFSIM_ENERGY', envval)
+  energy = trim(envval) == '1'
   phys%n_subdomains = nsubd
   phys%dt = dt
   phys%ntstep = ntstep
@@ -65,6 +76,8 @@ program refsim
       call update_boundaries(phys,geom)
       call solve_uvwp(phys%uvwp,phys%prop,geom,phys%dt,phys%nit,phys%ap,phys%anb,phys%b,phys%phic, &
                       phys%subdomain,phys%intf,phys%n_subdomains)
+      if (energy) call solve_energy(phys%energy,phys%prop,geom,phys%dt,phys%nit,phys%ap,phys%anb, &
+                                    phys%b,phys%phic)
     end do
     call update_time(phys)
   end do
@@ -77,5 +90,14 @@ program refsim
     write(u) q%mip, q%mip0, q%bu, q%bv, q%bw, q%d, q%dc
   end associate
   write(u) phys%ap, phys%b, phys%anb, phys%phic
+  if (energy) then
+    associate(q => phys%energy)
+      write(u) q%t, q%gt(1:3*ne), q%phi, q%phi0(1:ne), q%grad(1:3*ne)
+    end associate
+  end if
   close(u)
+  if (command_argument_count() >= 10) then
+    call get_command_argument(10, projpath)
+    call write_vtubin(phys%uvwp, geom, projpath, tstep)
+  end if
 end program

@@ -6,7 +6,7 @@ meshgen.box_mesh and stores inputs (mesh parameters), setup arrays, final
 fields and the residual lines the reference prints (4 significant digits,
 its es9.3 format) in tests/golden/<case>.npz.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [case ...]
 
 Needs the reference tree to build oracle/_ref; the fixtures it writes are
 data and travel with the repository.
@@ -23,8 +23,11 @@ sys.path.insert(0, os.path.dirname(HERE))
 import meshgen  # noqa: E402
 import refsim_io  # noqa: E402
 
-# name: (n, jitter, nsubd, rho, mu, dt, ntstep, ncoef)
+# name: (n, jitter, nsubd, rho, mu, dt, ntstep, ncoef); energy_* cases also
+# call the reference's solve_energy after every solve_uvwp (REFSIM_ENERGY=1)
 CASES = {
+    "energy6_jitter": (6, 0.15, 4, 1.0, 0.01, 0.01, 2, 2),
+    "energy7_1sd": (7, 0.1, 1, 1.0, 0.01, 0.01, 1, 3),
     "cavity8": (8, 0.0, 4, 1.0, 0.01, 0.01, 2, 3),
     "cavity6_jitter": (6, 0.2, 4, 1.0, 0.01, 0.01, 2, 3),
     "cavity8_jitter_1sd": (8, 0.2, 1, 1.0, 0.01, 0.01, 2, 2),
@@ -35,16 +38,19 @@ CASES = {
 
 KEEP = ["ef2nb_idx", "ef2nb", "s2g", "bs", "xc", "yc", "zc", "vol", "aip", "rip", "gf2g", "g2gf",
         "g2gf_idx", "u", "v", "w", "p", "gu", "gv", "gw", "gp", "gpc", "mip", "mip0", "bu", "bv",
-        "bw", "d", "dc", "ap", "b", "anb", "phic"]
+        "bw", "d", "dc", "ap", "b", "anb", "phic", "t", "gt", "h", "h0", "gh"]
 
 
-def main():
+def main(names):
     for name, (n, jit, nsubd, rho, mu, dt, nt, nc) in CASES.items():
+        if names and name not in names:
+            continue
         mesh = meshgen.box_mesh(n, jitter=jit)
         with tempfile.TemporaryDirectory() as td:
             mf, of = os.path.join(td, "m.bin"), os.path.join(td, "o.bin")
             meshgen.write_refsim_mesh(mesh, mf)
-            log, _ = refsim_io.run_refsim(mf, of, nsubd, rho, mu, dt, nt, nc, opt="O2")
+            log, _ = refsim_io.run_refsim(mf, of, nsubd, rho, mu, dt, nt, nc, opt="O2",
+                                          energy=name.startswith("energy"))
             d = refsim_io.read_dump(of)
         out = {k: d[k] for k in KEEP if k in d}
         out["params"] = np.array([n, jit, nsubd, rho, mu, dt, nt, nc], np.float64)
@@ -56,4 +62,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

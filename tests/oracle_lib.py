@@ -47,7 +47,8 @@ class OrCase(C.Structure):
                 ("mip", PD), ("mip0", PD),
                 ("bu", PD), ("bv", PD), ("bw", PD), ("d", PD), ("dc", PD),
                 ("ap", PD), ("anb", PD), ("b", PD), ("phic", PD),
-                ("nlog", C.c_int), ("caplog", C.c_int), ("log", C.POINTER(OrLog))]
+                ("nlog", C.c_int), ("caplog", C.c_int), ("log", C.POINTER(OrLog)),
+                ("cp", PD), ("tc", PD), ("t", PD), ("gt", PD), ("h", PD), ("h0", PD), ("gh", PD)]
 
 
 _lib = None
@@ -76,8 +77,11 @@ def lib():
         L.or_matinv3.argtypes = [PD, PD]
         L.or_calc_grad.argtypes = [PD, PD, PD, PD, PD, PI, PI, C.c_int, C.c_int, C.c_int]
         for fn in ("or_update_boundaries", "or_calc_coef_uvw", "or_calc_coef_p", "or_solve_uvwp",
-                   "or_update_time", "or_construct_uvwp"):
+                   "or_update_time", "or_construct_uvwp", "or_construct_energy",
+                   "or_update_boundaries_energy", "or_calc_coef_energy", "or_solve_energy",
+                   "or_update_time_energy"):
             getattr(L, fn).argtypes = [P]
+        L.or_run_energy.argtypes = [P, C.c_int, C.c_int]
         L.or_calc_mip.argtypes = [P, C.c_int]
         L.or_adjust_pc.argtypes = [P, PD, C.c_double]
         L.or_update_uvwp.argtypes = [P, PD]
@@ -113,6 +117,7 @@ class OracleCase:
             L.or_construct_subdomains(self.p)
         L.or_init_physics(self.p, rho, mu, dt, nit)
         L.or_construct_uvwp(self.p)
+        L.or_construct_energy(self.p)
 
     def __del__(self):
         if getattr(self, "p", None) is not None and _lib is not None:
@@ -136,7 +141,8 @@ class OracleCase:
         n = ne + nbf
         s = dict(ef2nb_idx=ne + 1, ef2nb=2 * L, s2g=nf, bs=nbf, xc=n, yc=n, zc=n, vol=ne,
                  aip=3 * nf, rip=3 * nf, gf2g=ne, g2gf=ne, g2gf_idx=c.nsubd + 1, rho=ne, mu=ne,
-                 mip=nf, mip0=nf, ap=ne, b=ne, anb=L, phic=n)
+                 mip=nf, mip0=nf, ap=ne, b=ne, anb=L, phic=n, cp=ne, tc=ne, t=n, h=n, h0=n,
+                 gt=3 * n, gh=3 * n)
         for k in ("u", "v", "w", "p", "u0", "v0", "w0"):
             s[k] = n
         for k in ("gu", "gv", "gw", "gp", "gpc"):
@@ -154,8 +160,8 @@ class OracleCase:
         self.c.alpha_u = alpha_u
         self.c.alpha_p = alpha_p
 
-    def run(self, ntstep, ncoef):
-        lib().or_run(self.p, ntstep, ncoef)
+    def run(self, ntstep, ncoef, energy=False):
+        (lib().or_run_energy if energy else lib().or_run)(self.p, ntstep, ncoef)
 
     def subdomain(self, k):
         s = self.c.sd[k]

@@ -30,18 +30,23 @@ def _unlimited_stack():
 
 
 def run_refsim(meshfile, outfile, nsubd, rho, mu, dt, ntstep, ncoef, nit=100, opt="O2", timeout=600,
-               binary=None):
+               binary=None, project=None, energy=False):
+    """project: a directory path ending in '/'; the run then ends with the
+    reference's write_vtubin into project/VTK/output-<ntstep+1>/uvwp.vtu."""
     args = [binary or refsim_path(opt), meshfile, outfile, str(nsubd), repr(float(rho)), repr(float(mu)),
             repr(float(dt)), str(ntstep), str(ncoef), str(nit)]
+    if project is not None:
+        args.append(project)
+    env = dict(os.environ, REFSIM_ENERGY="1" if energy else "0")
     p = subprocess.run(args, capture_output=True, text=True, timeout=timeout,
-                       preexec_fn=_unlimited_stack)
+                       preexec_fn=_unlimited_stack, env=env)
     if p.returncode != 0:
         raise RuntimeError("refsim failed: %s\n%s" % (p.returncode, p.stderr[-2000:]))
     log, secs = [], None
     run_refsim.last_stdout = p.stdout
     for line in p.stdout.splitlines():
         m = _LINE.match(line)
-        if m and m.group(1) in ("u", "v", "w", "pc"):
+        if m and m.group(1) in ("u", "v", "w", "pc", "e"):
             log.append((m.group(1), int(m.group(2)), float(m.group(3)), float(m.group(4)),
                         float(m.group(5))))
         if line.startswith("refsim: seconds"):
@@ -102,5 +107,11 @@ def read_dump(path):
     d["b"] = r.take("<f8", ne)
     d["anb"] = r.take("<f8", L)
     d["phic"] = r.take("<f8", n)
+    if r.off < len(r.buf):  # energy fields (REFSIM_ENERGY=1)
+        d["t"] = r.take("<f8", n)
+        d["gt"] = r.take("<f8", 3 * ne)
+        d["h"] = r.take("<f8", n)
+        d["h0"] = r.take("<f8", ne)
+        d["gh"] = r.take("<f8", 3 * ne)
     assert r.off == len(r.buf), (r.off, len(r.buf))
     return d

@@ -29,27 +29,38 @@ KEYS = ["ef2nb_idx", "ef2nb", "s2g", "bs", "xc", "vol", "aip", "u", "v", "w", "p
 
 
 def solver_lines(stdout):
-    return [l for l in stdout.splitlines() if l.strip().split(" ")[0] in ("u", "v", "w", "pc")]
+    return [l for l in stdout.splitlines() if l.strip().split(" ")[0] in ("u", "v", "w", "pc", "e")]
 
 
 @pytest.mark.skipif(not (os.path.exists(DROPSIM) and refsim_io.have_refsim("O2")),
                     reason="reference / drop-in builds absent")
-@pytest.mark.parametrize("n,jit,nsubd,nt,nc", [(8, 0.0, 4, 2, 3), (7, 0.2, 4, 1, 3), (6, 0.1, 1, 2, 2)])
-def test_dropin_modules_reproduce_the_reference(n, jit, nsubd, nt, nc):
+@pytest.mark.parametrize("n,jit,nsubd,nt,nc,energy", [(8, 0.0, 4, 2, 3, False), (7, 0.2, 4, 1, 3, False),
+                                                      (6, 0.1, 1, 2, 2, False), (6, 0.15, 4, 2, 2, True),
+                                                      (5, 0.0, 1, 1, 3, True)])
+def test_dropin_modules_reproduce_the_reference(n, jit, nsubd, nt, nc, energy):
+    """Both programs also end with the reference's own write_vtubin, which
+    reads the drop-in's uvwp_t: the VTU files must be identical.  With
+    energy, every outer iteration also calls solve_energy (the drop-in
+    mod_energy on the GPU in dropsim)."""
     g = meshgen.box_mesh(n, jitter=jit)
+    vtu = os.path.join("VTK", "output-%d" % (nt + 1), "uvwp.vtu")
     with tempfile.TemporaryDirectory() as td:
         mf = os.path.join(td, "m.bin")
         meshgen.write_refsim_mesh(g, mf)
-        refsim_io.run_refsim(mf, os.path.join(td, "r.bin"), nsubd, 1.0, 0.01, 0.01, nt, nc)
+        refsim_io.run_refsim(mf, os.path.join(td, "r.bin"), nsubd, 1.0, 0.01, 0.01, nt, nc,
+                             project=os.path.join(td, "r") + "/", energy=energy)
         ref_out = refsim_io.run_refsim.last_stdout
         refsim_io.run_refsim(mf, os.path.join(td, "d.bin"), nsubd, 1.0, 0.01, 0.01, nt, nc,
-                             binary=DROPSIM)
+                             binary=DROPSIM, project=os.path.join(td, "d") + "/", energy=energy)
         drop_out = refsim_io.run_refsim.last_stdout
         r = refsim_io.read_dump(os.path.join(td, "r.bin"))
         d = refsim_io.read_dump(os.path.join(td, "d.bin"))
-    for k in KEYS:
+        rv = open(os.path.join(td, "r", vtu), "rb").read()
+        dv = open(os.path.join(td, "d", vtu), "rb").read()
+    for k in KEYS + (["t", "gt", "h", "h0", "gh"] if energy else []):
         assert np.array_equal(r[k], d[k]), k
     assert solver_lines(ref_out) == solver_lines(drop_out)
+    assert rv == dv
 
 
 @pytest.mark.skipif(not os.path.exists(HOST), reason="cfdlite_gpu not built")

@@ -32,6 +32,8 @@ def assert_fields_equal(eq, oc, names=FIELDS):
         a, b = eq.get(name), oc.arr(name)
         if name == "gpc":
             a, b = a[:3 * oc.c.ne], b[:3 * oc.c.ne]
+        if name == "h0":  # construct_energy leaves the halo part of phi0 unset
+            a, b = a[:oc.c.ne], b[:oc.c.ne]
         assert np.array_equal(a, b), "%s: max |diff| %g" % (name, np.max(np.abs(a - b)))
 
 
@@ -181,3 +183,132 @@ def test_sweep_schedules_bitwise(monkeypatch, tile, mode, lds, span, pipe):
     oc.run(1, 2)
     assert_fields_equal(eq, oc, ["u", "v", "w", "p", "mip", "gp", "phic"])
     assert_logs_match(eq.log(), oc.log())
+
+
+def _scale_rows(oc, rows, s, rhs):
+    """Multiplies rows of the shared system by the power of two s: ap, the
+    row's anb entries and the right-hand sides (names in rhs).  The solution
+    of every row is unchanged, bit for bit."""
+    idx = oc.arr("ef2nb_idx")
+    ap, anb = oc.arr("ap"), oc.arr("anb")
+    for r in rows:
+        ap[r] *= s
+        anb[idx[r] - 1:idx[r + 1] - 1] *= s
+        for name in rhs:
+            oc.arr(name)[r] *= s
+
+
+def _tiny_rows(oc, rows, names, seed):
+    """Decouples rows from their neighbours (anb = 0) and gives them values
+    of 2^-1000 to 2^-1060 in the arrays named: their Gauss-Seidel updates
+    then divide tiny numbers in every sweep, with quotients near and below
+    the least normal double, and their last bits survive to the end."""
+    rng = np.random.default_rng(seed)
+    idx, anb = oc.arr("ef2nb_idx"), oc.arr("anb")
+    for r in rows:
+        anb[idx[r] - 1:idx[r + 1] - 1] = 0.0
+        for name in names:
+            oc.arr(name)[r] = np.ldexp(1.0 + rng.random(), -1000 - int(rng.integers(61)))
+
+
+@pytest.mark.parametrize("case", ["tiny_rows", "small_rows", "large_rows"])
+def test_pc_sweep_division_fallback(monkeypatch, case):
+    """The LDS sweep divides by precomputed reciprocals of ap and sor only
+    where that is exact: dividends of frexp exponent within +-750 and
+    divisors within [2^-64, 2^64].  A batch that meets anything else is
+    recomputed with the hardware division: here decoupled rows with
+    right-hand sides near 2^-1030 (where about one chained reciprocal
+    quotient in ten is not correctly rounded) and rows whose ap is scaled
+    out of the divisor range.  (Dividends beyond 2^750 cannot reach a sweep:
+    the initial residual, a root mean square, overflows and the solver
+    stops before it.)  The
+    pressure-correction solve stays bitwise equal to the oracle's.  Tile 3
+    puts such rows and ordinary ones into different batches."""
+    monkeypatch.setenv("CFDL_TILE", "3")
+    eq, oc = make_pair(10, 0.1, 4)
+    eq.run(1, 2)
+    oc.run(1, 2)
+    eq.calc_mip(True)
+    _ocall(oc, "or_calc_mip", 1)
+    eq.calc_coef_p()
+    _ocall(oc, "or_calc_coef_p")
+    ne = oc.c.ne
+    if case == "tiny_rows":
+        _tiny_rows(oc, range(3, ne, 7), ["b"], 5)
+    else:
+        _scale_rows(oc, range(3, ne, 7), 2.0 ** (-100 if case == "small_rows" else 100), ["b"])
+    for name in ("ap", "anb", "b"):
+        eq.set(name, oc.arr(name))
+    eq.set_a_0("phic")
+    oc.arr("phic")[:] = 0.0
+    eq.solve("pc")
+    oracle_lib.lib().or_solve(oc.p, b"pc", oc.c.phic)
+    assert_fields_equal(eq, oc, ["phic"])
+    assert np.count_nonzero(oc.arr("phic")[:ne]) > ne // 2
+
+
+@pytest.mark.parametrize("case", ["tiny_rows", "small_rows"])
+def test_momentum_sweep_division_fallback(monkeypatch, case):
+    """The three momentum solves share one LDS walk; its reciprocal division
+    falls back to the hardware division batch by batch as the pressure-
+    correction sweep does (decoupled rows with values near 2^-1030, or rows
+    scaled by 2^-100).  u, v and w equal the oracle's solve_gs results."""
+    import ctypes as C
+    monkeypatch.setenv("CFDL_TILE", "3")
+    eq, oc = make_pair(10, 0.1, 4)
+    eq.run(1, 2)
+    oc.run(1, 2)
+    eq.update_boundaries()
+    _ocall(oc, "or_update_boundaries")
+    eq.calc_coef_uvw()
+    _ocall(oc, "or_calc_coef_uvw")
+    if case == "tiny_rows":
+        _tiny_rows(oc, range(2, oc.c.ne, 5), ["bu", "bv", "bw", "u", "v", "w"], 6)
+    else:
+        _scale_rows(oc, range(2, oc.c.ne, 5), 2.0 ** -100, ["bu", "bv", "bw"])
+    for name in ("ap", "anb", "bu", "bv", "bw", "u", "v", "w"):
+        eq.set(name, oc.arr(name))
+    eq.solve_gs_uvw()
+    L, c = oracle_lib.lib(), oc.c
+    L.or_solve_gs.argtypes = [C.POINTER(oracle_lib.OrCase), C.c_char_p, oracle_lib.PD, oracle_lib.PD,
+                              oracle_lib.PD, oracle_lib.PD, oracle_lib.PI, oracle_lib.PI,
+                              C.c_int, C.c_int, C.c_int, C.c_int]
+    for phi, b in (("u", "bu"), ("v", "bv"), ("w", "bw")):
+        L.or_solve_gs(oc.p, phi.encode(), getattr(c, phi), c.ap, c.anb, getattr(c, b),
+                      c.ef2nb_idx, c.ef2nb, c.ne, c.nf, c.nbf, c.nit)
+    assert_fields_equal(eq, oc, ["u", "v", "w"])
+
+
+@pytest.mark.parametrize("n,jit,nsubd,nt,nc", [(6, 0.0, 4, 2, 2), (8, 0.2, 4, 1, 3), (7, 0.1, 1, 2, 2)])
+def test_energy_equation_bitwise(n, jit, nsubd, nt, nc):
+    """The energy equation solved after every solve_uvwp (src/equations/
+    mod_energy.f90:58-80) against the oracle's restatement, which is pinned to
+    the reference's own solve_energy: temperature, enthalpy, gradients, the
+    coefficient arrays it leaves and the 'e' solver lines."""
+    eq, oc = make_pair(n, jit, nsubd)
+    eq.run(nt, nc, energy=True)
+    oc.run(nt, nc, energy=True)
+    assert_fields_equal(eq, oc, ["t", "gt", "h", "h0", "gh", "u", "v", "w", "p", "ap", "anb", "b"])
+    assert_logs_match(eq.log(), oc.log())
+
+
+def test_energy_routines_one_by_one():
+    """BC setters, coefficients with non-uniform thermal properties, and the
+    temperature update, each against the oracle's routine on the same state."""
+    eq, oc = make_pair(6, 0.15, 4)
+    eq.run(1, 2)
+    oc.run(1, 2)
+    rng = np.random.default_rng(3)
+    ne = oc.c.ne
+    cp, tc = 900.0 + 200.0 * rng.random(ne), 2.0 + 6.0 * rng.random(ne)
+    eq.set_thermal(cp, tc)
+    oc.arr("cp")[:] = cp
+    oc.arr("tc")[:] = tc
+    eq.construct_energy()
+    oracle_lib.lib().or_construct_energy(oc.p)
+    eq.update_boundaries_energy()
+    oracle_lib.lib().or_update_boundaries_energy(oc.p)
+    assert_fields_equal(eq, oc, ["t", "h"])
+    eq.solve_energy()
+    oracle_lib.lib().or_solve_energy(oc.p)
+    assert_fields_equal(eq, oc, ["t", "gt", "h", "gh", "ap", "anb", "b"])

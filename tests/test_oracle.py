@@ -50,10 +50,14 @@ def test_oracle_matches_reference_goldens(path):
     if "g2gf" in g:
         for k in ("gf2g", "g2gf", "g2gf_idx"):
             assert np.array_equal(oc.arr(k), g[k]), k
-    oc.run(nt, nc)
+    energy = "t" in g
+    oc.run(nt, nc, energy=energy)
     for k in FIELDS:
         assert np.array_equal(oc.arr(k), g[k]), k
     assert np.array_equal(oc.arr("gpc")[:len(g["gpc"])], g["gpc"])
+    if energy:  # the reference's solve_energy after every solve_uvwp
+        for k in ("t", "gt", "h", "h0", "gh"):
+            assert np.array_equal(oc.arr(k)[:len(g[k])], g[k]), k
     log = oc.log()
     assert [l[0] for l in log] == list(g["log_name"])
     assert [l[1] for l in log] == list(g["log_it"])
@@ -373,3 +377,54 @@ def test_pressure_relaxation_scales_the_correction():
                        0.3 * (a.arr("gp")[:3 * ne] - ga[:3 * ne]), rtol=1e-12, atol=1e-14)
     assert np.array_equal(a.arr("mip"), b.arr("mip"))
     assert np.array_equal(a.arr("p")[ne:], pa[ne:])  # halos are not touched
+
+
+@pytest.mark.skipif(not refsim_io.have_refsim("O2"), reason="reference build absent")
+@pytest.mark.parametrize("n,jit,nsubd,nt,nc", [(5, 0.0, 4, 2, 2), (8, 0.2, 4, 1, 3), (6, 0.1, 1, 2, 2)])
+def test_oracle_energy_matches_reference_live(n, jit, nsubd, nt, nc):
+    """The energy equation (never solved by the reference's main loop) solved
+    after every solve_uvwp by the reference's own solve_energy and by the
+    oracle: temperature, enthalpy, both gradients, the shared coefficient
+    arrays it leaves and the 'e' solver lines."""
+    g = meshgen.box_mesh(n, jitter=jit)
+    with tempfile.TemporaryDirectory() as td:
+        mf = os.path.join(td, "m.bin")
+        meshgen.write_refsim_mesh(g, mf)
+        log, _ = refsim_io.run_refsim(mf, os.path.join(td, "r.bin"), nsubd, 1.0, 0.01, 0.01, nt, nc,
+                                      energy=True)
+        d = refsim_io.read_dump(os.path.join(td, "r.bin"))
+    oc = oracle_lib.OracleCase(g, meshgen.cavity_bc(g), nsubd=nsubd, rho=1.0, mu=0.01, dt=0.01)
+    oc.run(nt, nc, energy=True)
+    for k in ("t", "gt", "h", "h0", "gh", "u", "p", "ap", "anb", "b"):
+        assert np.array_equal(oc.arr(k)[:len(d[k])], d[k]), k
+    el = [l for l in oc.log() if l[0] == "e"]
+    rl = [l for l in log if l[0] == "e"]
+    assert [l[1] for l in el] == [l[1] for l in rl]
+    for a, b in zip(el, rl):
+        assert es93(a[2]) == b[2] and es93(a[3]) == b[3]
+
+
+def test_energy_closed_forms():
+    """Energy assembly on a uniform box at rest (mip = 0): no convection, so
+    ap - sum(anb) = rho*V/dt exactly where a cell has no boundary face; the
+    interior diffusion coefficient is tc/cp*h (wt = 1/2, |dr| = h, A = h^2);
+    the boundary one tc*A/ds/cp with ds = h/2; and b = ap0*h0 with zero
+    deferred correction while the gradients are zero."""
+    n = 5
+    g = meshgen.box_mesh(n)
+    oc = oracle_lib.OracleCase(g, meshgen.cavity_bc(g), nsubd=1, rho=1.0, mu=0.01, dt=0.01)
+    lib = oracle_lib.lib()
+    lib.or_update_boundaries_energy(oc.p)
+    lib.or_calc_coef_energy(oc.p)
+    c = oc.c
+    h = 1.0 / n
+    idx, ap, anb, b = oc.arr("ef2nb_idx"), oc.arr("ap"), oc.arr("anb"), oc.arr("b")
+    nb = oc.arr("ef2nb")[:c.L]
+    ap0 = 1.0 * h ** 3 / 0.01
+    for e in range(c.ne):
+        row = slice(idx[e] - 1, idx[e + 1] - 1)
+        halo = (nb[row] & 31) == 0
+        assert np.allclose(anb[row][~halo], 5.0 / 1000.0 * h, rtol=1e-12)
+        assert np.allclose(anb[row][halo], 5.0 * h * h / (h / 2) / 1000.0, rtol=1e-12)
+        assert ap[e] == pytest.approx(anb[row].sum() + ap0, rel=1e-12)
+        assert b[e] == pytest.approx(ap0 * 273.0 * 1000.0, rel=1e-14)

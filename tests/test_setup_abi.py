@@ -113,3 +113,13 @@ def test_bench_refuses_several_gpus():
     p = subprocess.run([sys.executable, os.path.join(REPO, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=120)
     assert p.returncode != 0 and "one GPU" in p.stderr
+
+
+def test_cavity_binding_needs_a_top_section():
+    """BCs bind by section name (mod_eqn_setup.f90:46-67): a mesh whose
+    sections carry no names has no 'top', and the cavity binding refuses it
+    (before it looks for a GPU) instead of binding every section as a wall."""
+    g = meshgen.box_mesh(4)
+    mesh = cfdl.Mesh.from_elements(g["x"], g["y"], g["z"], g["etype"], g["esec"], g["e2vx"]).partition(4)
+    with pytest.raises(cfdl.CfdlError, match="top"):
+        cfdl.UVWP.cavity(mesh)

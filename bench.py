@@ -165,8 +165,8 @@ def main():
                     help="measured HBM KiB per sweep launch (rocprofv3 FETCH_SIZE + WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
-              "(the multi-GPU sharding is a design in DESIGN.md, not built)" % args.gpus,
+        print("bench.py: runs on one GPU only; --gpus %d is not supported "
+              "(the solver does not shard the mesh over GPUs)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
 
@@ -224,7 +224,7 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "algorithmic_bytes_per_launch": round(BYTES_PER_CELL_SWEEP * cells),
-                "kernel": "k_gs_tiles_lds<true> (pressure-correction Gauss-Seidel, space-time tiles in LDS)",
+                "kernel": "k_gs_tiles_lds<true, true, 1> (pressure-correction Gauss-Seidel, space-time tiles in LDS)",
                 "launch_ms": round(kernel_ms, 5), "cells_per_launch": round(cells, 1),
                 "sampled_launches": int(st["sampled_launches"])}
     base = None if args.no_cpu else cpu_baseline(args.cpu_n, args.cpu_iters, mu)

@@ -694,11 +694,15 @@ k_gs_tiles(SysDesc s, const int *__restrict__ bseg, const int *__restrict__ seg,
 constexpr int kLdsThreads = 64;
 template <bool HEX, bool PIPE, int NC>
 __global__ void __launch_bounds__(kLdsThreads)
-k_gs_tiles_lds(SysDesc s, const int *__restrict__ bseg, const int *__restrict__ seg,
-               const unsigned short *__restrict__ lnbr, const int *__restrict__ xptr,
-               const int *__restrict__ xslot, int batch0, int backward, double sor,
-               double rsor, const int *__restrict__ active) {
+k_gs_tiles_lds(SysDesc s, const int4 *__restrict__ bdesc, const int *__restrict__ seg,
+               const unsigned short *__restrict__ lnbr, const int *__restrict__ xslot,
+               int batch0, int backward, double sor, double rsor,
+               const int *__restrict__ active) {
   extern __shared__ double lds[];
+  // the batch's descriptor (segment, cell, CSR and outside-neighbour ranges)
+  // comes in one load, issued with the active flags
+  const int b = batch0 + blockIdx.x;
+  const int4 d0 = bdesc[2 * b], d1 = bdesc[2 * b + 1];
   // NC right-hand sides share ap/anb (u, v, w of a momentum solve); a
   // right-hand side whose solve has stopped is neither updated nor written
   bool act[NC];
@@ -708,12 +712,10 @@ k_gs_tiles_lds(SysDesc s, const int *__restrict__ bseg, const int *__restrict__ 
     act[c] = active[c] != 0;
     any = any || act[c];
   }
-  if (!any) return;
-  const int b = batch0 + blockIdx.x;
-  const int s0 = bseg[b], nq = bseg[b + 1] - s0 - 1;
-  const int c0 = seg[s0], nc = seg[s0 + nq] - c0;
-  const int r0 = s.row[c0], nent = s.row[c0 + nc] - r0;
-  const int x0 = xptr[b], nx = xptr[b + 1] - x0;
+  // (no batch is empty: testing nc here makes both loads precede the branch)
+  if (!any | (d0.w == 0)) return;
+  const int s0 = d0.x, nq = d0.y, c0 = d0.z, nc = d0.w;
+  const int r0 = d1.x, nent = d1.y, x0 = d1.z, nx = d1.w;
   const int np = nc + nx;
   double *lphi = lds;               // NC x (nc + nx) values
   double *lap = lphi + NC * np;     // nc
@@ -725,7 +727,9 @@ k_gs_tiles_lds(SysDesc s, const int *__restrict__ bseg, const int *__restrict__ 
   unsigned short *lnb = reinterpret_cast<unsigned short *>(lrow + (HEX ? 0 : nc + 1));
   const double *pa = s.ap + c0, *pr = s.rap + c0, *pn = s.anb + r0;
   const unsigned short *pl = lnbr + r0;
-  const int sg = threadIdx.x <= nq ? seg[s0 + threadIdx.x] - c0 : 0;
+  // segment bound of lane q (lanes past nq reload the last one); used after
+  // the gather, so its load overlaps the gather's
+  const int sg_raw = seg[s0 + min(static_cast<int>(threadIdx.x), nq)];
   // gather: each round covers U*64 cells (values, coefficients, slots) and
   // 512 outside neighbours; every load of a round is issued before its LDS
   // stores, so a round costs two memory latencies (the outside indices, then
@@ -805,6 +809,7 @@ k_gs_tiles_lds(SysDesc s, const int *__restrict__ bseg, const int *__restrict__ 
     }
     for (int i = threadIdx.x; i <= nc; i += kLdsThreads) lrow[i] = s.row[c0 + i] - r0;
   }
+  const int sg = sg_raw - c0;
   if (threadIdx.x <= nq) lseg[threadIdx.x] = sg;
   __syncthreads();
   if (PIPE) {
@@ -1224,6 +1229,9 @@ struct cfdl_ctx {
     std::vector<int> launch_ptr;   // batches of launch m: [launch_ptr[m], launch_ptr[m+1])
     std::vector<int> launch_cells;  // cells updated by launch m
     int *bseg = nullptr, *seg = nullptr, *xptr = nullptr, *xslot = nullptr;
+    // per batch: {first segment, segments, first cell, cells, first CSR entry,
+    // entries, first outside neighbour, outside neighbours}
+    int4 *bdesc = nullptr;
     unsigned short *lnbr = nullptr;
     bool hex = false;   // every row has six entries
     bool pipe = false;  // hex, and no level segment has more than 64 cells
@@ -1405,16 +1413,16 @@ static void tiled_launches(cfdl_ctx *c, const cfdl_ctx::TileSched &t, const SysD
     const int b0 = t.launch_ptr[m], nb = t.launch_ptr[m + 1] - b0;
     if (t.pipe)
       hipLaunchKernelGGL((k_gs_tiles_lds<true, true, NC>), dim3(nb), dim3(kLdsThreads),
-                         t.lds_bytes, c->st, sys, t.bseg, t.seg, t.lnbr, t.xptr, t.xslot, b0, bwd,
-                         sor, rsor, active);
+                         t.lds_bytes, c->st, sys, t.bdesc, t.seg, t.lnbr, t.xslot, b0, bwd, sor,
+                         rsor, active);
     else if (t.hex)
       hipLaunchKernelGGL((k_gs_tiles_lds<true, false, NC>), dim3(nb), dim3(kLdsThreads),
-                         t.lds_bytes, c->st, sys, t.bseg, t.seg, t.lnbr, t.xptr, t.xslot, b0, bwd,
-                         sor, rsor, active);
+                         t.lds_bytes, c->st, sys, t.bdesc, t.seg, t.lnbr, t.xslot, b0, bwd, sor,
+                         rsor, active);
     else
       hipLaunchKernelGGL((k_gs_tiles_lds<false, false, NC>), dim3(nb), dim3(kLdsThreads),
-                         t.lds_bytes, c->st, sys, t.bseg, t.seg, t.lnbr, t.xptr, t.xslot, b0, bwd,
-                         sor, rsor, active);
+                         t.lds_bytes, c->st, sys, t.bdesc, t.seg, t.lnbr, t.xslot, b0, bwd, sor,
+                         rsor, active);
   }
 }
 
@@ -1786,6 +1794,13 @@ static int build_tile_tables(cfdl_ctx *c, const std::vector<int> &rowp,
   t.hex = hex;
   const char *pp = std::getenv("CFDL_TILE_PIPE");
   t.pipe = hex && seg_max <= kLdsThreads && !(pp && pp[0] == '0');
+  std::vector<int4> bdesc(2 * static_cast<size_t>(t.nbatch));
+  for (int i = 0; i < t.nbatch; i++) {
+    const int s0 = bseg[i], nq = bseg[i + 1] - s0 - 1, c0 = seg[s0], nc = seg[s0 + nq] - c0;
+    bdesc[2 * i] = make_int4(s0, nq, c0, nc);
+    bdesc[2 * i + 1] = make_int4(rowp[c0], rowp[c0 + nc] - rowp[c0], xptr[i], xptr[i + 1] - xptr[i]);
+  }
+  CALL(upload(c, &t.bdesc, bdesc));
   CALL(upload(c, &t.bseg, bseg));
   CALL(upload(c, &t.seg, seg));
   CALL(upload(c, &t.xptr, xptr));
@@ -1952,7 +1967,7 @@ int cfdl_ctx_create(int ne, int nf, int nbf, const int *ef2nb_idx, const int *ef
   const char *te = std::getenv("CFDL_TILE");
   const int edge = te ? std::max(1, std::atoi(te)) : 8;
   const char *sp = std::getenv("CFDL_TILE_SPAN");
-  const int span = sp ? std::min(63, std::max(1, std::atoi(sp))) : 16;
+  const int span = sp ? std::min(63, std::max(1, std::atoi(sp))) : 20;
   const char *mode = std::getenv("CFDL_GS_MODE");
   C->tiled = !(mode && std::string(mode) == "level");
   const char *rc = std::getenv("CFDL_TILE_RCP");

@@ -692,6 +692,18 @@ k_gs_tiles(SysDesc s, const int *__restrict__ bseg, const int *__restrict__ seg,
 // first LDS store, so the gather costs two memory latencies (the outside
 // indices, then their values).  The segment count is at most 63 (host).
 constexpr int kLdsThreads = 64;
+
+// Orders the LDS accesses of the one wavefront of an LDS-sweep workgroup: a
+// wavefront's LDS instructions execute in issue order, so what one lane
+// stores is seen by every later load of any lane without s_barrier or a
+// wait for the store.  The wavefront-scope fences keep the compiler from
+// moving LDS accesses across this point.
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <bool HEX, bool PIPE, int NC>
 __global__ void __launch_bounds__(kLdsThreads)
 k_gs_tiles_lds(SysDesc s, const int4 *__restrict__ bdesc, const int *__restrict__ seg,
@@ -717,6 +729,8 @@ k_gs_tiles_lds(SysDesc s, const int4 *__restrict__ bdesc, const int *__restrict_
   const int s0 = d0.x, nq = d0.y, c0 = d0.z, nc = d0.w;
   const int r0 = d1.x, nent = d1.y, x0 = d1.z, nx = d1.w;
   const int np = nc + nx;
+  // LDS layout of the plain walk: component-major values, then per-array
+  // coefficient ranges
   double *lphi = lds;               // NC x (nc + nx) values
   double *lap = lphi + NC * np;     // nc
   double *lra = lap + nc;           // nc reciprocals 1/ap
@@ -725,6 +739,21 @@ k_gs_tiles_lds(SysDesc s, const int4 *__restrict__ bdesc, const int *__restrict_
   int *lseg = reinterpret_cast<int *>(lanb + nent);  // nq + 1 bounds relative to c0
   int *lrow = lseg + nq + 1;                          // nc + 1 row starts (general rows)
   unsigned short *lnb = reinterpret_cast<unsigned short *>(lrow + (HEX ? 0 : nc + 1));
+  // LDS layout of the pipelined walk: lds[0..3] takes the stores of lanes
+  // without a cell; values are interleaved by component (slot s, component
+  // c at pphi[NC*s + c]); per cell ap, 1/ap and b (lane-contiguous arrays),
+  // its six anb, and the LDS byte addresses of its six neighbours' values
+  // as 16-bit numbers.  The kernel has no static LDS, so the dynamic block
+  // starts at address 0, pphi at 32, and the host keeps 32 + 8*NC*np within
+  // 16 bits for the batches of this walk.
+  double *pphi = lds + 4;
+  double *pap = pphi + NC * np;
+  double *pra = pap + nc;
+  double *pb = pra + nc;            // NC x nc
+  double *panb = pb + NC * nc;      // 6 per cell
+  unsigned short *psl = reinterpret_cast<unsigned short *>(panb + 6 * nc);  // 6 per cell
+  const unsigned lds_base = static_cast<unsigned>(
+      reinterpret_cast<size_t>((__attribute__((address_space(3))) double *)pphi));
   const double *pa = s.ap + c0, *pr = s.rap + c0, *pn = s.anb + r0;
   const unsigned short *pl = lnbr + r0;
   // segment bound of lane q (lanes past nq reload the last one); used after
@@ -778,18 +807,33 @@ k_gs_tiles_lds(SysDesc s, const int4 *__restrict__ bdesc, const int *__restrict_
     for (int u = 0; u < U; u++) {
       const int i = cb + u * kLdsThreads + threadIdx.x;
       if (i < nc) {
-        lap[i] = va[u];
-        lra[i] = vr[u];
+        if constexpr (PIPE) {
+          pap[i] = va[u];
+          pra[i] = vr[u];
 #pragma unroll
-        for (int c = 0; c < NC; c++) {
-          lphi[c * np + i] = vp[c][u];
-          lb[c * nc + i] = vb[c][u];
-        }
-        if (HEX) {
+          for (int c = 0; c < NC; c++) {
+            pphi[NC * i + c] = vp[c][u];
+            pb[c * nc + i] = vb[c][u];
+          }
 #pragma unroll
           for (int k = 0; k < 6; k++) {
-            lanb[6 * i + k] = vn[u][k];
-            lnb[6 * i + k] = sl[u][k];
+            panb[6 * i + k] = vn[u][k];
+            psl[6 * i + k] = static_cast<unsigned short>(lds_base + 8u * NC * sl[u][k]);
+          }
+        } else {
+          lap[i] = va[u];
+          lra[i] = vr[u];
+#pragma unroll
+          for (int c = 0; c < NC; c++) {
+            lphi[c * np + i] = vp[c][u];
+            lb[c * nc + i] = vb[c][u];
+          }
+          if (HEX) {
+#pragma unroll
+            for (int k = 0; k < 6; k++) {
+              lanb[6 * i + k] = vn[u][k];
+              lnb[6 * i + k] = sl[u][k];
+            }
           }
         }
       }
@@ -799,7 +843,12 @@ k_gs_tiles_lds(SysDesc s, const int4 *__restrict__ bdesc, const int *__restrict_
       const int j = xb + u * kLdsThreads + threadIdx.x;
       if (j < nx)
 #pragma unroll
-        for (int c = 0; c < NC; c++) lphi[c * np + nc + j] = vx[c][u];
+        for (int c = 0; c < NC; c++) {
+          if constexpr (PIPE)
+            pphi[NC * (nc + j) + c] = vx[c][u];
+          else
+            lphi[c * np + nc + j] = vx[c][u];
+        }
     }
   }
   if (!HEX) {
@@ -810,15 +859,17 @@ k_gs_tiles_lds(SysDesc s, const int4 *__restrict__ bdesc, const int *__restrict_
     for (int i = threadIdx.x; i <= nc; i += kLdsThreads) lrow[i] = s.row[c0 + i] - r0;
   }
   const int sg = sg_raw - c0;
-  if (threadIdx.x <= nq) lseg[threadIdx.x] = sg;
+  if (!PIPE && threadIdx.x <= nq) lseg[threadIdx.x] = sg;
   __syncthreads();
-  if (PIPE) {
-    // Every segment has at most 64 cells: lane l updates cell lseg[q] + l of
-    // segment q.  The operands that do not change during the sweep (ap, 1/ap,
-    // b, anb, slots) and the cell's own old values are loaded for the next
-    // segment while the current one computes, so after each barrier only
-    // the neighbour values are read from LDS.  Lanes without a cell load
-    // row 0 and store nothing.  Segment bounds come from lane q of sg.
+  if constexpr (PIPE) {
+    // Every segment has at most 64 cells: lane l updates cell sg[q] + l of
+    // segment q (bounds from lane q of sg).  Two operand sets alternate: while
+    // segment j computes from one, the other receives segment j+1's record
+    // and own old value (its cells are not written at level j); right after
+    // segment j's stores it receives its neighbour values.  The workgroup is
+    // one wavefront, whose LDS accesses execute in issue order, so those
+    // reads see the stores without waiting for them.  Lanes without a cell
+    // read row 0 and store to lds[0..3].
     // RCP: the divisions by ap and sor use div_rcp.  Without a branch in the
     // walk, each lane keeps the least and greatest frexp exponent of its
     // dividends x and a NaN detector (0*nv is NaN for an infinite or NaN
@@ -828,97 +879,96 @@ k_gs_tiles_lds(SysDesc s, const int4 *__restrict__ bdesc, const int *__restrict_
     // original values, which global memory still holds.
     const int lane = threadIdx.x;
     const double sm1 = sor - 1.0;
-    auto walk = [&](bool rcp) -> bool {
-      int emin = 0, emax = 0;
-      double nan_seen = 0.0;
-      int q = backward ? nq - 1 : 0;
-      int i = __builtin_amdgcn_readlane(sg, q) + lane;
-      bool on = i < __builtin_amdgcn_readlane(sg, q + 1);
-      int ic = on ? i : 0;
-      double a = lap[ic], ya = lra[ic], an[6], bv[NC], t[NC];
-      int sl[6];
+    struct Ops {
+      int i;
+      bool on;
+      double a, ya, bv[NC], t[NC], an[6], v[NC][6];
+      unsigned sl[6];
+    };
+    // record and own values of segment qq (no cell when !valid)
+    auto load_ops = [&](int qq, bool valid, Ops &o) {
+      int ii = 0;
+      bool oo = false;
+      if (valid) {
+        ii = __builtin_amdgcn_readlane(sg, qq) + lane;
+        oo = ii < __builtin_amdgcn_readlane(sg, qq + 1);
+      }
+      o.i = ii;
+      o.on = oo;
+      const int ic = oo ? ii : 0;
+      o.a = pap[ic];
+      o.ya = pra[ic];
 #pragma unroll
       for (int c = 0; c < NC; c++) {
-        bv[c] = lb[c * nc + ic];
-        t[c] = sm1 * a * lphi[c * np + ic];
+        o.bv[c] = pb[c * nc + ic];
+        o.t[c] = sm1 * o.a * pphi[NC * ic + c];
       }
 #pragma unroll
       for (int k = 0; k < 6; k++) {
-        sl[k] = lnb[6 * ic + k];
-        an[k] = lanb[6 * ic + k];
+        o.an[k] = panb[6 * ic + k];
+        o.sl[k] = psl[6 * ic + k];
       }
-      for (int j = 0; j < nq; j++) {
-        double v[NC][6];
+    };
+    // neighbour values through LDS addresses formed from the stored byte
+    // offsets (pphi's own LDS address is added once, at the gather)
+    auto load_v = [&](Ops &o) {
+#pragma unroll
+      for (int k = 0; k < 6; k++)
 #pragma unroll
         for (int c = 0; c < NC; c++)
-#pragma unroll
-          for (int k = 0; k < 6; k++) v[c][k] = lphi[c * np + sl[k]];
-        // the next segment's operands (its cells are not written this level)
-        const int qn = backward ? q - 1 : q + 1;
-        int in = 0;
-        bool onn = false;
-        if (j + 1 < nq) {
-          in = __builtin_amdgcn_readlane(sg, qn) + lane;
-          onn = in < __builtin_amdgcn_readlane(sg, qn + 1);
-        }
-        const int icn = onn ? in : 0;
-        const double a2 = lap[icn], ya2 = lra[icn];
-        double an2[6], b2[NC], p2[NC];
-        int sl2[6];
-#pragma unroll
-        for (int c = 0; c < NC; c++) {
-          b2[c] = lb[c * nc + icn];
-          p2[c] = lphi[c * np + icn];
-        }
-#pragma unroll
-        for (int k = 0; k < 6; k++) {
-          sl2[k] = lnb[6 * icn + k];
-          an2[k] = lanb[6 * icn + k];
-        }
+          o.v[c][k] = *reinterpret_cast<const __attribute__((address_space(3))) double *>(
+              static_cast<size_t>(o.sl[k] + 8 * c));
+    };
+    auto walk = [&](bool rcp) -> bool {
+      int emin = 0, emax = 0;
+      double nan_seen = 0.0;
+      // level j of the walk: segment q(j), then the operands of q(j + 1)
+      auto step = [&](int j, Ops &cur, Ops &nxt) {
+        const int qn = backward ? nq - 2 - j : j + 1;
+        load_ops(qn, j + 1 < nq, nxt);
+        // the operand loads go out before this level's arithmetic
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int c = 0; c < NC; c++) {
           if (!act[c]) continue;
-          double sum = bv[c];
+          double sum = cur.bv[c];
 #pragma unroll
-          for (int k = 0; k < 6; k++) sum = sum + an[k] * v[c][k];
-          const double x = sum + t[c];
+          for (int k = 0; k < 6; k++) sum = sum + cur.an[k] * cur.v[c][k];
+          const double x = sum + cur.t[c];
           double nv;
           if (rcp) {
             const int ex = __builtin_amdgcn_frexp_exp(x);  // 0 for x = 0
             emin = min(emin, ex);
             emax = max(emax, ex);
-            nv = div_rcp(div_rcp(x, a, ya), sor, rsor);
+            nv = div_rcp(div_rcp(x, cur.a, cur.ya), sor, rsor);
             nan_seen = fma(0.0, nv, nan_seen);
           } else {
-            nv = x / a / sor;
+            nv = x / cur.a / sor;
           }
-          if (on) lphi[c * np + i] = nv;
+          // unconditional store, so no branch separates it from the next
+          // segment's loads
+          pphi[cur.on ? NC * cur.i + c : c - 4] = nv;
         }
-        __syncthreads();
-        q = qn;
-        i = in;
-        on = onn;
-        a = a2;
-        ya = ya2;
-#pragma unroll
-        for (int c = 0; c < NC; c++) {
-          bv[c] = b2[c];
-          t[c] = sm1 * a2 * p2[c];
-        }
-#pragma unroll
-        for (int k = 0; k < 6; k++) {
-          sl[k] = sl2[k];
-          an[k] = an2[k];
-        }
+        wave_sync();  // the workgroup is one wavefront (kLdsThreads = 64)
+        load_v(nxt);  // past the last segment: row 0's values, unused
+      };
+      Ops A, B;
+      load_ops(backward ? nq - 1 : 0, true, A);
+      load_v(A);
+      int j = 0;
+      for (; j + 1 < nq; j += 2) {
+        step(j, A, B);
+        step(j + 1, B, A);
       }
+      if (j < nq) step(j, A, B);
       return emin < -kRcpExp || emax > kRcpExp || nan_seen != nan_seen;
     };
     if (rsor > 0.0) {
       if (__builtin_amdgcn_ballot_w64(walk(true))) {
         for (int r = threadIdx.x; r < nc; r += kLdsThreads)
 #pragma unroll
-          for (int c = 0; c < NC; c++) lphi[c * np + r] = s.phi[c][c0 + r];
-        __syncthreads();
+          for (int c = 0; c < NC; c++) pphi[NC * r + c] = s.phi[c][c0 + r];
+        wave_sync();
         walk(false);
       }
     } else {
@@ -957,7 +1007,8 @@ k_gs_tiles_lds(SysDesc s, const int4 *__restrict__ bdesc, const int *__restrict_
 #pragma unroll
   for (int c = 0; c < NC; c++) {
     if (!act[c]) continue;
-    for (int r = threadIdx.x; r < nc; r += kLdsThreads) s.phi[c][c0 + r] = lphi[c * np + r];
+    for (int r = threadIdx.x; r < nc; r += kLdsThreads)
+      s.phi[c][c0 + r] = PIPE ? pphi[NC * r + c] : lphi[c * np + r];
   }
 }
 
@@ -1606,12 +1657,12 @@ static int multi_subdomain_solver(cfdl_ctx *c) {
   return CFDL_OK;
 }
 
-// Dense ids of spatial tiles of about edge^3 cells: bins of the cell centres
-// in the bounding box of each group of positions [grp[s], grp[s+1]) (the
-// subdomains of the pc system, or all cells); tiles never cross groups.
-// cell[q] is the mesh cell at position q.
+// Dense ids of spatial tiles of about edge[0] x edge[1] x edge[2] cells
+// (x, y, z): bins of the cell centres in the bounding box of each group of
+// positions [grp[s], grp[s+1]) (the subdomains of the pc system, or all
+// cells); tiles never cross groups.  cell[q] is the mesh cell at position q.
 static void bin_tiles(const std::vector<int> &grp, const std::vector<int> &cell, const double *xc,
-                      const double *yc, const double *zc, const double *vol, int edge,
+                      const double *yc, const double *zc, const double *vol, const int *edge,
                       std::vector<int> *tid) {
   std::unordered_map<int64_t, int> tid_of;
   tid->assign(cell.size(), 0);
@@ -1623,12 +1674,13 @@ static void bin_tiles(const std::vector<int> &grp, const std::vector<int> &cell,
       for (int i = 0; i < 3; i++) lo[i] = std::min(lo[i], r[i]);
       vsum += vol[g];
     }
-    const double h = std::cbrt(vsum / (grp[s + 1] - grp[s])) * edge;
+    const double h = std::cbrt(vsum / (grp[s + 1] - grp[s]));
+    const double hx = h * edge[0], hy = h * edge[1], hz = h * edge[2];
     for (int q = grp[s]; q < grp[s + 1]; q++) {
       const int g = cell[q];
-      const int64_t bx = static_cast<int64_t>((xc[g] - lo[0]) / h);
-      const int64_t by = static_cast<int64_t>((yc[g] - lo[1]) / h);
-      const int64_t bz = static_cast<int64_t>((zc[g] - lo[2]) / h);
+      const int64_t bx = static_cast<int64_t>((xc[g] - lo[0]) / hx);
+      const int64_t by = static_cast<int64_t>((yc[g] - lo[1]) / hy);
+      const int64_t bz = static_cast<int64_t>((zc[g] - lo[2]) / hz);
       const int64_t key = ((static_cast<int64_t>(s) * 4096 + bx) * 4096 + by) * 4096 + bz;
       auto it = tid_of.find(key);
       if (it == tid_of.end()) it = tid_of.emplace(key, static_cast<int>(tid_of.size())).first;
@@ -1719,8 +1771,9 @@ static int build_tile_tables(cfdl_ctx *c, const std::vector<int> &rowp,
   std::vector<int> bseg(t.nbatch + 1, 0), seg, xptr(t.nbatch + 1, 0), xslot;
   std::vector<unsigned short> lnbr(rowp[ne]);
   std::unordered_map<int, int> ext;
-  size_t lds_max = 0;
+  size_t lds_max = 0, lds_pipe_max = 0;
   int seg_max = 0;
+  bool addr16 = true;
   bool fits = true, hex = true;
   for (int e = 0; e < ne; e++) hex = hex && rowp[e + 1] - rowp[e] == 6;
   for (int i = 0; i < t.nbatch; i++) {
@@ -1756,14 +1809,24 @@ static int build_tile_tables(cfdl_ctx *c, const std::vector<int> &rowp,
     }
     xptr[i + 1] = static_cast<int>(xslot.size());
     const size_t nent = rowp[B.c1] - rowp[B.c0];
+    // the two LDS layouts of k_gs_tiles_lds: plain walk, pipelined walk
+    const size_t np = static_cast<size_t>(nc) + ext.size();
     const size_t bytes = 8 * ((2 * nrhs + 2) * static_cast<size_t>(nc) + nrhs * ext.size() + nent) +
                          4 * (nq + 1 + (hex ? 0 : nc + 1)) + 2 * nent;
+    const size_t bytes_pipe = 8 * (4 + nrhs * np + (2 + nrhs + 6) * static_cast<size_t>(nc)) +
+                              2 * 6 * static_cast<size_t>(nc);
+    // the pipelined walk addresses values with 16-bit LDS byte addresses
+    if (32 + 8 * nrhs * np > 0xffff) addr16 = false;
     lds_max = std::max(lds_max, bytes);
+    lds_pipe_max = std::max(lds_pipe_max, bytes_pipe);
   }
   bseg[t.nbatch] = static_cast<int>(seg.size());
   for (int m = 0; m < nl; m++) t.launch_ptr[m + 1] += t.launch_ptr[m];
   t.nseg = static_cast<int>(seg.size()) - t.nbatch;
   if (xslot.empty()) xslot.push_back(0);
+  const char *pp = std::getenv("CFDL_TILE_PIPE");
+  t.pipe = hex && addr16 && seg_max <= kLdsThreads && !(pp && pp[0] == '0');
+  if (t.pipe) lds_max = lds_pipe_max;
   if (trace_on()) {
     std::fprintf(stderr, "cfdl: tiled sweep: %d launches, %d batches, %d segments (largest "
                  "%d cells), largest batch LDS %zu bytes\n", nl, t.nbatch, t.nseg, seg_max,
@@ -1792,8 +1855,6 @@ static int build_tile_tables(cfdl_ctx *c, const std::vector<int> &rowp,
   (void)hipGetLastError();
   t.lds_bytes = (lds_ok && fits && lds_max <= lds_cap) ? lds_max : 0;
   t.hex = hex;
-  const char *pp = std::getenv("CFDL_TILE_PIPE");
-  t.pipe = hex && seg_max <= kLdsThreads && !(pp && pp[0] == '0');
   std::vector<int4> bdesc(2 * static_cast<size_t>(t.nbatch));
   for (int i = 0; i < t.nbatch; i++) {
     const int s0 = bseg[i], nq = bseg[i + 1] - s0 - 1, c0 = seg[s0], nc = seg[s0 + nq] - c0;
@@ -1961,13 +2022,25 @@ int cfdl_ctx_create(int ne, int nf, int nbf, const int *ef2nb_idx, const int *ef
   C->en = EnergyDev{F[CFDL_T], F[CFDL_GT], F[CFDL_ENTH], F[CFDL_ENTH0], F[CFDL_GENTH]};
   hipLaunchKernelGGL(k_energy_init, dim3(nblocks(n)), dim3(kBlock), 0, C->st, C->md, C->en);
   trace("ctx_create: fields");
-  // sweep schedules: spatial tiles of about edge^3 cells (edge 8 measured
-  // fastest on the 128^3 cavity among 5..12), batches spanning fewer than
-  // `span` levels; CFDL_GS_MODE=level keeps one launch per level
-  const char *te = std::getenv("CFDL_TILE");
-  const int edge = te ? std::max(1, std::atoi(te)) : 8;
+  // sweep schedules: spatial tiles of edge[0] x edge[1] x edge[2] cells
+  // (CFDL_TILE = "A" for A^3 or "AxBxC"), batches spanning fewer than `span`
+  // levels; CFDL_GS_MODE=level keeps one launch per level.  16 x 8 x 4 cells
+  // and 32 levels measured fastest on the 128^3 cavity (fewest launches
+  // whose batches keep level segments within 64 cells and LDS within half
+  // a CU's, so two batches fit per CU).
+  int edge[3] = {16, 8, 4};
+  if (const char *te = std::getenv("CFDL_TILE")) {
+    int a = 0, b = 0, d = 0;
+    const int n = std::sscanf(te, "%dx%dx%d", &a, &b, &d);
+    if (n == 3) {
+      edge[0] = a, edge[1] = b, edge[2] = d;
+    } else if (n >= 1) {
+      edge[0] = edge[1] = edge[2] = a;
+    }
+    for (int i = 0; i < 3; i++) edge[i] = std::max(1, edge[i]);
+  }
   const char *sp = std::getenv("CFDL_TILE_SPAN");
-  const int span = sp ? std::min(63, std::max(1, std::atoi(sp))) : 20;
+  const int span = sp ? std::min(63, std::max(1, std::atoi(sp))) : 32;
   const char *mode = std::getenv("CFDL_GS_MODE");
   C->tiled = !(mode && std::string(mode) == "level");
   const char *rc = std::getenv("CFDL_TILE_RCP");

@@ -164,15 +164,16 @@ def test_set_get_roundtrip_and_errors():
 @pytest.mark.parametrize("tile,mode,lds,span,pipe", [
     ("2", "tiled", "1", "16", "1"), ("3", "tiled", "1", "3", "1"), ("3", "tiled", "0", "16", "1"),
     ("8", "tiled", "1", "16", "1"), ("8", "tiled", "1", "16", "0"), ("8", "level", "1", "16", "1"),
-    ("12", "tiled", "1", "32", "1")])
+    ("12", "tiled", "1", "32", "1"), ("4x2x3", "tiled", "1", "5", "1"),
+    ("16x8x4", "tiled", "1", "28", "1")])
 def test_sweep_schedules_bitwise(monkeypatch, tile, mode, lds, span, pipe):
     """The pressure-correction sweeps run as one launch per dependency level
     or as space-time tiles launched batch by batch (in LDS, or from global
     memory with CFDL_TILE_LDS=0).  In LDS, level segments of at most 64 cells
     take the register-pipelined walk unless CFDL_TILE_PIPE=0.  Small tiles and
     spans put many tile boundaries into small meshes; tile 12 with span 32
-    needs more than 64 KB of LDS per batch and has wider segments.  Every
-    schedule gives the oracle's bits."""
+    needs more than 64 KB of LDS per batch and has wider segments; tiles may
+    have a different edge per axis.  Every schedule gives the oracle's bits."""
     monkeypatch.setenv("CFDL_TILE", tile)
     monkeypatch.setenv("CFDL_GS_MODE", mode)
     monkeypatch.setenv("CFDL_TILE_LDS", lds)

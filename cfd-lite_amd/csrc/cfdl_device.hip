@@ -142,18 +142,23 @@ struct SysDesc {
   const double *rap;  // 1/ap per cell (tiled LDS sweeps), correctly rounded
 };
 
-// x / a, correctly rounded, from y = 1/a correctly rounded: q0 = x*y is
-// corrected once by the exact remainder, q0 + (x - a*q0)*y (Markstein's
-// theorem), with the sign of q0 (a zero quotient keeps its sign).  This
-// takes the division off the Gauss-Seidel critical path, since y is known
-// before the sweep.  The result is exact while q0, the remainder and r*y
-// stay normal: |q0| in [2^-960, 2^960], or q0 = 0 (then r = 0 and the sign
-// comes from q0).  Callers keep the operands inside that range (rcp_of and
-// kRcpExp) and fall back to the hardware division outside it.
-__device__ inline double div_rcp(double x, double a, double y) {
+// (x / a) / s, correctly rounded at each step, from y = 1/a and ys = 1/s
+// correctly rounded.  A quotient estimate q0 = x*y corrected once by the
+// exact remainder, q0 + (x - a*q0)*y, is the correctly rounded x/a
+// (Markstein's theorem) while q0, the remainder and r*y stay normal: |q0|
+// in [2^-960, 2^960], or q0 = 0.  The same step then divides by s.  Only
+// the sign of a zero result can come out wrong, and for s > 0 the final
+// sign is that of x/a, i.e. of q0 = x*y: one copysign at the end fixes it
+// (no intermediate is a zero of the wrong sign unless x/a is zero).  This
+// takes the divisions off the Gauss-Seidel critical path, since y and ys
+// are known before the sweep.  Callers keep the operands in range (rcp_of,
+// kRcpExp, s in [2^-64, 2^64]) and fall back to the hardware division
+// outside it.
+__device__ inline double div_rcp2(double x, double a, double y, double s, double ys) {
   const double q0 = x * y;
-  const double r = fma(-a, q0, x);
-  return __builtin_copysign(fma(r, y, q0), q0);
+  const double q1 = fma(fma(-a, q0, x), y, q0);
+  const double p0 = q1 * ys;
+  return __builtin_copysign(fma(fma(-s, p0, q1), ys, p0), q0);
 }
 
 // Operand ranges of the reciprocal division: divisors (ap, sor) with
@@ -163,7 +168,7 @@ __device__ inline double div_rcp(double x, double a, double y) {
 constexpr int kRcpExp = 750;
 constexpr double kRcpDivMin = 0x1p-64, kRcpDivMax = 0x1p64;
 
-// 1/a for div_rcp, NaN for a divisor outside [2^-64, 2^64] (zero, huge, NaN
+// 1/a for div_rcp2, NaN for a divisor outside [2^-64, 2^64] (zero, huge, NaN
 // included): every quotient that uses it is then NaN, which the sweep
 // detects and recomputes with the hardware division
 __device__ inline double rcp_of(double a) {
@@ -870,7 +875,7 @@ k_gs_tiles_lds(SysDesc s, const int4 *__restrict__ bdesc, const int *__restrict_
     // one wavefront, whose LDS accesses execute in issue order, so those
     // reads see the stores without waiting for them.  Lanes without a cell
     // read row 0 and store to lds[0..3].
-    // RCP: the divisions by ap and sor use div_rcp.  Without a branch in the
+    // RCP: the divisions by ap and sor use div_rcp2.  Without a branch in the
     // walk, each lane keeps the least and greatest frexp exponent of its
     // dividends x and a NaN detector (0*nv is NaN for an infinite or NaN
     // result, which an out-of-range 1/ap or an overflow produces); if any
@@ -940,7 +945,7 @@ k_gs_tiles_lds(SysDesc s, const int4 *__restrict__ bdesc, const int *__restrict_
             const int ex = __builtin_amdgcn_frexp_exp(x);  // 0 for x = 0
             emin = min(emin, ex);
             emax = max(emax, ex);
-            nv = div_rcp(div_rcp(x, cur.a, cur.ya), sor, rsor);
+            nv = div_rcp2(x, cur.a, cur.ya, sor, rsor);
             nan_seen = fma(0.0, nv, nan_seen);
           } else {
             nv = x / cur.a / sor;
@@ -1454,10 +1459,10 @@ static void tiled_launches(cfdl_ctx *c, const cfdl_ctx::TileSched &t, const SysD
                            bool forward, double sor, const int *active) {
   const int nl = static_cast<int>(t.launch_ptr.size()) - 1;
   const int bwd = forward ? 0 : 1;
-  // 1/sor correctly rounded, as div_rcp requires; 0 selects the hardware
-  // division in the kernel (CFDL_TILE_RCP=0, or sor outside div_rcp's
+  // 1/sor correctly rounded, as div_rcp2 requires; 0 selects the hardware
+  // division in the kernel (CFDL_TILE_RCP=0, or sor outside div_rcp2's
   // divisor range)
-  const bool rcp_ok = c->rcp && std::fabs(sor) >= kRcpDivMin && std::fabs(sor) <= kRcpDivMax;
+  const bool rcp_ok = c->rcp && sor >= kRcpDivMin && sor <= kRcpDivMax;
   const double rsor = rcp_ok ? 1.0 / sor : 0.0;
   for (int i = 0; i < nl; i++) {
     const int m = forward ? i : nl - 1 - i;

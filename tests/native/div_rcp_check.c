@@ -1,13 +1,15 @@
 /* Checks, on the host, the division used by the pipelined Gauss-Seidel walk
- * (div_rcp in cfd-lite_amd/csrc/cfdl_device.hip): with y = 1/a correctly
- * rounded, q0 = x*y, r = fma(-a, q0, x) and q = copysign(fma(r, y, q0), q0)
- * is the correctly rounded x/a (Markstein's theorem) while the quotient
- * estimate stays normal.  The walk divides twice, x/ap and then /sor, with
- * divisors in [2^-64, 2^64] and dividends of frexp exponent in [-750, 750]
- * or zero; outside that it falls back to the hardware division.  Operands
- * are drawn over those whole ranges, with significands near binade edges,
- * sparse bit patterns and signed zeros; every single and chained result
- * must equal the plain divisions bit for bit.
+ * (div_rcp2 in cfd-lite_amd/csrc/cfdl_device.hip): with y = 1/a correctly
+ * rounded, q0 = x*y, r = fma(-a, q0, x) and q = fma(r, y, q0) is the
+ * correctly rounded x/a (Markstein's theorem) while the quotient estimate
+ * stays normal; the walk divides that by sor > 0 the same way and fixes the
+ * sign of a zero result once, from q0.  It divides only with divisors in
+ * [2^-64, 2^64] and dividends of frexp exponent in [-750, 750] or zero;
+ * outside that it falls back to the hardware division.  Operands are drawn
+ * over those whole ranges (a of either sign, sor > 0), with significands
+ * near binade edges, sparse bit patterns and signed zeros; the single
+ * division (with its own copysign) and the chained one must equal the
+ * plain divisions bit for bit.
  * Usage: div_rcp_check SEED SAMPLES; exit status 0 when all agree. */
 #include <math.h>
 #include <stdint.h>
@@ -33,12 +35,18 @@ static double div_rcp(double x, double a, double y) {
   const double r = fma(-a, q0, x);
   return copysign(fma(r, y, q0), q0);
 }
+static double div_rcp2(double x, double a, double y, double s, double ys) {
+  const double q0 = x * y;
+  const double q1 = fma(fma(-a, q0, x), y, q0);
+  const double p0 = q1 * ys;
+  return copysign(fma(fma(-s, p0, q1), ys, p0), q0);
+}
 static long bad;
 static void check(double x, double a, double s) {
   volatile double av = a, sv = s;
   const double ya = 1.0 / av, ys = 1.0 / sv;
   const double q = div_rcp(x, a, ya), ref = x / av;
-  const double q2 = div_rcp(q, s, ys), ref2 = ref / sv;
+  const double q2 = div_rcp2(x, a, ya, s, ys), ref2 = ref / sv;
   if (memcmp(&q, &ref, 8) != 0 || memcmp(&q2, &ref2, 8) != 0) {
     if (bad < 5) printf("mismatch x=%a a=%a s=%a: %a %a vs %a %a\n", x, a, s, q, q2, ref, ref2);
     bad++;

@@ -1,13 +1,14 @@
 """The correctly rounded division of the pipelined Gauss-Seidel walk.
 
 The LDS sweep kernel replaces x/ap/sor on its critical path by
-div_rcp(div_rcp(x, ap, 1/ap), sor, 1/sor): with y = 1/a correctly rounded,
-q0 = x*y and one fused remainder correction give the correctly rounded
-quotient (Markstein's theorem), so the fields stay bit-identical to the
-reference's divisions.  tests/native/div_rcp_check.c applies the same
-operation sequence on the host to random and adversarial operands and
-compares every result with x/a bit for bit; the GPU parity tests cover the
-device code itself.
+div_rcp2(x, ap, 1/ap, sor, 1/sor): with y = 1/a correctly rounded, q0 = x*y
+and one fused remainder correction give the correctly rounded quotient
+(Markstein's theorem), twice, with one sign fix for zero results, so the
+fields stay bit-identical to the reference's divisions.
+tests/native/div_rcp_check.c applies the same operation sequence on the host
+to random and adversarial operands (signed zeros included) and compares
+every result with the plain divisions bit for bit; the GPU parity tests
+cover the device code itself.
 """
 import os
 import subprocess

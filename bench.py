@@ -86,6 +86,40 @@ def run_to_convergence(eq, ne, tstep_every, max_outer, tol, check_every=10):
     return k, time.perf_counter() - t, worst
 
 
+# host <-> device traffic of one drop-in solve_uvwp call
+# (cfd-lite_amd/fortran/dropin/mod_uvwp.f90: put_state, get_state, get_work)
+PUT_STATE = ("u", "v", "w", "p", "u0", "v0", "w0", "gu", "gv", "gw", "gp", "gpc", "mip", "mip0",
+             "bu", "bv", "bw", "d", "dc")
+GET_STATE = ("u", "v", "w", "p", "gu", "gv", "gw", "gp", "gpc", "mip", "bu", "bv", "bw", "d", "dc")
+GET_WORK = ("ap", "anb", "b", "phic")
+
+
+def pcie_rate(eq, ne, args, steps=4):
+    """Outer iterations as the drop-in Fortran modules run them: the host
+    arrays (pageable, as the Fortran host's) go to the device before every
+    solve_uvwp and come back after it.  Printed as its own JSON line; the
+    headline value stays the device-resident rate."""
+    host = {k: eq.get(k) for k in set(PUT_STATE + GET_STATE + GET_WORK)}
+    nbytes = 8 * (sum(host[k].size for k in PUT_STATE) +
+                  sum(host[k].size for k in GET_STATE + GET_WORK))
+    eq.synchronize()
+    t = time.perf_counter()
+    for k in range(steps):
+        for name in PUT_STATE:
+            eq.set(name, host[name])
+        eq.update_boundaries()
+        eq.solve_uvwp()
+        for name in GET_STATE + GET_WORK:
+            host[name] = eq.get(name)
+        if (k + 1) % args.tstep_every == 0:
+            eq.update_time()
+    eq.synchronize()
+    secs = time.perf_counter() - t
+    print(json.dumps({"mode": "pcie", "metric": METRIC, "value": round(ne * steps / secs, 1),
+                      "unit": "cell-iterations/s", "ms_per_step": round(1e3 * secs / steps, 3),
+                      "host_device_bytes_per_step": nbytes, "steps": steps}), flush=True)
+
+
 def write_ref_mesh(mesh, path):
     """Mesh file read by the reference driver (oracle/refdrv/drv_cell.f90)."""
     info = mesh.info()
@@ -161,6 +195,9 @@ def main():
                     help="run from rest to a 1e-6 scaled residual (or --max-outer) instead")
     ap.add_argument("--max-outer", type=int, default=500)
     ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--pcie", action="store_true",
+                    help="also time outer iterations with the host transfers of the drop-in "
+                         "Fortran solve_uvwp (state up, state and work arrays down)")
     ap.add_argument("--traffic-kib", type=float, default=None,
                     help="measured HBM KiB per sweep launch (rocprofv3 FETCH_SIZE + WRITE_SIZE)")
     args = ap.parse_args()
@@ -199,6 +236,9 @@ def main():
         step_no[0] += 1
         if step_no[0] % args.tstep_every == 0:
             eq.update_time()
+
+    if args.pcie:
+        pcie_rate(eq, ne, args)
 
     for _ in range(args.warmup):
         step()

@@ -693,7 +693,7 @@ k_gs_tiles(SysDesc s, const int *__restrict__ bseg, const int *__restrict__ seg,
 // the batch back.  Outside neighbours are constant during the launch: a
 // cell of another tile is never updated in the same launch.  lnbr gives,
 // per CSR entry, the LDS slot of the neighbour: < nc a batch cell, else
-// outside value nc + j.  Every load of a gather round is issued before the
+// outside value nc + j.  Every load of a gather pass is issued before the
 // first LDS store, so the gather costs two memory latencies (the outside
 // indices, then their values).  The segment count is at most 63 (host).
 constexpr int kLdsThreads = 64;
@@ -764,9 +764,9 @@ k_gs_tiles_lds(SysDesc s, const int4 *__restrict__ bdesc, const int *__restrict_
   // segment bound of lane q (lanes past nq reload the last one); used after
   // the gather, so its load overlaps the gather's
   const int sg_raw = seg[s0 + min(static_cast<int>(threadIdx.x), nq)];
-  // gather: each round covers U*64 cells (values, coefficients, slots) and
-  // 512 outside neighbours; every load of a round is issued before its LDS
-  // stores, so a round costs two memory latencies (the outside indices, then
+  // gather: each pass covers U*64 cells (values, coefficients, slots) and
+  // 512 outside neighbours; every load of a pass is issued before its LDS
+  // stores, so a pass costs two memory latencies (the outside indices, then
   // their values)
   constexpr int U = NC == 1 ? 8 : 4;
   const int nrc = (nc + U * kLdsThreads - 1) / (U * kLdsThreads);
@@ -786,7 +786,7 @@ k_gs_tiles_lds(SysDesc s, const int4 *__restrict__ bdesc, const int *__restrict_
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int i = cb + u * kLdsThreads + threadIdx.x;
-      if (cb + u * kLdsThreads < nc) {  // wave-uniform: rows of this round exist
+      if (cb + u * kLdsThreads < nc) {  // wave-uniform: rows of this pass exist
         const int ic = i < nc ? i : 0;  // lanes past the end reload row 0
         va[u] = pa[ic];
         vr[u] = pr[ic];
@@ -869,9 +869,10 @@ k_gs_tiles_lds(SysDesc s, const int4 *__restrict__ bdesc, const int *__restrict_
   if constexpr (PIPE) {
     // Every segment has at most 64 cells: lane l updates cell sg[q] + l of
     // segment q (bounds from lane q of sg).  Two operand sets alternate: while
-    // segment j computes from one, the other receives segment j+1's record
-    // and own old value (its cells are not written at level j); right after
-    // segment j's stores it receives its neighbour values.  The workgroup is
+    // segment j computes from one, the other receives segment j+1's ap, 1/ap,
+    // b, anb, neighbour addresses and own old value (its cells are not
+    // written at level j); right after segment j's stores it receives its
+    // neighbour values.  The workgroup is
     // one wavefront, whose LDS accesses execute in issue order, so those
     // reads see the stores without waiting for them.  Lanes without a cell
     // read row 0 and store to lds[0..3].
@@ -890,7 +891,7 @@ k_gs_tiles_lds(SysDesc s, const int4 *__restrict__ bdesc, const int *__restrict_
       double a, ya, bv[NC], t[NC], an[6], v[NC][6];
       unsigned sl[6];
     };
-    // record and own values of segment qq (no cell when !valid)
+    // operands and own values of segment qq (no cell when !valid)
     auto load_ops = [&](int qq, bool valid, Ops &o) {
       int ii = 0;
       bool oo = false;

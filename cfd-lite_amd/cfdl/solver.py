@@ -83,6 +83,16 @@ class UVWP:
                                   C.byref(h)))
         return cls(h, cls._sizes(ne, nf, nbf))
 
+    PC_ORDERS = {"reference": 0, "colored": 1}
+
+    def set_pc_order(self, order="reference"):
+        """Visiting order of the multi-subdomain pressure-correction smoother:
+        "reference" (the reference's order, the default) or "colored" (greedy
+        colouring of each subdomain, swept colour by colour; an extension)."""
+        if order not in self.PC_ORDERS:
+            raise CfdlError("pc order must be one of %s" % sorted(self.PC_ORDERS))
+        check(self._lib.cfdl_set_pc_order(self._h, self.PC_ORDERS[order]))
+
     def set_relaxation(self, alpha_u=1.0, alpha_p=1.0):
         """Config-mode under-relaxation (1, 1 = the reference algorithm)."""
         check(self._lib.cfdl_set_relaxation(self._h, alpha_u, alpha_p))

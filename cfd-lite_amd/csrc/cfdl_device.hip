@@ -1279,6 +1279,10 @@ struct cfdl_ctx {
   double *ap_p = nullptr, *b_p = nullptr, *anb_p = nullptr, *phic_p = nullptr, *rap_p = nullptr;
   std::vector<int> co;  // subdomain cell offsets (nsubd+1)
   Sched pcl;            // pc system by forward level (CFDL_GS_MODE=level)
+  // colour order of the pc smoother (cfdl_set_pc_order, an extension): the
+  // colours of a greedy colouring in each subdomain's local order, as levels
+  Sched pcc;
+  int pc_order = CFDL_PC_REFERENCE;
   int pc_nlev = 0;
   // space-time tiled schedule of the pc sweeps (the backward sweep runs it
   // in reverse)
@@ -1435,8 +1439,8 @@ static int sample_end(cfdl_ctx *c, cfdl_ctx::Sample *smp, int launches, int64_t 
 // visits the levels in order, backward in reverse (a cell's neighbours on
 // lower forward levels come earlier in the sequential order, so reversing
 // the level order is a valid backward schedule).
-static int pc_sweep(cfdl_ctx *c, const SysDesc &sys, bool forward, const int *active) {
-  const Sched &s = c->pcl;
+static int pc_sweep(cfdl_ctx *c, const Sched &s, const SysDesc &sys, bool forward,
+                    const int *active) {
   cfdl_ctx::Sample *smp = nullptr;
   CALL(sample_begin(c, &smp));
   for (int i = 0; i < s.nlev; i++) {
@@ -1639,12 +1643,15 @@ static int multi_subdomain_solver(cfdl_ctx *c) {
   const int nblock = (c->nit + 1) / 2;
   for (int blk = 1; blk <= nblock; blk++) {
     for (int rep = 0; rep < 2; rep++) {  // smoother_gs(..., 2): two symmetric iterations
-      if (c->tiled) {
+      if (c->pc_order == CFDL_PC_COLORED) {
+        CALL(pc_sweep(c, c->pcc, s1, true, act));
+        CALL(pc_sweep(c, c->pcc, s1, false, act));
+      } else if (c->tiled) {
         CALL(pc_sweep_tiled(c, s1, true, act));
         CALL(pc_sweep_tiled(c, s1, false, act));
       } else {
-        CALL(pc_sweep(c, s1, true, act));
-        CALL(pc_sweep(c, s1, false, act));
+        CALL(pc_sweep(c, c->pcl, s1, true, act));
+        CALL(pc_sweep(c, c->pcl, s1, false, act));
       }
     }
     if (c->nx > 0)
@@ -2187,8 +2194,25 @@ int cfdl_ctx_create(int ne, int nf, int nbf, const int *ef2nb_idx, const int *ef
       }
     }
     for (auto &v : xsrc) v = o2n[v];
+    // colour order (extension): greedy colouring of each subdomain in its
+    // local order, a cell taking the least colour that none of its earlier
+    // interior neighbours has; colour + 1 serves as the cell's level
+    std::vector<int> colq(ne, 0), coln(ne);
+    for (int s = 0; s < nsubd; s++)
+      for (int q = C->co[s]; q < C->co[s + 1]; q++) {
+        uint64_t used = 0;
+        for (int k = rowo[q]; k < rowo[q + 1]; k++) {
+          const int d = nbro[k];
+          if (d < ne && d < q && colq[d] < 64) used |= uint64_t{1} << colq[d];
+        }
+        int col = 0;
+        while (col < 64 && ((used >> col) & 1)) col++;
+        colq[q] = col;
+      }
+    for (int q = 0; q < ne; q++) coln[o2n[q]] = colq[q] + 1;
     CALL(build_tile_tables(C, rowp, nbrp, sdn, launchn, tidn, levn, 1, &C->ts));
     CALL(make_sched(C, levn, ne, &C->pcl));
+    CALL(make_sched(C, coln, ne, &C->pcc));
     C->Lp = static_cast<int>(nbrp.size());
     C->nh = nh;
     C->nx = static_cast<int>(xdst.size());
@@ -2314,6 +2338,16 @@ int cfdl_set_relaxation(cfdl_ctx *c, double alpha_u, double alpha_p) {
   }
   c->alpha_u = alpha_u;
   c->alpha_p = alpha_p;
+  return CFDL_OK;
+}
+
+int cfdl_set_pc_order(cfdl_ctx *c, int order) {
+  CALL(check_ctx(c));
+  if (order != CFDL_PC_REFERENCE && order != CFDL_PC_COLORED) {
+    set_error("cfdl_set_pc_order: order must be CFDL_PC_REFERENCE or CFDL_PC_COLORED");
+    return CFDL_EINVAL;
+  }
+  c->pc_order = order;
   return CFDL_OK;
 }
 

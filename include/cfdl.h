@@ -148,6 +148,19 @@ void cfdl_ctx_destroy(cfdl_ctx *c);
    default) is the reference algorithm bit for bit. */
 int cfdl_set_relaxation(cfdl_ctx *c, double alpha_u, double alpha_p);
 
+/* Visiting order of the multi-subdomain pressure-correction smoother
+   (smoother_gs inside multi_subdomain_solver, mod_solver.f90:124-228).
+   CFDL_PC_REFERENCE (the default) is the reference's order, bit for bit.
+   CFDL_PC_COLORED has no counterpart in the reference: every subdomain is
+   coloured greedily in its local order (a cell takes the least colour none
+   of its earlier interior neighbours has) and the sweeps visit the cells
+   colour by colour (forward) and in reverse (backward) -- the same
+   Gauss-Seidel update in another sequential order, with every stop rule
+   unchanged, in a few fully parallel launches per sweep.  No effect with one
+   subdomain (the pc solve is then solve_gs on the global system). */
+enum { CFDL_PC_REFERENCE = 0, CFDL_PC_COLORED = 1 };
+int cfdl_set_pc_order(cfdl_ctx *c, int order);
+
 /* Host-side state a Fortran drop-in keeps authoritative (properties_t and
    phys_t%dt, mod_properties.f90:9-51, mod_physics.f90:15; the BC type the
    setters assign, mod_uvwp.f90:500,525,555: 'dirichlet' -> CFDL_BC_WALL,

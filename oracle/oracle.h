@@ -86,6 +86,11 @@ typedef struct or_case {
      properties cp, tc (properties_t, mod_properties.f90:76-90) */
   double *cp, *tc;
   double *t, *gt, *h, *h0, *gh;
+  /* pressure-correction visiting order of the subdomain smoother: 0 is the
+     reference's (local cell order); 1 visits the cells colour by colour
+     (or_set_pc_order), an extension the reference does not have */
+  int pc_order;
+  int **pc_perm; /* per subdomain: its ne cells (1-based) in visiting order */
 } or_case;
 
 /* ---- setup ---- */
@@ -118,6 +123,12 @@ void or_calc_residual(const double *phi, const double *ap, const double *anb,
                       const double *b, const int *ef2nb_idx, const int *ef2nb,
                       int ne, int nf, int nbf, double *res, double *res_max);
 void or_solve(or_case *c, const char *name, double *phi);
+/* colour order of the subdomain pressure-correction smoother (extension):
+   order 1 colours every subdomain greedily in its local order (a cell takes
+   the least colour none of its earlier interior neighbours has) and visits
+   the cells by colour, then by local index; order 0 restores the
+   reference's order. */
+void or_set_pc_order(or_case *c, int order);
 
 /* ---- hot path (equation routines mirror mod_uvwp.f90) ---- */
 void or_update_boundaries(or_case *c);

@@ -218,6 +218,60 @@ static void update_halos(or_intf *t, or_subd *s1, or_subd *s2) {
   }
 }
 
+/* smoother_gs (src/modules/mod_solver.f90:191-228) visiting the cells in the
+   order perm[0..ne-1] (1-based cells) forward and in reverse backward: the
+   same update, a different sequential order (colour order, extension) */
+static void smoother_gs_perm(const char *name, double *phi, const double *ap, const double *anb,
+                             const double *b, const int *ef2nb_idx, const int *ef2nb, int ne,
+                             const int *perm, int nit) {
+  double sor = sor_of(name);
+  for (int it = 1; it <= nit; it++) {
+    for (int k = 0; k < ne; k++) {
+      int e = perm[k];
+      double s = gs_sum(phi, anb, b, ef2nb_idx, ef2nb, e);
+      phi[e - 1] = (s + (sor - 1.0) * ap[e - 1] * phi[e - 1]) / ap[e - 1] / sor;
+    }
+    for (int k = ne - 1; k >= 0; k--) {
+      int e = perm[k];
+      double s = gs_sum(phi, anb, b, ef2nb_idx, ef2nb, e);
+      phi[e - 1] = (s + (sor - 1.0) * ap[e - 1] * phi[e - 1]) / ap[e - 1] / sor;
+    }
+  }
+}
+
+void or_set_pc_order(or_case *c, int order) {
+  if (c->pc_perm) {
+    for (int k = 0; k < c->nsubd; k++) free(c->pc_perm[k]);
+    free(c->pc_perm);
+    c->pc_perm = NULL;
+  }
+  c->pc_order = order;
+  if (order != 1) return;
+  c->pc_perm = calloc((size_t)c->nsubd, sizeof(int *));
+  for (int cc = 0; cc < c->nsubd; cc++) {
+    const or_subd *s = &c->sd[cc];
+    int *col = calloc((size_t)s->ne, sizeof(int));
+    int ncol = 0;
+    for (int e = 1; e <= s->ne; e++) {
+      unsigned long long used = 0; /* a hex cell has at most 6 neighbours */
+      for (int idx = s->ef2nb_idx[e - 1]; idx <= s->ef2nb_idx[e] - 1; idx++) {
+        int d = PK_E(s->ef2nb[idx - 1]);
+        if (d <= s->ne && d < e && col[d - 1] < 64) used |= 1ull << col[d - 1];
+      }
+      int k = 0;
+      while (k < 64 && (used >> k & 1ull)) k++;
+      col[e - 1] = k;
+      if (k + 1 > ncol) ncol = k + 1;
+    }
+    int *perm = malloc((size_t)s->ne * sizeof(int)), n = 0;
+    for (int k = 0; k < ncol; k++)
+      for (int e = 1; e <= s->ne; e++)
+        if (col[e - 1] == k) perm[n++] = e;
+    free(col);
+    c->pc_perm[cc] = perm;
+  }
+}
+
 /* multi_subdomain_solver, src/modules/mod_solver.f90:124-189 (quirks Q1-Q3) */
 static void multi_subdomain_solver(or_case *c, const char *name, double *phi) {
   int ns = c->nsubd, nit = c->nit;
@@ -237,8 +291,12 @@ static void multi_subdomain_solver(or_case *c, const char *name, double *phi) {
   while (it < nit && rf > target) {
     for (int cc = 0; cc < ns; cc++) {
       or_subd *s = &c->sd[cc];
-      or_smoother_gs(name, s->phic, s->ap, s->anb, s->b, s->ef2nb_idx, s->ef2nb, s->ne, s->nf,
-                     s->nbf, 2);
+      if (c->pc_order == 1)
+        smoother_gs_perm(name, s->phic, s->ap, s->anb, s->b, s->ef2nb_idx, s->ef2nb, s->ne,
+                         c->pc_perm[cc], 2);
+      else
+        or_smoother_gs(name, s->phic, s->ap, s->anb, s->b, s->ef2nb_idx, s->ef2nb, s->ne,
+                       s->nf, s->nbf, 2);
     }
     it += 2;
     for (int i = 0; i < ns; i++)

@@ -48,7 +48,8 @@ class OrCase(C.Structure):
                 ("bu", PD), ("bv", PD), ("bw", PD), ("d", PD), ("dc", PD),
                 ("ap", PD), ("anb", PD), ("b", PD), ("phic", PD),
                 ("nlog", C.c_int), ("caplog", C.c_int), ("log", C.POINTER(OrLog)),
-                ("cp", PD), ("tc", PD), ("t", PD), ("gt", PD), ("h", PD), ("h0", PD), ("gh", PD)]
+                ("cp", PD), ("tc", PD), ("t", PD), ("gt", PD), ("h", PD), ("h0", PD), ("gh", PD),
+                ("pc_order", C.c_int), ("pc_perm", C.POINTER(PI))]
 
 
 _lib = None
@@ -155,6 +156,17 @@ class OracleCase:
         c = self.c
         return [(c.log[k].name.decode(), c.log[k].it, c.log[k].res_i, c.log[k].res_f,
                  c.log[k].res_max) for k in range(c.nlog)]
+
+    def set_pc_order(self, order="reference"):
+        """Colour order of the subdomain pc smoother (or_set_pc_order)."""
+        L = lib()
+        L.or_set_pc_order.argtypes = [C.c_void_p, C.c_int]
+        L.or_set_pc_order(C.cast(self.p, C.c_void_p), {"reference": 0, "colored": 1}[order])
+
+    def pc_perm(self, k):
+        """Visiting order (1-based local cells) of subdomain k's pc smoother
+        under the colour order."""
+        return np.ctypeslib.as_array(self.c.pc_perm[k], shape=(self.c.sd[k].ne,)).copy()
 
     def set_relaxation(self, alpha_u=1.0, alpha_p=1.0):
         self.c.alpha_u = alpha_u

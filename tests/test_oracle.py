@@ -428,3 +428,51 @@ def test_energy_closed_forms():
         assert np.allclose(anb[row][halo], 5.0 * h * h / (h / 2) / 1000.0, rtol=1e-12)
         assert ap[e] == pytest.approx(anb[row].sum() + ap0, rel=1e-12)
         assert b[e] == pytest.approx(ap0 * 273.0 * 1000.0, rel=1e-14)
+
+
+def _greedy_colours(sd):
+    """Greedy colouring of a subdomain system in its local order, restated:
+    a cell takes the least colour none of its earlier interior neighbours
+    has (halo entries, packed index > ne, are not cells of the system)."""
+    ne, idx, ef = sd["ne"], sd["ef2nb_idx"], sd["ef2nb"]
+    col = np.zeros(ne + 1, dtype=np.int64)
+    for e in range(1, ne + 1):
+        used = {int(col[d]) for d in (int(ef[i - 1]) >> 5 for i in range(idx[e - 1], idx[e]))
+                if d <= ne and d < e}
+        c = 0
+        while c in used:
+            c += 1
+        col[e] = c
+    return col
+
+
+def test_pc_colour_order_is_a_proper_greedy_colouring():
+    """The colour order of the subdomain pressure-correction smoother (an
+    extension, no reference counterpart) visits every cell once, in the
+    order (greedy colour, local index), and no two neighbouring cells share
+    a colour, so each colour's updates are independent and the GPU may run
+    them in parallel.  With it the solve takes another Gauss-Seidel path to
+    the same linear systems: after six outer iterations the fields stay
+    within 1e-3 of the reference order's and the solver lines agree in
+    their iteration counts."""
+    g = meshgen.box_mesh(10, jitter=0.15)
+    ref = oracle_lib.OracleCase(g, meshgen.cavity_bc(g), nsubd=4, rho=1.0, mu=0.0025, dt=0.01)
+    col = oracle_lib.OracleCase(g, meshgen.cavity_bc(g), nsubd=4, rho=1.0, mu=0.0025, dt=0.01)
+    col.set_pc_order("colored")
+    for k in range(4):
+        sd = col.subdomain(k)
+        ne, idx, ef = sd["ne"], sd["ef2nb_idx"], sd["ef2nb"]
+        colours = _greedy_colours(sd)
+        assert list(col.pc_perm(k)) == sorted(range(1, ne + 1), key=lambda e: (colours[e], e))
+        for e in range(1, ne + 1):
+            for i in range(idx[e - 1], idx[e]):
+                d = int(ef[i - 1]) >> 5
+                if d <= ne:
+                    assert colours[d] != colours[e]
+        assert 2 <= colours.max() + 1 <= 7
+    ref.run(2, 3)
+    col.run(2, 3)
+    assert [(l[0], l[1]) for l in ref.log()] == [(l[0], l[1]) for l in col.log()]
+    for name in ("u", "v", "w"):
+        assert np.max(np.abs(ref.arr(name) - col.arr(name))) < 1e-3
+    assert not np.array_equal(ref.arr("p"), col.arr("p"))

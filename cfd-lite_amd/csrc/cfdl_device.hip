@@ -1275,6 +1275,9 @@ struct cfdl_ctx {
   // pressure-correction subdomain systems (concatenated)
   int Lp = 0, nh = 0, nx = 0;
   int *rowp = nullptr, *nbrp = nullptr, *l2g_cell = nullptr, *halo_src = nullptr;
+  // the pc system numbered by (subdomain, colour, local order) for the
+  // colour order: each colour of a subdomain is a contiguous cell range
+  int *rowp_c = nullptr, *nbrp_c = nullptr, *l2g_c = nullptr, *xsrc_c = nullptr;
   int *xdst = nullptr, *xsrc = nullptr, *rowg = nullptr;
   double *ap_p = nullptr, *b_p = nullptr, *anb_p = nullptr, *phic_p = nullptr, *rap_p = nullptr;
   std::vector<int> co;  // subdomain cell offsets (nsubd+1)
@@ -1612,9 +1615,13 @@ static int solve_gs_n(cfdl_ctx *c, int ny, double **phi, const double **b,
 
 static int multi_subdomain_solver(cfdl_ctx *c) {
   const int ne = c->ne, ns = c->nsubd;
+  // the colour order runs on the colour numbering of the same system
+  const bool col = c->pc_order == CFDL_PC_COLORED;
+  const int *l2g = col ? c->l2g_c : c->l2g_cell, *rowp = col ? c->rowp_c : c->rowp;
+  const int *nbrp = col ? c->nbrp_c : c->nbrp, *xsrc = col ? c->xsrc_c : c->xsrc;
   // assemble_coef, mod_subdomains.f90:162-189
-  hipLaunchKernelGGL(k_pc_assemble, dim3(nblocks(ne)), dim3(kBlock), 0, c->st, ne, c->l2g_cell,
-                     c->rowp, c->md.row, c->fd.ap, c->fd.b, c->fd.anb, c->fd.phic, c->ap_p,
+  hipLaunchKernelGGL(k_pc_assemble, dim3(nblocks(ne)), dim3(kBlock), 0, c->st, ne, l2g,
+                     rowp, c->md.row, c->fd.ap, c->fd.b, c->fd.anb, c->fd.phic, c->ap_p,
                      c->b_p, c->anb_p, c->phic_p, c->rap_p);
   if (c->nh > 0)
     hipLaunchKernelGGL(k_gather, dim3(nblocks(c->nh)), dim3(kBlock), 0, c->st, c->nh, c->halo_src,
@@ -1630,8 +1637,8 @@ static int multi_subdomain_solver(cfdl_ctx *c) {
     s.b[y] = c->b_p;
     sc.ne[y] = static_cast<double>(c->co[y + 1] - c->co[y]);
   }
-  s.row = c->rowp;
-  s.nbr = c->nbrp;
+  s.row = rowp;
+  s.nbr = nbrp;
   s.ap = c->ap_p;
   s.anb = c->anb_p;
   s.rap = c->rap_p;
@@ -1656,14 +1663,14 @@ static int multi_subdomain_solver(cfdl_ctx *c) {
     }
     if (c->nx > 0)
       hipLaunchKernelGGL(k_exchange, dim3(nblocks(c->nx)), dim3(kBlock), 0, c->st, c->nx, c->xdst,
-                         c->xsrc, c->phic_p, act);
+                         xsrc, c->phic_p, act);
     const int has_res = ((2 * blk) % 10 == 0) ? 1 : 0;
     if (has_res) CALL(residual(c, s, c->nblk_p, c->chunk_p, 0));
     hipLaunchKernelGGL(k_pc_block_end, dim3(1), dim3(64), 0, c->st, c->pctl, c->red, sc, c->nit,
                        has_res);
   }
   // update_phi, mod_subdomains.f90:214-229
-  hipLaunchKernelGGL(k_scatter, dim3(nblocks(ne)), dim3(kBlock), 0, c->st, ne, c->l2g_cell,
+  hipLaunchKernelGGL(k_scatter, dim3(nblocks(ne)), dim3(kBlock), 0, c->st, ne, l2g,
                      c->phic_p, c->fd.phic);
   hipLaunchKernelGGL(k_log_pc, dim3(1), dim3(64), 0, c->st, c->pctl, c->dlog, c->dlog_n);
   HIPCHK(hipGetLastError());
@@ -2193,6 +2200,7 @@ int cfdl_ctx_create(int ne, int nf, int nbf, const int *ef2nb_idx, const int *ef
         nbrp.push_back(v < ne ? o2n[v] : v);
       }
     }
+    const std::vector<int> xsrc_old = xsrc;
     for (auto &v : xsrc) v = o2n[v];
     // colour order (extension): greedy colouring of each subdomain in its
     // local order, a cell taking the least colour that none of its earlier
@@ -2209,7 +2217,25 @@ int cfdl_ctx_create(int ne, int nf, int nbf, const int *ef2nb_idx, const int *ef
         while (col < 64 && ((used >> col) & 1)) col++;
         colq[q] = col;
       }
-    for (int q = 0; q < ne; q++) coln[o2n[q]] = colq[q] + 1;
+    // the colour numbering: (subdomain, colour, local order)
+    std::vector<int> c2o(ne), o2c(ne);
+    for (int q = 0; q < ne; q++) c2o[q] = q;
+    std::stable_sort(c2o.begin(), c2o.end(), [&](int x, int y) {
+      if (sd_of[l2g[x]] != sd_of[l2g[y]]) return sd_of[l2g[x]] < sd_of[l2g[y]];
+      return colq[x] < colq[y];
+    });
+    for (int i = 0; i < ne; i++) o2c[c2o[i]] = i;
+    std::vector<int> rowc(ne + 1, 0), nbrc, l2gc(ne), xsrcc(xsrc_old.size());
+    nbrc.reserve(nbro.size());
+    for (int i = 0; i < ne; i++) {
+      const int q = c2o[i];
+      l2gc[i] = l2g[q];
+      coln[i] = colq[q] + 1;
+      rowc[i + 1] = rowc[i] + (rowo[q + 1] - rowo[q]);
+      for (int k = rowo[q]; k < rowo[q + 1]; k++)
+        nbrc.push_back(nbro[k] < ne ? o2c[nbro[k]] : nbro[k]);
+    }
+    for (size_t j = 0; j < xsrc_old.size(); j++) xsrcc[j] = o2c[xsrc_old[j]];
     CALL(build_tile_tables(C, rowp, nbrp, sdn, launchn, tidn, levn, 1, &C->ts));
     CALL(make_sched(C, levn, ne, &C->pcl));
     CALL(make_sched(C, coln, ne, &C->pcc));
@@ -2222,6 +2248,10 @@ int cfdl_ctx_create(int ne, int nf, int nbf, const int *ef2nb_idx, const int *ef
     CALL(upload(C, &C->halo_src, halo_src));
     CALL(upload(C, &C->xdst, xdst));
     CALL(upload(C, &C->xsrc, xsrc));
+    CALL(upload(C, &C->rowp_c, rowc));
+    CALL(upload(C, &C->nbrp_c, nbrc));
+    CALL(upload(C, &C->l2g_c, l2gc));
+    CALL(upload(C, &C->xsrc_c, xsrcc));
     CALL(dalloc(C, &C->ap_p, ne));
     CALL(dalloc(C, &C->rap_p, ne));
     CALL(dalloc(C, &C->b_p, ne));

@@ -644,6 +644,30 @@ __global__ void k_gs_level(SysDesc s, const int *__restrict__ cells, int n, doub
   const int e = cells[t];
   const int r0 = s.row[e], r1 = s.row[e + 1];
   const double a = s.ap[e];
+  if (r1 - r0 == 6) {
+    // hexahedral row: every index, coefficient and value is loaded before
+    // the row sum, which then runs in CSR order as below
+    int nb[6];
+    double an[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      nb[k] = s.nbr[r0 + k];
+      an[k] = s.anb[r0 + k];
+    }
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+      if (!active[c]) continue;
+      double *phi = s.phi[c];
+      double v[6];
+#pragma unroll
+      for (int k = 0; k < 6; k++) v[k] = phi[nb[k]];
+      double sum = s.b[c][e];
+#pragma unroll
+      for (int k = 0; k < 6; k++) sum = sum + an[k] * v[k];
+      phi[e] = (sum + (sor - 1.0) * a * phi[e]) / a / sor;
+    }
+    return;
+  }
 #pragma unroll
   for (int c = 0; c < NC; c++) {
     if (!active[c]) continue;

@@ -1054,7 +1054,22 @@ __global__ void k_resid_partial(SysDesc s, int chunk, int nblk, int absmax, doub
   double sum = 0.0, mx = 0.0;
   for (int e = lo + tid; e < hi; e += blockDim.x) {
     double r = b[e];
-    for (int k = s.row[e]; k < s.row[e + 1]; k++) r = r + s.anb[k] * phi[s.nbr[k]];
+    const int r0 = s.row[e], r1 = s.row[e + 1];
+    if (r1 - r0 == 6) {  // hexahedral row: all loads first, the sum in CSR order
+      int nb[6];
+      double an[6], v[6];
+#pragma unroll
+      for (int k = 0; k < 6; k++) {
+        nb[k] = s.nbr[r0 + k];
+        an[k] = s.anb[r0 + k];
+      }
+#pragma unroll
+      for (int k = 0; k < 6; k++) v[k] = phi[nb[k]];
+#pragma unroll
+      for (int k = 0; k < 6; k++) r = r + an[k] * v[k];
+    } else {
+      for (int k = r0; k < r1; k++) r = r + s.anb[k] * phi[s.nbr[k]];
+    }
     r = r - s.ap[e] * phi[e];
     if (absmax) r = fabs(r);
     mx = fmax(r, mx);

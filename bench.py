@@ -16,9 +16,11 @@ kernel), 108 algorithmic bytes per cell update (FP64 CSR sweep: b, ap, 6
 anb, 6 neighbour indices, own phi read and write) times the cells of one
 launch, over the mean launch duration: HIP events on the context stream
 around every 7th sweep of the timed region (all of its launches, back to
-back), elapsed time divided by launches.  --traffic-kib adds the HBM bytes
-per launch measured with rocprofv3 (FETCH_SIZE + WRITE_SIZE, calibrated on
-the copy kernel of update_time).
+back), elapsed time divided by launches.  traffic: the HBM bytes per launch
+that rocprofv3 counted for this kernel on the 128^3 workload with the
+default schedule (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors a 256 MiB
+device copy gives), reported for that workload only; --traffic-kib gives
+another measured figure.
 cpu_baseline: the reference Fortran (flang -O2 build in oracle/_ref) on one
 host core, 64^3 sample (see --cpu-*), or the oracle's C restatement when the
 reference build is absent.
@@ -85,6 +87,12 @@ def run_to_convergence(eq, ne, tstep_every, max_outer, tol, check_every=10):
     eq.synchronize()
     return k, time.perf_counter() - t, worst
 
+
+# HBM KiB per launch of the pc sweep kernel on the 128^3 cavity, default
+# schedule, from rocprofv3 counter runs (profiles/k_gs_tiles_lds_traffic_
+# cavity128_fp64.md): mean FETCH_SIZE 3630.8 KiB x 1.99974 + mean WRITE_SIZE
+# 445.7 KiB x 1.0 over the 7400 launches of one outer iteration
+TRAFFIC_KIB_128 = 7706.4
 
 # host <-> device traffic of one drop-in solve_uvwp call
 # (cfd-lite_amd/fortran/dropin/mod_uvwp.f90: put_state, get_state, get_work)
@@ -266,7 +274,12 @@ def main():
     kernel_ms = st["launch_ms"]
     cells = st["launch_cells"]
     achieved = BYTES_PER_CELL_SWEEP * cells / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else 0.0
-    traffic = None if args.traffic_kib is None else round(args.traffic_kib * 1024.0)
+    traffic_kib = args.traffic_kib
+    schedule_env = any(os.environ.get(k) for k in ("CFDL_TILE", "CFDL_TILE_SPAN", "CFDL_GS_MODE",
+                                                  "CFDL_TILE_LDS", "CFDL_TILE_PIPE"))
+    if traffic_kib is None and args.n == 128 and not schedule_env:
+        traffic_kib = TRAFFIC_KIB_128
+    traffic = None if traffic_kib is None else round(traffic_kib * 1024.0)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "algorithmic_bytes_per_launch": round(BYTES_PER_CELL_SWEEP * cells),

@@ -96,8 +96,7 @@ TRAFFIC_KIB_128 = 7706.4
 
 # host <-> device traffic of one drop-in solve_uvwp call
 # (cfd-lite_amd/fortran/dropin/mod_uvwp.f90: put_state, get_state, get_work)
-PUT_STATE = ("u", "v", "w", "p", "u0", "v0", "w0", "gu", "gv", "gw", "gp", "gpc", "mip", "mip0",
-             "bu", "bv", "bw", "d", "dc")
+PUT_STATE = ("u", "v", "w", "p", "u0", "v0", "w0", "gu", "gv", "gw", "gp", "mip", "mip0")
 GET_STATE = ("u", "v", "w", "p", "gu", "gv", "gw", "gp", "gpc", "mip", "bu", "bv", "bw", "d", "dc")
 GET_WORK = ("ap", "anb", "b", "phic")
 

@@ -103,14 +103,26 @@ contains
     call cfdl_check(cfdl_get_field(eqn%ctx, field, a, int(size(a), c_int64_t)), 'download')
   end subroutine
 
-  subroutine put_state(eqn)
+  ! the state solve_uvwp reads.  bu, bv, bw, d, dc and gpc are not sent:
+  ! solve_uvwp assigns them before any read (mod_uvwp.f90:236-238, 277-283,
+  ! 131 before 385), so host changes to them (the VTU writer overwrites dc)
+  ! cannot reach its results.
+  subroutine put_inputs(eqn)
     class(uvwp_t) :: eqn
     call dev_put(eqn, CFDL_U, eqn%u); call dev_put(eqn, CFDL_V, eqn%v); call dev_put(eqn, CFDL_W, eqn%w)
     call dev_put(eqn, CFDL_P, eqn%p)
     call dev_put(eqn, CFDL_U0, eqn%u0); call dev_put(eqn, CFDL_V0, eqn%v0); call dev_put(eqn, CFDL_W0, eqn%w0)
     call dev_put(eqn, CFDL_GU, eqn%gu); call dev_put(eqn, CFDL_GV, eqn%gv); call dev_put(eqn, CFDL_GW, eqn%gw)
-    call dev_put(eqn, CFDL_GP, eqn%gp); call dev_put(eqn, CFDL_GPC, eqn%gpc)
+    call dev_put(eqn, CFDL_GP, eqn%gp)
     call dev_put(eqn, CFDL_MIP, eqn%mip); call dev_put(eqn, CFDL_MIP0, eqn%mip0)
+  end subroutine
+
+  ! the whole state, for the routines called one by one (calc_coef_p reads
+  ! dc, calc_mip d, update_uvwp dc and gpc)
+  subroutine put_state(eqn)
+    class(uvwp_t) :: eqn
+    call put_inputs(eqn)
+    call dev_put(eqn, CFDL_GPC, eqn%gpc)
     call dev_put(eqn, CFDL_BU, eqn%bu); call dev_put(eqn, CFDL_BV, eqn%bv); call dev_put(eqn, CFDL_BW, eqn%bw)
     call dev_put(eqn, CFDL_D, eqn%d); call dev_put(eqn, CFDL_DC, eqn%dc)
   end subroutine
@@ -213,7 +225,7 @@ contains
     type(intf_t) :: intf(nsubd,nsubd)
     call ensure_ctx(eqn, geom, prop, dt, nit, nsubd)
     call sync_params(eqn, prop, dt)
-    call put_state(eqn)
+    call put_inputs(eqn)
     call cfdl_check(cfdl_update_boundaries(eqn%ctx), 'update_boundaries')
     call cfdl_check(cfdl_solve_uvwp(eqn%ctx), 'solve_uvwp')
     call get_state(eqn)

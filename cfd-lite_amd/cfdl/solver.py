@@ -183,6 +183,14 @@ class UVWP:
             check(self._lib.cfdl_run(self._h, ntstep, ncoef))
 
     # energy equation, src/equations/mod_energy.f90
+    def set_properties(self, rho, mu):
+        """Per-cell density and viscosity (properties_t%rho, %mu)."""
+        rho = np.ascontiguousarray(rho, np.float64)
+        mu = np.ascontiguousarray(mu, np.float64)
+        if rho.size != self.sizes["ap"] or mu.size != self.sizes["ap"]:
+            raise CfdlError("rho and mu need one value per cell")
+        check(self._lib.cfdl_set_properties(self._h, rho.ctypes.data_as(PD), mu.ctypes.data_as(PD)))
+
     def set_thermal(self, cp, tc):
         cp = np.ascontiguousarray(cp, np.float64)
         tc = np.ascontiguousarray(tc, np.float64)

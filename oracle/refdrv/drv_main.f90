@@ -9,6 +9,8 @@
 ! usage: refsim mesh.bin out.bin nsubd rho mu dt ntstep ncoef nit [project/]
 ! With the optional project path the run ends like src/main.f90:89, with the
 ! reference's write_vtubin of the uvwp fields into project/VTK/output-N/.
+! With REFSIM_PROPS=<file> the per-cell density and viscosity come from that
+! file (ne float64 values of rho, then ne of mu) instead of the constants.
 ! With REFSIM_ENERGY=1 in the environment every outer iteration also calls
 ! the reference's solve_energy (src/equations/mod_energy.f90:58-80), which
 ! src/main.f90 never calls, and the energy fields are appended to the dump.
@@ -26,6 +28,8 @@ program refsim
   real :: rho, mu, dt
   integer(8) :: t0, t1, rate
   character(len=8) :: envval
+  character(len=512) :: propfile
+  integer :: plen, up
   logical :: energy
 
   call get_command_argument(1, meshfile)
@@ -51,6 +55,13 @@ program refsim
   phys%prop%mu = mu
   ne = geom%ne
   nbf = geom%nbf
+  call get_environment_variable('REFSIM_PROPS', propfile, plen)
+  if (plen > 0) then
+    open(newunit=up, file=propfile(1:plen), access='stream', form='unformatted', status='old')
+    read(up) phys%prop%rho(1:ne)
+    read(up) phys%prop%mu(1:ne)
+    close(up)
+  end if
 
   open(newunit=u, file=outfile, access='stream', form='unformatted', status='replace')
   write(u) geom%ne, geom%nf, geom%nbf, phys%n_subdomains

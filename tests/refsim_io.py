@@ -30,14 +30,23 @@ def _unlimited_stack():
 
 
 def run_refsim(meshfile, outfile, nsubd, rho, mu, dt, ntstep, ncoef, nit=100, opt="O2", timeout=600,
-               binary=None, project=None, energy=False):
+               binary=None, project=None, energy=False, props=None):
     """project: a directory path ending in '/'; the run then ends with the
-    reference's write_vtubin into project/VTK/output-<ntstep+1>/uvwp.vtu."""
+    reference's write_vtubin into project/VTK/output-<ntstep+1>/uvwp.vtu.
+    props: (rho, mu) per-cell arrays, written next to outfile and read by the
+    driver instead of the constants rho, mu."""
     args = [binary or refsim_path(opt), meshfile, outfile, str(nsubd), repr(float(rho)), repr(float(mu)),
             repr(float(dt)), str(ntstep), str(ncoef), str(nit)]
     if project is not None:
         args.append(project)
     env = dict(os.environ, REFSIM_ENERGY="1" if energy else "0")
+    env.pop("REFSIM_PROPS", None)
+    if props is not None:
+        pf = outfile + ".props"
+        with open(pf, "wb") as f:
+            f.write(np.ascontiguousarray(props[0], np.float64).tobytes())
+            f.write(np.ascontiguousarray(props[1], np.float64).tobytes())
+        env["REFSIM_PROPS"] = pf
     p = subprocess.run(args, capture_output=True, text=True, timeout=timeout,
                        preexec_fn=_unlimited_stack, env=env)
     if p.returncode != 0:

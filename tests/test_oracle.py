@@ -476,3 +476,29 @@ def test_pc_colour_order_is_a_proper_greedy_colouring():
     for name in ("u", "v", "w"):
         assert np.max(np.abs(ref.arr(name) - col.arr(name))) < 1e-3
     assert not np.array_equal(ref.arr("p"), col.arr("p"))
+
+
+@pytest.mark.skipif(not refsim_io.have_refsim("O2"), reason="reference build absent")
+@pytest.mark.parametrize("n,jit,nsubd", [(6, 0.2, 4), (5, 0.1, 1)])
+def test_oracle_matches_reference_nonuniform_properties(n, jit, nsubd):
+    """Per-cell density and viscosity (random in [0.8, 1.2] and [0.005,
+    0.015]) fed to the reference's own properties_t arrays: the face
+    interpolations of rho and mu weigh the two cells differently, which the
+    uniform cases cannot check.  Fields and solver lines equal the
+    reference's bit for bit."""
+    g = meshgen.box_mesh(n, jitter=jit)
+    oc = oracle_lib.OracleCase(g, meshgen.cavity_bc(g), nsubd=nsubd, rho=1.0, mu=0.01, dt=0.02)
+    rng = np.random.default_rng(7)
+    ne = oc.c.ne
+    rho, mu = 0.8 + 0.4 * rng.random(ne), 0.005 + 0.01 * rng.random(ne)
+    with tempfile.TemporaryDirectory() as td:
+        mf, of = os.path.join(td, "m.bin"), os.path.join(td, "o.bin")
+        meshgen.write_refsim_mesh(g, mf)
+        log, _ = refsim_io.run_refsim(mf, of, nsubd, 1.0, 0.01, 0.02, 2, 3, props=(rho, mu))
+        d = refsim_io.read_dump(of)
+    oc.arr("rho")[:] = rho
+    oc.arr("mu")[:] = mu
+    oc.run(2, 3)
+    for k in FIELDS:
+        assert np.array_equal(oc.arr(k), d[k]), k
+    assert [(l[0], l[1]) for l in oc.log()] == [(l[0], l[1]) for l in log]

@@ -52,8 +52,14 @@ def assert_logs_match(gl, ol):
     (8, 0.2, 1, 2, 2),
     (10, 0.15, 4, 1, 3),
     (9, 0.0, 4, 1, 2),
+    (12, 0.1, 2, 1, 3),
+    (10, 0.1, 8, 2, 2),
+    (16, 0.0, 16, 1, 2),
 ])
 def test_run_bitwise(n, jitter, nsubd, ntstep, ncoef):
+    """Whole outer iterations against the oracle on uniform and jittered
+    meshes, with 1, 2, 4, 8 and 16 subdomains (the oracle's own runs with 2
+    and 8 subdomains are checked against the reference)."""
     eq, oc = make_pair(n, jitter, nsubd)
     eq.run(ntstep, ncoef)
     oc.run(ntstep, ncoef)

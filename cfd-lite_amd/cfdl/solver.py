@@ -93,6 +93,24 @@ class UVWP:
             raise CfdlError("pc order must be one of %s" % sorted(self.PC_ORDERS))
         check(self._lib.cfdl_set_pc_order(self._h, self.PC_ORDERS[order]))
 
+    BC_KINDS = {"wall": 0, "lid": 1, "symmetry": 2}
+
+    def set_bc_kind(self, first, last, kind):
+        """Boundary type of the halos first..last (1-based element numbers), as
+        the setters dirichlet0 / lid / symmetry assign it (mod_uvwp.f90:493-570):
+        "wall", "lid" or "symmetry"."""
+        if kind not in self.BC_KINDS:
+            raise CfdlError("bc kind must be one of %s" % sorted(self.BC_KINDS))
+        check(self._lib.cfdl_set_bc_kind(self._h, first, last, self.BC_KINDS[kind]))
+
+    def set_section_bc(self, mesh, name, kind):
+        """set_bc_kind over the halos of the mesh's boundary section name."""
+        if not mesh.names or name not in mesh.names[1:]:
+            raise CfdlError("no boundary section %r" % (name,))
+        esec = mesh.get("esec").reshape(-1, 2)
+        s = mesh.names.index(name)
+        self.set_bc_kind(int(esec[s][0]), int(esec[s][1]), kind)
+
     def set_relaxation(self, alpha_u=1.0, alpha_p=1.0):
         """Config-mode under-relaxation (1, 1 = the reference algorithm)."""
         check(self._lib.cfdl_set_relaxation(self._h, alpha_u, alpha_p))

@@ -11,6 +11,9 @@
 ! reference's write_vtubin of the uvwp fields into project/VTK/output-N/.
 ! With REFSIM_PROPS=<file> the per-cell density and viscosity come from that
 ! file (ne float64 values of rho, then ne of mu) instead of the constants.
+! With REFSIM_SYMMETRY='<name> <name> ...' the uvwp setters of those boundary
+! sections are rebound to the reference's symmetry (mod_uvwp.f90:517-546),
+! which src/equations/mod_uvwp.f90:73-78 never binds.
 ! With REFSIM_ENERGY=1 in the environment every outer iteration also calls
 ! the reference's solve_energy (src/equations/mod_energy.f90:58-80), which
 ! src/main.f90 never calls, and the energy fields are appended to the dump.
@@ -19,6 +22,7 @@ program refsim
   use mod_physics
   use mod_solver
   use mod_vtu_output
+  use mod_uvwp, only: symmetry
   implicit none
   type(geometry_t) :: geom
   type(phys_t) :: phys
@@ -29,7 +33,8 @@ program refsim
   integer(8) :: t0, t1, rate
   character(len=8) :: envval
   character(len=512) :: propfile
-  integer :: plen, up
+  integer :: plen, up, ib
+  character(len=512) :: symlist
   logical :: energy
 
   call get_command_argument(1, meshfile)
@@ -62,6 +67,11 @@ program refsim
     read(up) phys%prop%mu(1:ne)
     close(up)
   end if
+  call get_environment_variable('REFSIM_SYMMETRY', symlist)
+  do ib=1,size(phys%uvwp%bcs)
+    if (index(' '//trim(symlist)//' ', ' '//trim(phys%uvwp%bcs(ib)%name)//' ') > 0) &
+      phys%uvwp%bcs(ib)%coef => symmetry
+  end do
 
   open(newunit=u, file=outfile, access='stream', form='unformatted', status='replace')
   write(u) geom%ne, geom%nf, geom%nbf, phys%n_subdomains

@@ -105,11 +105,12 @@ def box_mesh(nx, ny=None, nz=None, lx=1.0, ly=1.0, lz=1.0, jitter=0.0, seed=2026
                 ne=ne, nbf=nbf)
 
 
-def cavity_bc(mesh):
-    """Lid-driven cavity binding, src/equations/mod_uvwp.f90:73-78."""
+def cavity_bc(mesh, symmetry=()):
+    """Lid-driven cavity binding, src/equations/mod_uvwp.f90:73-78; the
+    sections named in symmetry take the symmetry setter (:517-546) instead."""
     kinds = [-1]
     for name in mesh["names"][1:]:
-        kinds.append(BC_LID if name == "top" else BC_WALL)
+        kinds.append(BC_SYMMETRY if name in symmetry else BC_LID if name == "top" else BC_WALL)
     return np.array(kinds, np.int32)
 
 

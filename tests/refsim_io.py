@@ -30,17 +30,20 @@ def _unlimited_stack():
 
 
 def run_refsim(meshfile, outfile, nsubd, rho, mu, dt, ntstep, ncoef, nit=100, opt="O2", timeout=600,
-               binary=None, project=None, energy=False, props=None):
+               binary=None, project=None, energy=False, props=None, symmetry=()):
     """project: a directory path ending in '/'; the run then ends with the
     reference's write_vtubin into project/VTK/output-<ntstep+1>/uvwp.vtu.
     props: (rho, mu) per-cell arrays, written next to outfile and read by the
-    driver instead of the constants rho, mu."""
+    driver instead of the constants rho, mu.
+    symmetry: boundary section names whose uvwp setter the driver rebinds to
+    the reference's symmetry (mod_uvwp.f90:517-546)."""
     args = [binary or refsim_path(opt), meshfile, outfile, str(nsubd), repr(float(rho)), repr(float(mu)),
             repr(float(dt)), str(ntstep), str(ncoef), str(nit)]
     if project is not None:
         args.append(project)
     env = dict(os.environ, REFSIM_ENERGY="1" if energy else "0")
     env.pop("REFSIM_PROPS", None)
+    env["REFSIM_SYMMETRY"] = " ".join(symmetry)
     if props is not None:
         pf = outfile + ".props"
         with open(pf, "wb") as f:

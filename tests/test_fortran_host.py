@@ -34,24 +34,27 @@ def solver_lines(stdout):
 
 @pytest.mark.skipif(not (os.path.exists(DROPSIM) and refsim_io.have_refsim("O2")),
                     reason="reference / drop-in builds absent")
-@pytest.mark.parametrize("n,jit,nsubd,nt,nc,energy", [(8, 0.0, 4, 2, 3, False), (7, 0.2, 4, 1, 3, False),
-                                                      (6, 0.1, 1, 2, 2, False), (6, 0.15, 4, 2, 2, True),
-                                                      (5, 0.0, 1, 1, 3, True)])
-def test_dropin_modules_reproduce_the_reference(n, jit, nsubd, nt, nc, energy):
+@pytest.mark.parametrize("n,jit,nsubd,nt,nc,energy,sym", [
+    (8, 0.0, 4, 2, 3, False, ()), (7, 0.2, 4, 1, 3, False, ()), (6, 0.1, 1, 2, 2, False, ()),
+    (6, 0.15, 4, 2, 2, True, ()), (5, 0.0, 1, 1, 3, True, ()),
+    (7, 0.2, 4, 2, 2, False, ("north", "bottom"))])
+def test_dropin_modules_reproduce_the_reference(n, jit, nsubd, nt, nc, energy, sym):
     """Both programs also end with the reference's own write_vtubin, which
     reads the drop-in's uvwp_t: the VTU files must be identical.  With
     energy, every outer iteration also calls solve_energy (the drop-in
-    mod_energy on the GPU in dropsim)."""
+    mod_energy on the GPU in dropsim).  With sym, both drivers bind those
+    sections' setters to their module's symmetry."""
     g = meshgen.box_mesh(n, jitter=jit)
     vtu = os.path.join("VTK", "output-%d" % (nt + 1), "uvwp.vtu")
     with tempfile.TemporaryDirectory() as td:
         mf = os.path.join(td, "m.bin")
         meshgen.write_refsim_mesh(g, mf)
         refsim_io.run_refsim(mf, os.path.join(td, "r.bin"), nsubd, 1.0, 0.01, 0.01, nt, nc,
-                             project=os.path.join(td, "r") + "/", energy=energy)
+                             project=os.path.join(td, "r") + "/", energy=energy, symmetry=sym)
         ref_out = refsim_io.run_refsim.last_stdout
         refsim_io.run_refsim(mf, os.path.join(td, "d.bin"), nsubd, 1.0, 0.01, 0.01, nt, nc,
-                             binary=DROPSIM, project=os.path.join(td, "d") + "/", energy=energy)
+                             binary=DROPSIM, project=os.path.join(td, "d") + "/", energy=energy,
+                             symmetry=sym)
         drop_out = refsim_io.run_refsim.last_stdout
         r = refsim_io.read_dump(os.path.join(td, "r.bin"))
         d = refsim_io.read_dump(os.path.join(td, "d.bin"))

@@ -502,3 +502,32 @@ def test_oracle_matches_reference_nonuniform_properties(n, jit, nsubd):
     for k in FIELDS:
         assert np.array_equal(oc.arr(k), d[k]), k
     assert [(l[0], l[1]) for l in oc.log()] == [(l[0], l[1]) for l in log]
+
+
+@pytest.mark.skipif(not refsim_io.have_refsim("O2"), reason="reference build absent")
+@pytest.mark.parametrize("n,jit,nsubd,sym", [(6, 0.2, 4, ("north", "bottom")), (5, 0.15, 1, ("east",))])
+def test_oracle_matches_reference_symmetry(n, jit, nsubd, sym):
+    """Symmetry sections: the driver rebinds their setters to the reference's
+    symmetry (mod_uvwp.f90:517-546), which reflects the inner cell's velocity
+    about the face normal and sets 'zero_flux', so the momentum assembly
+    skips their wall-shear term (:266).  On a jittered mesh the normals are
+    off the axes and every velocity component enters the reflection.  Fields
+    and solver lines equal the reference's bit for bit."""
+    g = meshgen.box_mesh(n, jitter=jit)
+    oc = oracle_lib.OracleCase(g, meshgen.cavity_bc(g, sym), nsubd=nsubd, rho=1.0, mu=0.01, dt=0.02)
+    with tempfile.TemporaryDirectory() as td:
+        mf, of = os.path.join(td, "m.bin"), os.path.join(td, "o.bin")
+        meshgen.write_refsim_mesh(g, mf)
+        log, _ = refsim_io.run_refsim(mf, of, nsubd, 1.0, 0.01, 0.02, 2, 3, symmetry=sym)
+        d = refsim_io.read_dump(of)
+    oc.run(2, 3)
+    for k in FIELDS:
+        assert np.array_equal(oc.arr(k), d[k]), k
+    assert [l[:2] for l in oc.log()] == [l[:2] for l in log]
+    for a, b in zip(oc.log(), log):  # the reference prints 4 significant digits
+        assert np.allclose(a[2:], b[2:], rtol=6e-4, atol=0), (a, b)
+    # the symmetry halos differ from a wall's: their velocity is not zero
+    sec = [i for i, nm in enumerate(g["names"]) if nm in sym]
+    esec = np.asarray(g["esec"]).reshape(-1, 2)
+    halo = np.concatenate([np.arange(esec[i][0], esec[i][1] + 1) - 1 for i in sec])
+    assert np.abs(d["u"][halo]).max() + np.abs(d["v"][halo]).max() > 0

@@ -202,10 +202,12 @@ def main():
                     help="run from rest to a 1e-6 scaled residual (or --max-outer) instead")
     ap.add_argument("--max-outer", type=int, default=500)
     ap.add_argument("--tol", type=float, default=1e-6)
-    ap.add_argument("--pc-order", choices=("reference", "colored"), default="reference",
+    ap.add_argument("--pc-order", choices=("reference", "colored", "mixed"), default="reference",
                     help="visiting order of the pressure-correction smoother; 'colored' is an "
                          "extension (greedy colouring, one launch per colour) reported on its "
-                         "own line, not as the headline value")
+                         "own line, not as the headline value; 'mixed' also runs the colour "
+                         "order with FP32 sweeps under FP64 defect correction and reports its "
+                         "field error against the FP64 colour order")
     ap.add_argument("--pcie", action="store_true",
                     help="also time outer iterations with the host transfers of the drop-in "
                          "Fortran solve_uvwp (state up, state and work arrays down)")
@@ -306,17 +308,22 @@ def main():
           "outer iteration %.1f ms" % (st["pc_launches"], st["pc_cells"] / max(st["pc_launches"], 1),
                                        st["pc_levels_per_sym_sweep"], 1e3 * secs / args.steps),
           file=sys.stderr)
-    if args.pc_order == "colored":
-        colored_rate(cfdl, mesh, mu, ne, args)
+    if args.pc_order in ("colored", "mixed"):
+        ref = colored_rate(cfdl, mesh, mu, ne, args, "colored")
+    if args.pc_order == "mixed":
+        colored_rate(cfdl, mesh, mu, ne, args, "mixed", ref)
 
 
-def colored_rate(cfdl, mesh, mu, ne, args):
-    """The same outer iterations with the colour-order pressure-correction
-    smoother (an extension; the headline value keeps the reference order),
-    from rest on a fresh context.  Printed as its own JSON line."""
+def colored_rate(cfdl, mesh, mu, ne, args, order, fp64=None):
+    """The same outer iterations with a colour-order pressure-correction
+    smoother (extensions; the headline value keeps the reference order),
+    from rest on a fresh context.  Printed as its own JSON line.  With fp64
+    (the FP64 colour order's fields after the same iterations) the line also
+    gives each field's relative L2 difference from them.  Returns the
+    fields."""
     eq = cfdl.UVWP.cavity(mesh, rho=1.0, mu=mu, dt=0.01, nit=100)
     eq.set_relaxation(args.alpha_u, args.alpha_p)
-    eq.set_pc_order("colored")
+    eq.set_pc_order(order)
     n = 0
     for k in range(args.warmup + args.steps):
         if k == args.warmup:
@@ -329,10 +336,19 @@ def colored_rate(cfdl, mesh, mu, ne, args):
             eq.update_time()
     eq.synchronize()
     secs = time.perf_counter() - t
-    print(json.dumps({"mode": "pc-colored", "metric": METRIC,
-                      "value": round(ne * args.steps / secs, 1), "unit": "cell-iterations/s",
-                      "ms_per_step": round(1e3 * secs / args.steps, 3), "steps": args.steps,
-                      "scaled_residual_reached": scaled_residual(eq.log())}), flush=True)
+    fields = {k: eq.get(k)[:ne] for k in ("u", "v", "w", "p")}
+    line = {"mode": "pc-" + order, "metric": METRIC, "value": round(ne * args.steps / secs, 1),
+            "unit": "cell-iterations/s", "ms_per_step": round(1e3 * secs / args.steps, 3),
+            "steps": args.steps, "outer_iterations_from_rest": args.warmup + args.steps,
+            "dtype": "f32 pc sweeps, f64 otherwise" if order == "mixed" else "f64",
+            "scaled_residual_reached": scaled_residual(eq.log())}
+    if fp64 is not None:
+        vel = np.linalg.norm(np.concatenate([fp64[k] for k in ("u", "v", "w")]))
+        line["rel_l2_vs_fp64_colored"] = {
+            k: float(np.linalg.norm(fields[k] - fp64[k]) /
+                     (vel if k != "p" else np.linalg.norm(fp64[k]))) for k in fields}
+    print(json.dumps(line), flush=True)
+    return fields
 
 
 if __name__ == "__main__":

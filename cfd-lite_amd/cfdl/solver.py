@@ -83,12 +83,14 @@ class UVWP:
                                   C.byref(h)))
         return cls(h, cls._sizes(ne, nf, nbf))
 
-    PC_ORDERS = {"reference": 0, "colored": 1}
+    PC_ORDERS = {"reference": 0, "colored": 1, "mixed": 2}
 
     def set_pc_order(self, order="reference"):
         """Visiting order of the multi-subdomain pressure-correction smoother:
-        "reference" (the reference's order, the default) or "colored" (greedy
-        colouring of each subdomain, swept colour by colour; an extension)."""
+        "reference" (the reference's order, the default), "colored" (greedy
+        colouring of each subdomain, swept colour by colour; an extension) or
+        "mixed" (the colour order with FP32 sweeps on the correction system
+        under FP64 defect correction at every residual check; an extension)."""
         if order not in self.PC_ORDERS:
             raise CfdlError("pc order must be one of %s" % sorted(self.PC_ORDERS))
         check(self._lib.cfdl_set_pc_order(self._h, self.PC_ORDERS[order]))

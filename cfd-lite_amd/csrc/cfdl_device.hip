@@ -142,6 +142,14 @@ struct SysDesc {
   const double *rap;  // 1/ap per cell (tiled LDS sweeps), correctly rounded
 };
 
+// A single right-hand side in FP32: the correction system of the
+// mixed-precision pressure-correction solve (ap, anb, residual, correction)
+struct SysDesc32 {
+  const int *row, *nbr;
+  const float *ap, *anb, *b;
+  float *phi;
+};
+
 // (x / a) / s, correctly rounded at each step, from y = 1/a and ys = 1/s
 // correctly rounded.  A quotient estimate q0 = x*y corrected once by the
 // exact remainder, q0 + (x - a*q0)*y, is the correctly rounded x/a
@@ -678,6 +686,57 @@ __global__ void k_gs_level(SysDesc s, const int *__restrict__ cells, int n, doub
   }
 }
 
+// The same update in FP32 on a correction system: cells of one colour (all
+// independent), row sum in CSR order, then (sum + (sor-1)*ap*phi)/ap/sor with
+// IEEE single-precision divisions.
+__global__ void k_gs_level32(SysDesc32 s, const int *__restrict__ cells, int n, float sor,
+                             const int *__restrict__ active) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n || !active[0]) return;
+  const int e = cells[t];
+  const int r0 = s.row[e], r1 = s.row[e + 1];
+  const float a = s.ap[e];
+  float sum = s.b[e];
+  if (r1 - r0 == 6) {
+    int nb[6];
+    float an[6], v[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      nb[k] = s.nbr[r0 + k];
+      an[k] = s.anb[r0 + k];
+    }
+#pragma unroll
+    for (int k = 0; k < 6; k++) v[k] = s.phi[nb[k]];
+#pragma unroll
+    for (int k = 0; k < 6; k++) sum = sum + an[k] * v[k];
+  } else {
+    for (int k = r0; k < r1; k++) sum = sum + s.anb[k] * s.phi[s.nbr[k]];
+  }
+  s.phi[e] = (sum + (sor - 1.0f) * a * s.phi[e]) / a / sor;
+}
+
+// Mixed-precision pc solve: FP32 copies of ap and anb, and a zero correction
+// (cells and halos).  Then, at every residual check, the correction is added
+// to the FP64 values and cleared (once the solve has stopped there is
+// nothing to add, so the check-time update is skipped).
+__global__ void k_mixed_prep(int ne, int L, int nall, const double *__restrict__ ap,
+                             const double *__restrict__ anb, float *__restrict__ a32,
+                             float *__restrict__ anb32, float *__restrict__ d32) {
+  const int stride = gridDim.x * blockDim.x, n = max(L, max(ne, nall));
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    if (i < L) anb32[i] = static_cast<float>(anb[i]);
+    if (i < ne) a32[i] = static_cast<float>(ap[i]);
+    if (i < nall) d32[i] = 0.0f;
+  }
+}
+__global__ void k_mixed_update(int nall, double *__restrict__ phic, float *__restrict__ d32,
+                               const int *__restrict__ active) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nall || (active && !active[0])) return;
+  phic[i] = phic[i] + static_cast<double>(d32[i]);
+  d32[i] = 0.0f;
+}
+
 // Space-time tiled sweeps of the pressure-correction system.  The host
 // numbers that system so that every (launch, tile) batch is one contiguous
 // cell range whose dependency-level segments are contiguous sub-ranges:
@@ -1045,7 +1104,10 @@ k_gs_tiles_lds(SysDesc s, const int4 *__restrict__ bdesc, const int *__restrict_
 // Residual partial sums: block (x, y) covers cells [beg_y + x*chunk, ...).
 // absmax: |r| and max|r| (solve_gs, :309-321); else signed r and max(r,0)
 // (calc_residual, :239-251).  Fixed-order tree, so reproducible.
-__global__ void k_resid_partial(SysDesc s, int chunk, int nblk, int absmax, double *part) {
+// rout, when given, receives each cell's r rounded to FP32 (the right-hand
+// side of the mixed-precision correction system).
+__global__ void k_resid_partial(SysDesc s, int chunk, int nblk, int absmax, double *part,
+                                float *rout) {
   __shared__ double ssum[kBlock], smax[kBlock];
   const int y = blockIdx.y, x = blockIdx.x, tid = threadIdx.x;
   const int lo = s.beg[y] + x * chunk;
@@ -1071,6 +1133,7 @@ __global__ void k_resid_partial(SysDesc s, int chunk, int nblk, int absmax, doub
       for (int k = r0; k < r1; k++) r = r + s.anb[k] * phi[s.nbr[k]];
     }
     r = r - s.ap[e] * phi[e];
+    if (rout) rout[e] = static_cast<float>(r);
     if (absmax) r = fabs(r);
     mx = fmax(r, mx);
     sum = sum + r * r;
@@ -1227,8 +1290,8 @@ __global__ void k_gather(int n, const int *idx, const double *src, double *dst) 
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dst[i] = src[idx[i]];
 }
-__global__ void k_exchange(int n, const int *dst, const int *src, double *phic,
-                           const int *active) {
+template <typename T>
+__global__ void k_exchange(int n, const int *dst, const int *src, T *phic, const int *active) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n && active[0]) phic[dst[i]] = phic[src[i]];
 }
@@ -1325,6 +1388,9 @@ struct cfdl_ctx {
   // colours of a greedy colouring in each subdomain's local order, as levels
   Sched pcc;
   int pc_order = CFDL_PC_REFERENCE;
+  // FP32 correction system of CFDL_PC_COLORED_MIXED (colour numbering):
+  // ap, anb, residual and correction (cells and halos)
+  float *a32 = nullptr, *anb32 = nullptr, *r32 = nullptr, *d32 = nullptr;
   int pc_nlev = 0;
   // space-time tiled schedule of the pc sweeps (the backward sweep runs it
   // in reverse)
@@ -1499,6 +1565,26 @@ static int pc_sweep(cfdl_ctx *c, const Sched &s, const SysDesc &sys, bool forwar
   return CFDL_OK;
 }
 
+// An FP32 colour sweep of the mixed-precision correction system (sor 1.02 in
+// single precision), colours in order forward and in reverse backward.
+static int pc_sweep32(cfdl_ctx *c, const Sched &s, const SysDesc32 &sys, bool forward,
+                      const int *active) {
+  cfdl_ctx::Sample *smp = nullptr;
+  CALL(sample_begin(c, &smp));
+  for (int i = 0; i < s.nlev; i++) {
+    const int l = forward ? i : s.nlev - 1 - i;
+    const int n = s.ptr[l + 1] - s.ptr[l];
+    hipLaunchKernelGGL(k_gs_level32, dim3(nblocks(n)), dim3(kBlock), 0, c->st, sys,
+                       s.cells + s.ptr[l], n, 1.02f, active);
+  }
+  CALL(sample_end(c, smp, s.nlev, c->ne));
+  c->pc_launches += s.nlev;
+  c->pc_cells += c->ne;
+  c->gs_launches += s.nlev;
+  HIPCHK(hipGetLastError());
+  return CFDL_OK;
+}
+
 // One tiled sweep of a system in its tile numbering: the launches of the
 // schedule in order (forward) or in reverse (backward), NC right-hand sides.
 template <int NC>
@@ -1552,9 +1638,10 @@ static int pc_sweep_tiled(cfdl_ctx *c, const SysDesc &sys, bool forward, const i
   return CFDL_OK;
 }
 
-static int residual(cfdl_ctx *c, const SysDesc &sys, int nblk, int chunk, int absmax) {
+static int residual(cfdl_ctx *c, const SysDesc &sys, int nblk, int chunk, int absmax,
+                    float *rout = nullptr) {
   hipLaunchKernelGGL(k_resid_partial, dim3(nblk, sys.ny), dim3(kBlock), 0, c->st, sys, chunk,
-                     nblk, absmax, c->part);
+                     nblk, absmax, c->part, rout);
   hipLaunchKernelGGL(k_resid_final, dim3(sys.ny), dim3(kBlock), 0, c->st, c->part, nblk, c->red);
   HIPCHK(hipGetLastError());
   return CFDL_OK;
@@ -1654,8 +1741,9 @@ static int solve_gs_n(cfdl_ctx *c, int ny, double **phi, const double **b,
 
 static int multi_subdomain_solver(cfdl_ctx *c) {
   const int ne = c->ne, ns = c->nsubd;
-  // the colour order runs on the colour numbering of the same system
-  const bool col = c->pc_order == CFDL_PC_COLORED;
+  // the colour orders run on the colour numbering of the same system
+  const bool mixed = c->pc_order == CFDL_PC_COLORED_MIXED;
+  const bool col = c->pc_order == CFDL_PC_COLORED || mixed;
   const int *l2g = col ? c->l2g_c : c->l2g_cell, *rowp = col ? c->rowp_c : c->rowp;
   const int *nbrp = col ? c->nbrp_c : c->nbrp, *xsrc = col ? c->xsrc_c : c->xsrc;
   // assemble_coef, mod_subdomains.f90:162-189
@@ -1681,7 +1769,18 @@ static int multi_subdomain_solver(cfdl_ctx *c) {
   s.ap = c->ap_p;
   s.anb = c->anb_p;
   s.rap = c->rap_p;
-  CALL(residual(c, s, c->nblk_p, c->chunk_p, 0));
+  // mixed precision (an extension): FP64 defect correction around FP32
+  // sweeps.  The sweeps solve ap*d - sum(anb*d_nb) = r for a correction d
+  // (halos exchanged like the values); at every residual check the
+  // correction is added to the FP64 values and cleared, and the FP64 true
+  // residual gives both the stop test's norms and the next r.
+  const int nall = ne + c->nh;
+  SysDesc32 s32{rowp, nbrp, c->a32, c->anb32, c->r32, c->d32};
+  if (mixed)
+    hipLaunchKernelGGL(k_mixed_prep, dim3(std::min(nblocks(std::max(c->Lp, nall)), 4096)),
+                       dim3(kBlock), 0, c->st,
+                       ne, c->Lp, nall, c->ap_p, c->anb_p, c->a32, c->anb32, c->d32);
+  CALL(residual(c, s, c->nblk_p, c->chunk_p, 0, mixed ? c->r32 : nullptr));
   hipLaunchKernelGGL(k_pc_init, dim3(1), dim3(64), 0, c->st, c->pctl, c->red, sc, c->nit);
   SysDesc s1 = s;  // the sweeps see the whole concatenated system as one RHS
   s1.ny = 1;
@@ -1689,7 +1788,10 @@ static int multi_subdomain_solver(cfdl_ctx *c) {
   const int nblock = (c->nit + 1) / 2;
   for (int blk = 1; blk <= nblock; blk++) {
     for (int rep = 0; rep < 2; rep++) {  // smoother_gs(..., 2): two symmetric iterations
-      if (c->pc_order == CFDL_PC_COLORED) {
+      if (mixed) {
+        CALL(pc_sweep32(c, c->pcc, s32, true, act));
+        CALL(pc_sweep32(c, c->pcc, s32, false, act));
+      } else if (c->pc_order == CFDL_PC_COLORED) {
         CALL(pc_sweep(c, c->pcc, s1, true, act));
         CALL(pc_sweep(c, c->pcc, s1, false, act));
       } else if (c->tiled) {
@@ -1700,14 +1802,25 @@ static int multi_subdomain_solver(cfdl_ctx *c) {
         CALL(pc_sweep(c, c->pcl, s1, false, act));
       }
     }
-    if (c->nx > 0)
-      hipLaunchKernelGGL(k_exchange, dim3(nblocks(c->nx)), dim3(kBlock), 0, c->st, c->nx, c->xdst,
-                         xsrc, c->phic_p, act);
+    if (c->nx > 0 && mixed)
+      hipLaunchKernelGGL(k_exchange<float>, dim3(nblocks(c->nx)), dim3(kBlock), 0, c->st, c->nx,
+                         c->xdst, xsrc, c->d32, act);
+    else if (c->nx > 0)
+      hipLaunchKernelGGL(k_exchange<double>, dim3(nblocks(c->nx)), dim3(kBlock), 0, c->st, c->nx,
+                         c->xdst, xsrc, c->phic_p, act);
     const int has_res = ((2 * blk) % 10 == 0) ? 1 : 0;
-    if (has_res) CALL(residual(c, s, c->nblk_p, c->chunk_p, 0));
+    if (has_res && mixed)
+      hipLaunchKernelGGL(k_mixed_update, dim3(nblocks(nall)), dim3(kBlock), 0, c->st, nall,
+                         c->phic_p, c->d32, act);
+    if (has_res) CALL(residual(c, s, c->nblk_p, c->chunk_p, 0, mixed ? c->r32 : nullptr));
     hipLaunchKernelGGL(k_pc_block_end, dim3(1), dim3(64), 0, c->st, c->pctl, c->red, sc, c->nit,
                        has_res);
   }
+  // the correction left since the last check (a cap of nit iterations that
+  // is not a multiple of ten), added whether or not the solve is still active
+  if (mixed)
+    hipLaunchKernelGGL(k_mixed_update, dim3(nblocks(nall)), dim3(kBlock), 0, c->st, nall,
+                       c->phic_p, c->d32, static_cast<const int *>(nullptr));
   // update_phi, mod_subdomains.f90:214-229
   hipLaunchKernelGGL(k_scatter, dim3(nblocks(ne)), dim3(kBlock), 0, c->st, ne, l2g,
                      c->phic_p, c->fd.phic);
@@ -2412,9 +2525,16 @@ int cfdl_set_relaxation(cfdl_ctx *c, double alpha_u, double alpha_p) {
 
 int cfdl_set_pc_order(cfdl_ctx *c, int order) {
   CALL(check_ctx(c));
-  if (order != CFDL_PC_REFERENCE && order != CFDL_PC_COLORED) {
-    set_error("cfdl_set_pc_order: order must be CFDL_PC_REFERENCE or CFDL_PC_COLORED");
+  if (order != CFDL_PC_REFERENCE && order != CFDL_PC_COLORED && order != CFDL_PC_COLORED_MIXED) {
+    set_error("cfdl_set_pc_order: order must be CFDL_PC_REFERENCE, CFDL_PC_COLORED or "
+              "CFDL_PC_COLORED_MIXED");
     return CFDL_EINVAL;
+  }
+  if (order == CFDL_PC_COLORED_MIXED && !c->d32) {
+    CALL(dalloc(c, &c->a32, c->ne));
+    CALL(dalloc(c, &c->anb32, c->Lp));
+    CALL(dalloc(c, &c->r32, c->ne));
+    CALL(dalloc(c, &c->d32, static_cast<int64_t>(c->ne) + c->nh));
   }
   c->pc_order = order;
   return CFDL_OK;

@@ -156,9 +156,18 @@ int cfdl_set_relaxation(cfdl_ctx *c, double alpha_u, double alpha_p);
    of its earlier interior neighbours has) and the sweeps visit the cells
    colour by colour (forward) and in reverse (backward) -- the same
    Gauss-Seidel update in another sequential order, with every stop rule
-   unchanged, in a few fully parallel launches per sweep.  No effect with one
-   subdomain (the pc solve is then solve_gs on the global system). */
-enum { CFDL_PC_REFERENCE = 0, CFDL_PC_COLORED = 1 };
+   unchanged, in a few fully parallel launches per sweep.
+   CFDL_PC_COLORED_MIXED (no counterpart in the reference either) is the
+   colour order with an FP32 inner solve under FP64 defect correction: the
+   sweeps run in single precision on the correction system
+   ap*d - sum(anb*d_nb) = r, and at every residual check (every ten
+   iterations, as in the reference) d is added to the FP64 values and
+   cleared, and the FP64 true residual r = b + sum(anb*pc_nb) - ap*pc gives
+   the stop test's norms and the next right-hand side.  Assembly, residuals,
+   stop rules and the fields outside the pc solve stay FP64.
+   Neither colour order has an effect with one subdomain (the pc solve is
+   then solve_gs on the global system). */
+enum { CFDL_PC_REFERENCE = 0, CFDL_PC_COLORED = 1, CFDL_PC_COLORED_MIXED = 2 };
 int cfdl_set_pc_order(cfdl_ctx *c, int order);
 
 /* Host-side state a Fortran drop-in keeps authoritative (properties_t and

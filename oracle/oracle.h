@@ -88,7 +88,9 @@ typedef struct or_case {
   double *t, *gt, *h, *h0, *gh;
   /* pressure-correction visiting order of the subdomain smoother: 0 is the
      reference's (local cell order); 1 visits the cells colour by colour
-     (or_set_pc_order), an extension the reference does not have */
+     (or_set_pc_order), an extension the reference does not have; 2 is the
+     colour order with FP32 sweeps on the correction system under FP64
+     defect correction at every residual check (an extension as well) */
   int pc_order;
   int **pc_perm; /* per subdomain: its ne cells (1-based) in visiting order */
 } or_case;

@@ -246,7 +246,7 @@ void or_set_pc_order(or_case *c, int order) {
     c->pc_perm = NULL;
   }
   c->pc_order = order;
-  if (order != 1) return;
+  if (order != 1 && order != 2) return;
   c->pc_perm = calloc((size_t)c->nsubd, sizeof(int *));
   for (int cc = 0; cc < c->nsubd; cc++) {
     const or_subd *s = &c->sd[cc];
@@ -272,9 +272,129 @@ void or_set_pc_order(or_case *c, int order) {
   }
 }
 
+/* Mixed-precision colour order (an extension, no counterpart in the
+   reference): smoother_gs_perm's update in single precision on the
+   correction system ap*d - sum(anb*d_nb) = r, cells in perm order forward
+   and reversed backward, sor 1.02f, IEEE single divisions. */
+static void smoother_gs_perm32(float *d, const float *a, const float *anb, const float *r,
+                               const int *ef2nb_idx, const int *ef2nb, int ne, const int *perm,
+                               int nit) {
+  const float sor = 1.02f;
+  for (int it = 1; it <= nit; it++) {
+    for (int dir = 0; dir < 2; dir++) {
+      for (int k = 0; k < ne; k++) {
+        int e = perm[dir == 0 ? k : ne - 1 - k];
+        float sum = r[e - 1];
+        for (int idx = ef2nb_idx[e - 1]; idx <= ef2nb_idx[e] - 1; idx++)
+          sum = sum + anb[idx - 1] * d[PK_E(ef2nb[idx - 1]) - 1];
+        d[e - 1] = (sum + (sor - 1.0f) * a[e - 1] * d[e - 1]) / a[e - 1] / sor;
+      }
+    }
+  }
+}
+
+/* update_halos (mod_subdomains.f90:191-212) on the FP32 corrections */
+static void update_halos32(const or_intf *t, const or_subd *s1, const or_subd *s2, float *d1,
+                           float *d2) {
+  if (t->c1 == t->c2) return;
+  for (int cs = 0; cs < t->ncs; cs++) {
+    int g1 = PK_E(t->index1[cs]), lf1 = PK_F(t->index1[cs]);
+    int gnb1 = PK_E(s1->ef2nb[s1->ef2nb_idx[g1 - 1] + lf1 - 2]);
+    int g2 = PK_E(t->index2[cs]), lf2 = PK_F(t->index2[cs]);
+    int gnb2 = PK_E(s2->ef2nb[s2->ef2nb_idx[g2 - 1] + lf2 - 2]);
+    d1[gnb1 - 1] = d2[g2 - 1];
+    d2[gnb2 - 1] = d1[g1 - 1];
+  }
+}
+
+/* The FP64 residual of calc_residual (norms) together with each cell's
+   r = b + sum(anb*phi_nb) - ap*phi rounded to FP32. */
+static void residual_mixed(const or_subd *s, float *r32, double *res, double *rmax) {
+  or_calc_residual(s->phic, s->ap, s->anb, s->b, s->ef2nb_idx, s->ef2nb, s->ne, s->nf, s->nbf,
+                   res, rmax);
+  for (int e = 1; e <= s->ne; e++)
+    r32[e - 1] = (float)(gs_sum(s->phic, s->anb, s->b, s->ef2nb_idx, s->ef2nb, e) -
+                         s->ap[e - 1] * s->phic[e - 1]);
+}
+
+/* multi_subdomain_solver (mod_solver.f90:124-189, quirks Q1-Q3 kept) with
+   the mixed-precision inner solve: the two-sweep blocks, halo exchanges and
+   residual checks of the reference loop, on an FP32 correction d; at every
+   check phic += d on cells and halos, d = 0, and the FP64 residual of the
+   updated phic gives the norms and the next r; once more after the loop. */
+static void multi_subdomain_solver_mixed(or_case *c, const char *name, double *phi) {
+  int ns = c->nsubd, nit = c->nit;
+  assemble_coef(c, phi);
+  float **a32 = calloc((size_t)ns, sizeof(float *)), **anb32 = calloc((size_t)ns, sizeof(float *));
+  float **r32 = calloc((size_t)ns, sizeof(float *)), **d32 = calloc((size_t)ns, sizeof(float *));
+  double res, rmax, ri = 0.0, rf = 0.0, rmt = 0.0;
+  for (int cc = 0; cc < ns; cc++) {
+    or_subd *s = &c->sd[cc];
+    int L = s->ef2nb_idx[s->ne] - 1;
+    a32[cc] = malloc((size_t)s->ne * sizeof(float));
+    anb32[cc] = malloc((size_t)L * sizeof(float));
+    r32[cc] = malloc((size_t)s->ne * sizeof(float));
+    d32[cc] = calloc((size_t)(s->ne + s->nbf), sizeof(float));
+    for (int e = 0; e < s->ne; e++) a32[cc][e] = (float)s->ap[e];
+    for (int k = 0; k < L; k++) anb32[cc][k] = (float)s->anb[k];
+    residual_mixed(s, r32[cc], &res, &rmax);
+    ri = ri + res * res;
+    rmt = fmax(rmt, rmax);
+  }
+  ri = sqrt(ri / (double)ns);
+  int it = 0;
+  rf = ri;
+  double target = ri / 10.0;
+  while (it < nit && rf > target) {
+    for (int cc = 0; cc < ns; cc++) {
+      or_subd *s = &c->sd[cc];
+      smoother_gs_perm32(d32[cc], a32[cc], anb32[cc], r32[cc], s->ef2nb_idx, s->ef2nb, s->ne,
+                         c->pc_perm[cc], 2);
+    }
+    it += 2;
+    for (int i = 0; i < ns; i++)
+      for (int j = i + 1; j < ns; j++)
+        update_halos32(&c->intf[i * ns + j], &c->sd[i], &c->sd[j], d32[i], d32[j]);
+    if (it % 10 == 0) {
+      for (int cc = 0; cc < ns; cc++) {
+        or_subd *s = &c->sd[cc];
+        for (int i = 0; i < s->ne + s->nbf; i++) {
+          s->phic[i] = s->phic[i] + (double)d32[cc][i];
+          d32[cc][i] = 0.0f;
+        }
+        residual_mixed(s, r32[cc], &res, &rmax);
+        rf = rf + res * res; /* Q1 */
+        rmt = fmax(rmt, rmax);
+      }
+      rf = sqrt(rf / (double)ns);
+    }
+  }
+  for (int cc = 0; cc < ns; cc++) {
+    or_subd *s = &c->sd[cc];
+    for (int i = 0; i < s->ne + s->nbf; i++) s->phic[i] = s->phic[i] + (double)d32[cc][i];
+    free(a32[cc]);
+    free(anb32[cc]);
+    free(r32[cc]);
+    free(d32[cc]);
+  }
+  free(a32);
+  free(anb32);
+  free(r32);
+  free(d32);
+  log_line(c, name, it, ri, rf, rmt);
+  for (int cc = 0; cc < ns; cc++) {
+    or_subd *s = &c->sd[cc];
+    for (int e = 1; e <= s->ne; e++) phi[c->g2gf[c->g2gf_idx[cc] + e - 2] - 1] = s->phic[e - 1];
+  }
+}
+
 /* multi_subdomain_solver, src/modules/mod_solver.f90:124-189 (quirks Q1-Q3) */
 static void multi_subdomain_solver(or_case *c, const char *name, double *phi) {
   int ns = c->nsubd, nit = c->nit;
+  if (c->pc_order == 2) {
+    multi_subdomain_solver_mixed(c, name, phi);
+    return;
+  }
   assemble_coef(c, phi);
   double res, rmax, ri = 0.0, rf = 0.0, rmt = 0.0;
   for (int cc = 0; cc < ns; cc++) {

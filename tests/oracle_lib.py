@@ -161,7 +161,7 @@ class OracleCase:
         """Colour order of the subdomain pc smoother (or_set_pc_order)."""
         L = lib()
         L.or_set_pc_order.argtypes = [C.c_void_p, C.c_int]
-        L.or_set_pc_order(C.cast(self.p, C.c_void_p), {"reference": 0, "colored": 1}[order])
+        L.or_set_pc_order(C.cast(self.p, C.c_void_p), {"reference": 0, "colored": 1, "mixed": 2}[order])
 
     def pc_perm(self, k):
         """Visiting order (1-based local cells) of subdomain k's pc smoother

@@ -406,3 +406,21 @@ def test_energy_routines_one_by_one():
     eq.solve_energy()
     oracle_lib.lib().or_solve_energy(oc.p)
     assert_fields_equal(eq, oc, ["t", "gt", "h", "gh", "ap", "anb", "b"])
+
+
+@pytest.mark.parametrize("n,jit,relax", [(8, 0.1, False), (12, 0.15, True), (32, 0.0, True)])
+def test_mixed_pc_order_bitwise(n, jit, relax):
+    """The mixed-precision colour order (an extension): FP32 colour sweeps
+    on the correction system, FP64 defect correction at every residual check.
+    The oracle restates the same single-precision operations in C, so fields
+    and solver lines match bit for bit."""
+    eq, oc = make_pair(n, jit, 4, mu=0.0025)
+    if relax:
+        eq.set_relaxation(0.7, 0.3)
+        oc.set_relaxation(0.7, 0.3)
+    eq.set_pc_order("mixed")
+    oc.set_pc_order("mixed")
+    eq.run(2, 3)
+    oc.run(2, 3)
+    assert_fields_equal(eq, oc, ["u", "v", "w", "p", "mip", "gp", "phic"])
+    assert_logs_match(eq.log(), oc.log())

@@ -531,3 +531,37 @@ def test_oracle_matches_reference_symmetry(n, jit, nsubd, sym):
     esec = np.asarray(g["esec"]).reshape(-1, 2)
     halo = np.concatenate([np.arange(esec[i][0], esec[i][1] + 1) - 1 for i in sec])
     assert np.abs(d["u"][halo]).max() + np.abs(d["v"][halo]).max() > 0
+
+
+@pytest.mark.parametrize("n,jit,relax", [(10, 0.15, False), (9, 0.0, True)])
+def test_mixed_pc_solve_keeps_fp64_accuracy(n, jit, relax):
+    """The mixed-precision colour order (an extension): FP32 sweeps on the
+    correction system ap*d - sum(anb*d_nb) = r, with the FP64 values updated
+    and the FP64 residual recomputed at every check.  In exact arithmetic the
+    sweeps of d from zero are the sweeps of pc itself, so the fields must
+    agree with the FP64 colour order to far better than single precision
+    (an FP32 iterate without the correction loop, a wrong sign or a missed
+    halo exchange of d each leave differences of 1e-7 or more), with the
+    same iteration counts, and must still differ in the last bits."""
+    g = meshgen.box_mesh(n, jitter=jit)
+    runs = {}
+    for mode in ("colored", "mixed"):
+        oc = oracle_lib.OracleCase(g, meshgen.cavity_bc(g), nsubd=4, rho=1.0, mu=0.0025, dt=0.01)
+        if relax:
+            oc.set_relaxation(0.7, 0.3)
+        oc.set_pc_order(mode)
+        oc.run(2, 3)
+        runs[mode] = oc
+    a, b = runs["colored"], runs["mixed"]
+    # velocity components against the velocity field's size (v and w of a
+    # symmetric cavity are close to zero), p and mip against their own
+    vel = np.linalg.norm(np.concatenate([a.arr(k) for k in ("u", "v", "w")]))
+    for k in ("u", "v", "w", "p", "mip"):
+        x, y = a.arr(k), b.arr(k)
+        scale = vel if k in ("u", "v", "w") else np.linalg.norm(x)
+        assert np.linalg.norm(x - y) <= 1e-8 * scale, k
+    assert not np.array_equal(a.arr("p"), b.arr("p"))
+    la, lb = a.log(), b.log()
+    assert [l[:2] for l in la] == [l[:2] for l in lb]
+    for x, y in zip(la, lb):
+        assert np.allclose(x[2:], y[2:], rtol=1e-6, atol=0), (x, y)

@@ -689,12 +689,23 @@ __global__ void k_gs_level(SysDesc s, const int *__restrict__ cells, int n, doub
 // The same update in FP32 on a correction system: cells of one colour (all
 // independent), row sum in CSR order, then (sum + (sor-1)*ap*phi)/ap/sor with
 // IEEE single-precision divisions.
-__global__ void k_gs_level32(SysDesc32 s, const int *__restrict__ cells, int n, float sor,
+// One colour is a contiguous cell range in each subdomain of the colour
+// numbering: cells beg[y] .. beg[y] + off[y+1] - off[y] - 1 are launch
+// positions off[y] .. off[y+1] - 1.  HEX: every row has six entries, so row
+// e starts at 6e.
+struct ColRanges {
+  int n;
+  int beg[kMaxSys], off[kMaxSys + 1];
+};
+template <bool HEX>
+__global__ void k_gs_level32(SysDesc32 s, ColRanges cr, float sor,
                              const int *__restrict__ active) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n || !active[0]) return;
-  const int e = cells[t];
-  const int r0 = s.row[e], r1 = s.row[e + 1];
+  if (t >= cr.off[cr.n] || !active[0]) return;
+  int y = 0;
+  while (t >= cr.off[y + 1]) y++;
+  const int e = cr.beg[y] + (t - cr.off[y]);
+  const int r0 = HEX ? 6 * e : s.row[e], r1 = HEX ? r0 + 6 : s.row[e + 1];
   const float a = s.ap[e];
   float sum = s.b[e];
   if (r1 - r0 == 6) {
@@ -1387,6 +1398,8 @@ struct cfdl_ctx {
   // colour order of the pc smoother (cfdl_set_pc_order, an extension): the
   // colours of a greedy colouring in each subdomain's local order, as levels
   Sched pcc;
+  std::vector<ColRanges> pcc_rng;  // per colour, its range in each subdomain
+  bool pc_hex = false;             // every pc row has six entries (row e at 6e)
   int pc_order = CFDL_PC_REFERENCE;
   // FP32 correction system of CFDL_PC_COLORED_MIXED (colour numbering):
   // ap, anb, residual and correction (cells and halos)
@@ -1567,20 +1580,23 @@ static int pc_sweep(cfdl_ctx *c, const Sched &s, const SysDesc &sys, bool forwar
 
 // An FP32 colour sweep of the mixed-precision correction system (sor 1.02 in
 // single precision), colours in order forward and in reverse backward.
-static int pc_sweep32(cfdl_ctx *c, const Sched &s, const SysDesc32 &sys, bool forward,
-                      const int *active) {
+static int pc_sweep32(cfdl_ctx *c, const SysDesc32 &sys, bool forward, const int *active) {
   cfdl_ctx::Sample *smp = nullptr;
   CALL(sample_begin(c, &smp));
-  for (int i = 0; i < s.nlev; i++) {
-    const int l = forward ? i : s.nlev - 1 - i;
-    const int n = s.ptr[l + 1] - s.ptr[l];
-    hipLaunchKernelGGL(k_gs_level32, dim3(nblocks(n)), dim3(kBlock), 0, c->st, sys,
-                       s.cells + s.ptr[l], n, 1.02f, active);
+  const int nl = static_cast<int>(c->pcc_rng.size());
+  for (int i = 0; i < nl; i++) {
+    const ColRanges &cr = c->pcc_rng[forward ? i : nl - 1 - i];
+    if (c->pc_hex)
+      hipLaunchKernelGGL(k_gs_level32<true>, dim3(nblocks(cr.off[cr.n])), dim3(kBlock), 0, c->st,
+                         sys, cr, 1.02f, active);
+    else
+      hipLaunchKernelGGL(k_gs_level32<false>, dim3(nblocks(cr.off[cr.n])), dim3(kBlock), 0,
+                         c->st, sys, cr, 1.02f, active);
   }
-  CALL(sample_end(c, smp, s.nlev, c->ne));
-  c->pc_launches += s.nlev;
+  CALL(sample_end(c, smp, nl, c->ne));
+  c->pc_launches += nl;
   c->pc_cells += c->ne;
-  c->gs_launches += s.nlev;
+  c->gs_launches += nl;
   HIPCHK(hipGetLastError());
   return CFDL_OK;
 }
@@ -1789,8 +1805,8 @@ static int multi_subdomain_solver(cfdl_ctx *c) {
   for (int blk = 1; blk <= nblock; blk++) {
     for (int rep = 0; rep < 2; rep++) {  // smoother_gs(..., 2): two symmetric iterations
       if (mixed) {
-        CALL(pc_sweep32(c, c->pcc, s32, true, act));
-        CALL(pc_sweep32(c, c->pcc, s32, false, act));
+        CALL(pc_sweep32(c, s32, true, act));
+        CALL(pc_sweep32(c, s32, false, act));
       } else if (c->pc_order == CFDL_PC_COLORED) {
         CALL(pc_sweep(c, c->pcc, s1, true, act));
         CALL(pc_sweep(c, c->pcc, s1, false, act));
@@ -2391,6 +2407,24 @@ int cfdl_ctx_create(int ne, int nf, int nbf, const int *ef2nb_idx, const int *ef
     CALL(build_tile_tables(C, rowp, nbrp, sdn, launchn, tidn, levn, 1, &C->ts));
     CALL(make_sched(C, levn, ne, &C->pcl));
     CALL(make_sched(C, coln, ne, &C->pcc));
+    // the same colours as ranges per subdomain (colour numbering is
+    // subdomain-major, colour-minor)
+    C->pcc_rng.assign(C->pcc.nlev, ColRanges{});
+    for (int l = 0; l < C->pcc.nlev; l++) {
+      ColRanges &cr = C->pcc_rng[l];
+      cr.n = nsubd;
+      cr.off[0] = 0;
+      for (int y = 0; y < nsubd; y++) {
+        int b = C->co[y + 1], e1 = C->co[y];
+        for (int i = C->co[y]; i < C->co[y + 1]; i++)
+          if (coln[i] == l + 1) b = std::min(b, i), e1 = i + 1;
+        cr.beg[y] = b;
+        cr.off[y + 1] = cr.off[y] + std::max(0, e1 - b);
+      }
+    }
+    C->pc_hex = true;
+    for (int i = 0; i < ne && C->pc_hex; i++) C->pc_hex = rowc[i] == 6 * i;
+    C->pc_hex = C->pc_hex && rowc[ne] == 6 * ne;
     C->Lp = static_cast<int>(nbrp.size());
     C->nh = nh;
     C->nx = static_cast<int>(xdst.size());

@@ -408,13 +408,17 @@ def test_energy_routines_one_by_one():
     assert_fields_equal(eq, oc, ["t", "gt", "h", "gh", "ap", "anb", "b"])
 
 
-@pytest.mark.parametrize("n,jit,relax", [(8, 0.1, False), (12, 0.15, True), (32, 0.0, True)])
-def test_mixed_pc_order_bitwise(n, jit, relax):
+@pytest.mark.parametrize("n,jit,relax,nsubd,nit", [(8, 0.1, False, 4, 100), (12, 0.15, True, 4, 100),
+                                                    (32, 0.0, True, 4, 100), (10, 0.1, False, 2, 25),
+                                                    (12, 0.0, True, 8, 7)])
+def test_mixed_pc_order_bitwise(n, jit, relax, nsubd, nit):
     """The mixed-precision colour order (an extension): FP32 colour sweeps
     on the correction system, FP64 defect correction at every residual check.
     The oracle restates the same single-precision operations in C, so fields
-    and solver lines match bit for bit."""
-    eq, oc = make_pair(n, jit, 4, mu=0.0025)
+    and solver lines match bit for bit.  An iteration cap that is not a
+    multiple of ten (25, 7) leaves a correction after the last check (none at
+    all with 7), which the final update adds."""
+    eq, oc = make_pair(n, jit, nsubd, mu=0.0025, nit=nit)
     if relax:
         eq.set_relaxation(0.7, 0.3)
         oc.set_relaxation(0.7, 0.3)

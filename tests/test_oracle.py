@@ -533,20 +533,26 @@ def test_oracle_matches_reference_symmetry(n, jit, nsubd, sym):
     assert np.abs(d["u"][halo]).max() + np.abs(d["v"][halo]).max() > 0
 
 
-@pytest.mark.parametrize("n,jit,relax", [(10, 0.15, False), (9, 0.0, True)])
-def test_mixed_pc_solve_keeps_fp64_accuracy(n, jit, relax):
+@pytest.mark.parametrize("n,jit,relax,nit", [(10, 0.15, False, 100), (9, 0.0, True, 100),
+                                             (8, 0.1, False, 25)])
+def test_mixed_pc_solve_keeps_fp64_accuracy(n, jit, relax, nit):
     """The mixed-precision colour order (an extension): FP32 sweeps on the
     correction system ap*d - sum(anb*d_nb) = r, with the FP64 values updated
     and the FP64 residual recomputed at every check.  In exact arithmetic the
     sweeps of d from zero are the sweeps of pc itself, so the fields must
-    agree with the FP64 colour order to far better than single precision
-    (an FP32 iterate without the correction loop, a wrong sign or a missed
-    halo exchange of d each leave differences of 1e-7 or more), with the
-    same iteration counts, and must still differ in the last bits."""
+    stay close to the FP64 colour order's, with the same iteration counts,
+    and still differ in the last bits.  The bounds are set between measured
+    values: a solve that ends on a check leaves p within 3e-10 (relative L2),
+    while a variant that never refreshes r (FP32 corrections accumulated over
+    the whole solve) is off by 1.5e-8 or more, and a missed final update or a
+    wrong correction by far more.  With a cap of 25 the last correction comes
+    after the last check and keeps FP32 accuracy (p 1.2e-8); there mip still
+    separates the two variants (5e-10 against 5e-9)."""
     g = meshgen.box_mesh(n, jitter=jit)
     runs = {}
     for mode in ("colored", "mixed"):
-        oc = oracle_lib.OracleCase(g, meshgen.cavity_bc(g), nsubd=4, rho=1.0, mu=0.0025, dt=0.01)
+        oc = oracle_lib.OracleCase(g, meshgen.cavity_bc(g), nsubd=4, rho=1.0, mu=0.0025, dt=0.01,
+                                   nit=nit)
         if relax:
             oc.set_relaxation(0.7, 0.3)
         oc.set_pc_order(mode)
@@ -556,10 +562,14 @@ def test_mixed_pc_solve_keeps_fp64_accuracy(n, jit, relax):
     # velocity components against the velocity field's size (v and w of a
     # symmetric cavity are close to zero), p and mip against their own
     vel = np.linalg.norm(np.concatenate([a.arr(k) for k in ("u", "v", "w")]))
+    if nit % 10 == 0:
+        tol = {"u": 1e-10, "v": 1e-10, "w": 1e-10, "p": 2e-9, "mip": 1e-9}
+    else:
+        tol = {"u": 1e-10, "v": 1e-10, "w": 1e-10, "p": 5e-8, "mip": 2e-9}
     for k in ("u", "v", "w", "p", "mip"):
         x, y = a.arr(k), b.arr(k)
         scale = vel if k in ("u", "v", "w") else np.linalg.norm(x)
-        assert np.linalg.norm(x - y) <= 1e-8 * scale, k
+        assert np.linalg.norm(x - y) <= tol[k] * scale, k
     assert not np.array_equal(a.arr("p"), b.arr("p"))
     la, lb = a.log(), b.log()
     assert [l[:2] for l in la] == [l[:2] for l in lb]

@@ -354,6 +354,144 @@ __global__ void k_coef_uvw(MeshDev m, FieldsDev f, double dt, double alpha_u) {
   f.dc[e] = vol / dc;
 }
 
+// The same assembly for meshes whose rows all have six entries (row e at 6e),
+// six threads per cell: thread j of a cell computes the terms of its row
+// entry j (the loop body above) into LDS, then the cell's first thread adds
+// them in CSR order exactly as the loop does (interior entries only for the
+// flux, secondary-stress and deferred-correction sums) and finishes the cell.
+// Interior anb are written by their entry's thread, boundary anb by the cell.
+constexpr int kFaceThreads = 256, kFaceCells = kFaceThreads / 6;
+__global__ void __launch_bounds__(kFaceThreads)
+k_coef_uvw6(MeshDev m, FieldsDev f, double dt, double alpha_u) {
+  __shared__ double st[9][kFaceThreads];  // d, fnb, fl, secondary stress, deferred correction
+  __shared__ int sin_[kFaceThreads];      // the entry is interior
+  const int lc = threadIdx.x / 6, j = threadIdx.x - 6 * lc;
+  const int e = blockIdx.x * kFaceCells + lc;
+  const bool valid = lc < kFaceCells && e < m.ne;
+  if (valid) {
+    const int k = 6 * e + j;
+    const int nb = m.nbr[k];
+    double fnb = 0.0, d = 0.0, fl = 0.0, ss[3] = {0.0, 0.0, 0.0}, dfc[3] = {0.0, 0.0, 0.0};
+    if (nb < m.ne) {
+      double rp[3];
+      centre(m, e, rp);
+      const int fs = m.fsg[k];
+      const double sg = fs >= 0 ? 1.0 : -1.0;
+      const int fg = abs(fs) - 1;
+      const double *a = m.aip + 3 * fg, *rip = m.rip + 3 * fg;
+      const double area = area_of(a);
+      double nrm[3], rpnb[3], dr[3], drip[3], rpp[3], rnp[3], drp[3];
+      for (int i = 0; i < 3; i++) nrm[i] = sg * a[i] / area;
+      centre(m, nb, rpnb);
+      for (int i = 0; i < 3; i++) dr[i] = rpnb[i] - rp[i];
+      const double ds = sqrt(dot3(dr, dr));
+      const double wt = vec_weight(rip, rp, rpnb);
+      for (int i = 0; i < 3; i++) drip[i] = rip[i] - rp[i];
+      double t = dot3(drip, nrm);
+      for (int i = 0; i < 3; i++) rpp[i] = rip[i] - t * nrm[i];
+      for (int i = 0; i < 3; i++) drip[i] = rip[i] - rpnb[i];
+      t = dot3(drip, nrm);
+      for (int i = 0; i < 3; i++) rnp[i] = rip[i] - t * nrm[i];
+      for (int i = 0; i < 3; i++) drp[i] = rnp[i] - rpp[i];
+      const double dsp = sqrt(dot3(drp, drp));
+      fl = -(sg * f.mip[fg]);
+      fnb = fmax(fl, 0.0);
+      const double muip = (1.0 - wt) * m.mu[e] + wt * m.mu[nb];
+      d = muip * area / ds;
+      const double ge[3][3] = {{f.gu[3 * e], f.gu[3 * e + 1], f.gu[3 * e + 2]},
+                               {f.gv[3 * e], f.gv[3 * e + 1], f.gv[3 * e + 2]},
+                               {f.gw[3 * e], f.gw[3 * e + 1], f.gw[3 * e + 2]}};
+      const double gn[3][3] = {{f.gu[3 * nb], f.gu[3 * nb + 1], f.gu[3 * nb + 2]},
+                               {f.gv[3 * nb], f.gv[3 * nb + 1], f.gv[3 * nb + 2]},
+                               {f.gw[3 * nb], f.gw[3 * nb + 1], f.gw[3 * nb + 2]}};
+      for (int mm = 0; mm < 3; mm++) {
+        double gip[3];
+        for (int q = 0; q < 3; q++) gip[q] = (1.0 - wt) * ge[q][mm] + wt * gn[q][mm];
+        ss[mm] = muip * area * dot3(gip, dr) / ds;
+      }
+      for (int q = 0; q < 3; q++) {
+        double gip[3];
+        for (int i = 0; i < 3; i++) gip[i] = (1.0 - wt) * ge[q][i] + wt * gn[q][i];
+        dfc[q] = muip * area * (dot3(gip, drp) / dsp - dot3(gip, dr) / ds);
+      }
+      f.anb[k] = d + fnb;
+    }
+    st[0][threadIdx.x] = d;
+    st[1][threadIdx.x] = fnb;
+    st[2][threadIdx.x] = fl;
+    for (int i = 0; i < 3; i++) {
+      st[3 + i][threadIdx.x] = ss[i];
+      st[6 + i][threadIdx.x] = dfc[i];
+    }
+    sin_[threadIdx.x] = nb < m.ne;
+  }
+  __syncthreads();
+  if (!valid || j != 0) return;
+  double rp[3];
+  centre(m, e, rp);
+  double ap = 0.0, sumf = 0.0, sumss[3] = {0.0, 0.0, 0.0}, sumdefc[3] = {0.0, 0.0, 0.0};
+  double anbk[6];
+  for (int q = 0; q < 6; q++) {
+    const int t = threadIdx.x + q;
+    const double d = st[0][t], fnb = st[1][t];
+    if (sin_[t]) {
+      sumf = sumf + st[2][t];
+      for (int i = 0; i < 3; i++) sumss[i] = sumss[i] + st[3 + i][t];
+      for (int i = 0; i < 3; i++) sumdefc[i] = sumdefc[i] + st[6 + i][t];
+    }
+    anbk[q] = d + fnb;
+    ap = ap + d + fnb;
+  }
+  const double vol = m.vol[e];
+  const double ap0 = m.rho[e] * vol / dt;
+  ap = ap + ap0;
+  double bu = ap0 * f.u0[e] + sumf * f.u[e] - vol * f.gp[3 * e] + sumss[0] + sumdefc[0];
+  double bv = ap0 * f.v0[e] + sumf * f.v[e] - vol * f.gp[3 * e + 1] + sumss[1] + sumdefc[1];
+  double bw = ap0 * f.w0[e] + sumf * f.w[e] - vol * f.gp[3 * e + 2] + sumss[2] + sumdefc[2];
+  for (int q = 0; q < 6; q++)
+    if (!sin_[threadIdx.x + q]) f.anb[6 * e + q] = anbk[q];
+  for (int q = m.bcrow[e]; q < m.bcrow[e + 1]; q++) {
+    const int k = m.bcslot[q], H = m.nbr[k];
+    const int fg = m.fsg[k] - 1;  // boundary faces are always owner side
+    const double *a = m.aip + 3 * fg;
+    const double area = area_of(a);
+    const double nrm[3] = {a[0] / area, a[1] / area, a[2] / area};
+    const double dr[3] = {m.xc[H] - rp[0], m.yc[H] - rp[1], m.zc[H] - rp[2]};
+    const double ds = sqrt(dot3(dr, dr));
+    const double fz = 0.0, d = m.mu[e] * area / ds;
+    if (m.hkind[H - m.ne] != CFDL_BC_SYMMETRY) {  // 'dirichlet', :255-265
+      const double ue = f.u[e], ve = f.v[e], we = f.w[e];
+      double vr[3] = {ue, ve, we};
+      const double t = dot3(vr, nrm);
+      for (int i = 0; i < 3; i++) vr[i] = vr[i] - t * nrm[i];
+      vr[0] = f.u[H] - vr[0];
+      vr[1] = f.v[H] - vr[1];
+      vr[2] = f.w[H] - vr[2];
+      bu = bu + d * vr[0] - d * ue;
+      bv = bv + d * vr[1] - d * ve;
+      bw = bw + d * vr[2] - d * we;
+    }
+    ap = ap + d + fz;
+    anbk[k - 6 * e] = anbk[k - 6 * e] + d + fz;
+    f.anb[k] = anbk[k - 6 * e];
+  }
+  if (alpha_u != 1.0) {
+    const double apr = ap / alpha_u, da = apr - ap;
+    bu = bu + da * f.u[e];
+    bv = bv + da * f.v[e];
+    bw = bw + da * f.w[e];
+    ap = apr;
+  }
+  f.ap[e] = ap;
+  f.bu[e] = bu;
+  f.bv[e] = bv;
+  f.bw[e] = bw;
+  f.d[e] = vol / ap;
+  double dc = ap;
+  for (int q = 0; q < 6; q++) dc = dc - anbk[q];
+  f.dc[e] = vol / dc;
+}
+
 // ---------------------------------------------------------------------------
 // Energy equation (src/equations/mod_energy.f90; never solved by the
 // reference's main loop, whose update_boundaries still applies its BCs).
@@ -568,6 +706,65 @@ __global__ void k_coef_p(MeshDev m, FieldsDev f) {
     const double d = 0.0;  // 'dirichlet' and 'zero_flux' both add nothing
     ap = ap + d;
     f.anb[k] = f.anb[k] + d;
+    b = b - f.mip[m.fsg[k] - 1];
+  }
+  f.ap[e] = ap;
+  f.b[e] = b;
+}
+
+// calc_coef_p for meshes whose rows all have six entries, six threads per
+// cell as in k_coef_uvw6: entry terms in parallel, then the cell's sums in
+// CSR order.
+__global__ void __launch_bounds__(kFaceThreads) k_coef_p6(MeshDev m, FieldsDev f) {
+  __shared__ double sd[kFaceThreads], sfl[kFaceThreads];
+  __shared__ int sin_[kFaceThreads];
+  const int lc = threadIdx.x / 6, j = threadIdx.x - 6 * lc;
+  const int e = blockIdx.x * kFaceCells + lc;
+  const bool valid = lc < kFaceCells && e < m.ne;
+  if (valid) {
+    const int k = 6 * e + j;
+    const int nb = m.nbr[k];
+    double d = 0.0, fl = 0.0;
+    if (nb < m.ne) {
+      double rp[3];
+      centre(m, e, rp);
+      const int fs = m.fsg[k];
+      const double sg = fs >= 0 ? 1.0 : -1.0;
+      const int fg = abs(fs) - 1;
+      const double *a = m.aip + 3 * fg, *rip = m.rip + 3 * fg;
+      const double area = area_of(a);
+      double nrm[3], rpnb[3], dr[3];
+      for (int i = 0; i < 3; i++) nrm[i] = sg * a[i] / area;
+      centre(m, nb, rpnb);
+      for (int i = 0; i < 3; i++) dr[i] = rpnb[i] - rp[i];
+      const double wt = vec_weight(rip, rp, rpnb);
+      fl = -(sg * f.mip[fg]);
+      const double rhoip = (1.0 - wt) * m.rho[e] + wt * m.rho[nb];
+      d = ((1.0 - wt) * f.dc[e] + wt * f.dc[nb]) / dot3(dr, nrm) * rhoip * area;
+      f.anb[k] = d;
+    }
+    sd[threadIdx.x] = d;
+    sfl[threadIdx.x] = fl;
+    sin_[threadIdx.x] = nb < m.ne;
+  }
+  __syncthreads();
+  if (!valid || j != 0) return;
+  double ap = 0.0, sumf = 0.0;
+  for (int q = 0; q < 6; q++) {
+    const int t = threadIdx.x + q;
+    if (sin_[t]) {
+      sumf = sumf + sfl[t];
+    } else {
+      f.anb[6 * e + q] = sd[t];
+    }
+    ap = ap + sd[t];
+  }
+  double b = sumf;
+  for (int q = m.bcrow[e]; q < m.bcrow[e + 1]; q++) {
+    const int k = m.bcslot[q];
+    const double d = 0.0;  // 'dirichlet' and 'zero_flux' both add nothing
+    ap = ap + d;
+    f.anb[k] = sd[threadIdx.x + (k - 6 * e)] + d;
     b = b - f.mip[m.fsg[k] - 1];
   }
   f.ap[e] = ap;
@@ -1426,6 +1623,7 @@ struct cfdl_ctx {
   int *l2g_m = nullptr, *rowm = nullptr, *nbrm = nullptr;
   double *ap_m = nullptr, *anb_m = nullptr, *rap_m = nullptr, *b_m[3] = {}, *phi_m[3] = {};
   bool tiled = true;
+  bool mhex = false;  // every row of the mesh CSR has six entries (row e at 6e)
   bool rcp = true;  // reciprocal-based division in the pipelined walk
   // control
   GsCtl *gctl = nullptr;
@@ -2188,6 +2386,8 @@ int cfdl_ctx_create(int ne, int nf, int nbf, const int *ef2nb_idx, const int *ef
   // thermal properties as init_properties sets them, mod_properties.f90:88-89
   CALL(up64(&d_cp, std::vector<double>(ne, 1000.0).data(), ne));
   CALL(up64(&d_tc, std::vector<double>(ne, 5.0).data(), ne));
+  C->mhex = row[ne] == 6 * ne;
+  for (int e = 0; e < ne && C->mhex; e++) C->mhex = row[e] == 6 * e;
   C->md = MeshDev{ne, nf, nbf, L, d_row, d_nbr, d_fsg, d_own, d_fcell, d_bcrow, d_bcslot,
                   d_hcell, d_hslot, d_hkind, d_xc, d_yc, d_zc, d_vol, d_aip, d_rip, d_rho, d_mu,
                   d_cp, d_tc};
@@ -2668,8 +2868,12 @@ int cfdl_solve_energy(cfdl_ctx *c) {
 
 int cfdl_calc_coef_uvw(cfdl_ctx *c) {
   CALL(check_ctx(c));
-  hipLaunchKernelGGL(k_coef_uvw, dim3(nblocks(c->ne)), dim3(kBlock), 0, c->st, c->md, c->fd, c->dt,
-                     c->alpha_u);
+  if (c->mhex)
+    hipLaunchKernelGGL(k_coef_uvw6, dim3((c->ne + kFaceCells - 1) / kFaceCells), dim3(kFaceThreads),
+                       0, c->st, c->md, c->fd, c->dt, c->alpha_u);
+  else
+    hipLaunchKernelGGL(k_coef_uvw, dim3(nblocks(c->ne)), dim3(kBlock), 0, c->st, c->md, c->fd,
+                       c->dt, c->alpha_u);
   HIPCHK(hipGetLastError());
   return CFDL_OK;
 }
@@ -2716,7 +2920,11 @@ int cfdl_calc_mip(cfdl_ctx *c, int rhie_chow) {
 
 int cfdl_calc_coef_p(cfdl_ctx *c) {
   CALL(check_ctx(c));
-  hipLaunchKernelGGL(k_coef_p, dim3(nblocks(c->ne)), dim3(kBlock), 0, c->st, c->md, c->fd);
+  if (c->mhex)
+    hipLaunchKernelGGL(k_coef_p6, dim3((c->ne + kFaceCells - 1) / kFaceCells), dim3(kFaceThreads),
+                       0, c->st, c->md, c->fd);
+  else
+    hipLaunchKernelGGL(k_coef_p, dim3(nblocks(c->ne)), dim3(kBlock), 0, c->st, c->md, c->fd);
   HIPCHK(hipGetLastError());
   return CFDL_OK;
 }

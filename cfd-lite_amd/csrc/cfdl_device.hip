@@ -2386,7 +2386,10 @@ int cfdl_ctx_create(int ne, int nf, int nbf, const int *ef2nb_idx, const int *ef
   // thermal properties as init_properties sets them, mod_properties.f90:88-89
   CALL(up64(&d_cp, std::vector<double>(ne, 1000.0).data(), ne));
   CALL(up64(&d_tc, std::vector<double>(ne, 5.0).data(), ne));
-  C->mhex = row[ne] == 6 * ne;
+  // CFDL_COEF_SERIAL=1 keeps the one-thread-per-cell assembly kernels, which
+  // also serve meshes with other row lengths
+  const char *cs = std::getenv("CFDL_COEF_SERIAL");
+  C->mhex = row[ne] == 6 * ne && !(cs && cs[0] == '1');
   for (int e = 0; e < ne && C->mhex; e++) C->mhex = row[e] == 6 * e;
   C->md = MeshDev{ne, nf, nbf, L, d_row, d_nbr, d_fsg, d_own, d_fcell, d_bcrow, d_bcslot,
                   d_hcell, d_hslot, d_hkind, d_xc, d_yc, d_zc, d_vol, d_aip, d_rip, d_rho, d_mu,

@@ -428,3 +428,17 @@ def test_mixed_pc_order_bitwise(n, jit, relax, nsubd, nit):
     oc.run(2, 3)
     assert_fields_equal(eq, oc, ["u", "v", "w", "p", "mip", "gp", "phic"])
     assert_logs_match(eq.log(), oc.log())
+
+
+def test_serial_assembly_kernels_bitwise(monkeypatch):
+    """The one-thread-per-cell assembly kernels (k_coef_uvw, k_coef_p), which
+    meshes with other row lengths would use, selected with CFDL_COEF_SERIAL=1
+    when the context is created: fields and solver lines match the oracle."""
+    monkeypatch.setenv("CFDL_COEF_SERIAL", "1")
+    eq, oc = make_pair(9, 0.15, 4)
+    eq.set_relaxation(0.7, 0.3)
+    oc.set_relaxation(0.7, 0.3)
+    eq.run(2, 3)
+    oc.run(2, 3)
+    assert_fields_equal(eq, oc)
+    assert_logs_match(eq.log(), oc.log())

@@ -9,7 +9,11 @@
  * Each expression keeps the reference's evaluation order (Fortran evaluates
  * a*b/c as (a*b)/c, a-b-c as (a-b)-c); compile without FMA contraction.
  * Parity: pinned bitwise against the reference compiled from its own
- * sources (oracle/_ref) and against closed forms in the CPU tests.
+ * sources (oracle/_ref) and against closed forms in the CPU tests.  The
+ * extension modes (config-mode relaxation, the colour order and the
+ * mixed-precision pc solve) have no reference counterpart: they are pinned
+ * by their reduction to the reference (relaxation 1), by the colouring rule,
+ * and by the mixed mode's distance from the FP64 colour order.
  */
 #include "oracle.h"
 

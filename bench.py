@@ -208,6 +208,9 @@ def main():
                          "own line, not as the headline value; 'mixed' also runs the colour "
                          "order with FP32 sweeps under FP64 defect correction and reports its "
                          "field error against the FP64 colour order")
+    ap.add_argument("--gs-order", choices=("reference", "colored"), default="reference",
+                    help="visiting order of solve_gs (the momentum solves) in the --pc-order "
+                         "extension lines; the headline value always keeps the reference order")
     ap.add_argument("--pcie", action="store_true",
                     help="also time outer iterations with the host transfers of the drop-in "
                          "Fortran solve_uvwp (state up, state and work arrays down)")
@@ -324,6 +327,7 @@ def colored_rate(cfdl, mesh, mu, ne, args, order, fp64=None):
     eq = cfdl.UVWP.cavity(mesh, rho=1.0, mu=mu, dt=0.01, nit=100)
     eq.set_relaxation(args.alpha_u, args.alpha_p)
     eq.set_pc_order(order)
+    eq.set_gs_order(args.gs_order)
     n = 0
     for k in range(args.warmup + args.steps):
         if k == args.warmup:
@@ -337,7 +341,8 @@ def colored_rate(cfdl, mesh, mu, ne, args, order, fp64=None):
     eq.synchronize()
     secs = time.perf_counter() - t
     fields = {k: eq.get(k)[:ne] for k in ("u", "v", "w", "p")}
-    line = {"mode": "pc-" + order, "metric": METRIC, "value": round(ne * args.steps / secs, 1),
+    mode = "pc-" + order + ("/gs-colored" if args.gs_order == "colored" else "")
+    line = {"mode": mode, "metric": METRIC, "value": round(ne * args.steps / secs, 1),
             "unit": "cell-iterations/s", "ms_per_step": round(1e3 * secs / args.steps, 3),
             "steps": args.steps, "outer_iterations_from_rest": args.warmup + args.steps,
             "dtype": "f32 pc sweeps, f64 otherwise" if order == "mixed" else "f64",

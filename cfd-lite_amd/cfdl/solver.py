@@ -95,6 +95,17 @@ class UVWP:
             raise CfdlError("pc order must be one of %s" % sorted(self.PC_ORDERS))
         check(self._lib.cfdl_set_pc_order(self._h, self.PC_ORDERS[order]))
 
+    GS_ORDERS = {"reference": 0, "colored": 1}
+
+    def set_gs_order(self, order="reference"):
+        """Visiting order of solve_gs (the momentum and energy solves, and the
+        pc solve with one subdomain): "reference" (the reference's global
+        order, the default) or "colored" (greedy colouring of the global
+        system, swept colour by colour; an extension)."""
+        if order not in self.GS_ORDERS:
+            raise CfdlError("gs order must be one of %s" % sorted(self.GS_ORDERS))
+        check(self._lib.cfdl_set_gs_order(self._h, self.GS_ORDERS[order]))
+
     BC_KINDS = {"wall": 0, "lid": 1, "symmetry": 2}
 
     def set_bc_kind(self, first, last, kind):

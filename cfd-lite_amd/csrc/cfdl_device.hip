@@ -841,12 +841,13 @@ __global__ void k_mass_imbalance(MeshDev m, const double *__restrict__ mip, doub
 // sharing ap/anb (smoother_gs / solve_gs inner loop, mod_solver.f90:205-224,
 // 287-307).  phi(e) = (sumnb + (sor-1)*ap*phi(e))/ap/sor.
 // ---------------------------------------------------------------------------
+// cells == nullptr: the cells are base .. base + n - 1 (a colour range).
 template <int NC>
-__global__ void k_gs_level(SysDesc s, const int *__restrict__ cells, int n, double sor,
+__global__ void k_gs_level(SysDesc s, const int *__restrict__ cells, int base, int n, double sor,
                            const int *__restrict__ active) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  const int e = cells[t];
+  const int e = cells ? cells[t] : base + t;
   const int r0 = s.row[e], r1 = s.row[e + 1];
   const double a = s.ap[e];
   if (r1 - r0 == 6) {
@@ -1623,6 +1624,11 @@ struct cfdl_ctx {
   int *l2g_m = nullptr, *rowm = nullptr, *nbrm = nullptr;
   double *ap_m = nullptr, *anb_m = nullptr, *rap_m = nullptr, *b_m[3] = {}, *phi_m[3] = {};
   bool tiled = true;
+  // colour order of solve_gs (cfdl_set_gs_order, an extension): the global
+  // system numbered by (colour, global order), each colour one range
+  int gs_order = CFDL_GS_REFERENCE;
+  int *l2g_gc = nullptr, *rowgc = nullptr, *nbrgc = nullptr;
+  std::vector<int> gcol_ptr;
   bool mhex = false;  // every row of the mesh CSR has six entries (row e at 6e)
   bool rcp = true;  // reciprocal-based division in the pipelined walk
   // control
@@ -1716,10 +1722,10 @@ static int sweep(cfdl_ctx *c, const Sched &s, const SysDesc &sys, int nc, double
     const int n = s.ptr[l + 1] - s.ptr[l];
     if (nc == 3)
       hipLaunchKernelGGL(k_gs_level<3>, dim3(nblocks(n)), dim3(kBlock), 0, c->st, sys,
-                         s.cells + s.ptr[l], n, sor, active);
+                         s.cells + s.ptr[l], 0, n, sor, active);
     else
       hipLaunchKernelGGL(k_gs_level<1>, dim3(nblocks(n)), dim3(kBlock), 0, c->st, sys,
-                         s.cells + s.ptr[l], n, sor, active);
+                         s.cells + s.ptr[l], 0, n, sor, active);
   }
   c->gs_launches += s.nlev;
   HIPCHK(hipGetLastError());
@@ -1766,7 +1772,7 @@ static int pc_sweep(cfdl_ctx *c, const Sched &s, const SysDesc &sys, bool forwar
     const int l = forward ? i : s.nlev - 1 - i;
     const int n = s.ptr[l + 1] - s.ptr[l];
     hipLaunchKernelGGL(k_gs_level<1>, dim3(nblocks(n)), dim3(kBlock), 0, c->st, sys,
-                       s.cells + s.ptr[l], n, 1.02, active);
+                       s.cells + s.ptr[l], 0, n, 1.02, active);
   }
   CALL(sample_end(c, smp, s.nlev, c->ne));
   c->pc_launches += s.nlev;
@@ -1879,15 +1885,18 @@ static int solve_gs_n(cfdl_ctx *c, int ny, double **phi, const double **b,
   Names nm{};
   for (int y = 0; y < ny; y++) std::strncpy(nm.n[y], names[y], 7);
   // tiled: the system and its right-hand sides in tile numbering (one or
-  // three right-hand sides; solve_gs_uvw passes three)
-  const bool tiled = c->mtiled && (ny == 1 || ny == 3);
+  // three right-hand sides; solve_gs_uvw passes three); colour order: in the
+  // colour numbering of the same system, through the same copies
+  const bool gcol = c->gs_order == CFDL_GS_COLORED && (ny == 1 || ny == 3);
+  const bool tiled = !gcol && c->mtiled && (ny == 1 || ny == 3);
+  const bool mapped = tiled || gcol;
   TileMap tmap{};
-  if (tiled) {
+  if (mapped) {
     tmap.ne = c->ne;
     tmap.nbf = c->nbf;
     tmap.ny = ny;
-    tmap.l2g = c->l2g_m;
-    tmap.rowm = c->rowm;
+    tmap.l2g = gcol ? c->l2g_gc : c->l2g_m;
+    tmap.rowm = gcol ? c->rowgc : c->rowm;
     tmap.rowg = c->md.row;
     tmap.ap = c->fd.ap;
     tmap.anb = c->fd.anb;
@@ -1908,20 +1917,36 @@ static int solve_gs_n(cfdl_ctx *c, int ny, double **phi, const double **b,
   for (int y = 0; y < ny; y++) {
     s.beg[y] = 0;
     s.end[y] = c->ne;
-    s.phi[y] = tiled ? c->phi_m[y] : phi[y];
-    s.b[y] = tiled ? c->b_m[y] : b[y];
+    s.phi[y] = mapped ? c->phi_m[y] : phi[y];
+    s.b[y] = mapped ? c->b_m[y] : b[y];
   }
-  s.row = tiled ? c->rowm : c->md.row;
-  s.nbr = tiled ? c->nbrm : c->md.nbr;
-  s.ap = tiled ? c->ap_m : c->fd.ap;
-  s.anb = tiled ? c->anb_m : c->fd.anb;
-  s.rap = tiled ? c->rap_m : nullptr;
+  s.row = gcol ? c->rowgc : tiled ? c->rowm : c->md.row;
+  s.nbr = gcol ? c->nbrgc : tiled ? c->nbrm : c->md.nbr;
+  s.ap = mapped ? c->ap_m : c->fd.ap;
+  s.anb = mapped ? c->anb_m : c->fd.anb;
+  s.rap = mapped ? c->rap_m : nullptr;
   CALL(residual(c, s, c->nblk_g, c->chunk_g, 1));
   hipLaunchKernelGGL(k_gs_init, dim3(1), dim3(64), 0, c->st, c->gctl, c->red, ny,
                      static_cast<double>(c->ne), c->nit);
   // one SysDesc per sweep kernel with its RHS subset in slots 0..ny-1
   for (int it = 0; it < c->nit; it++) {
-    if (tiled) {
+    if (gcol) {
+      // colours in order, then in reverse; one launch per colour
+      const int nq = static_cast<int>(c->gcol_ptr.size()) - 1;
+      for (int d = 0; d < 2; d++)
+        for (int i = 0; i < nq; i++) {
+          const int q = d == 0 ? i : nq - 1 - i;
+          const int b0 = c->gcol_ptr[q], n = c->gcol_ptr[q + 1] - b0;
+          if (ny == 3)
+            hipLaunchKernelGGL(k_gs_level<3>, dim3(nblocks(n)), dim3(kBlock), 0, c->st, s,
+                               static_cast<const int *>(nullptr), b0, n, sor, gs_active(c));
+          else
+            hipLaunchKernelGGL(k_gs_level<1>, dim3(nblocks(n)), dim3(kBlock), 0, c->st, s,
+                               static_cast<const int *>(nullptr), b0, n, sor, gs_active(c));
+        }
+      c->gs_launches += 2 * nq;
+      HIPCHK(hipGetLastError());
+    } else if (tiled) {
       const int nl = static_cast<int>(c->tm.launch_ptr.size()) - 1;
       if (ny == 3) {
         tiled_launches<3>(c, c->tm, s, true, sor, gs_active(c));
@@ -1946,7 +1971,7 @@ static int solve_gs_n(cfdl_ctx *c, int ny, double **phi, const double **b,
     for (int y = 0; y < ny; y++) any = any || c->host_flags[y];
     if (!any) break;
   }
-  if (tiled)
+  if (mapped)
     hipLaunchKernelGGL(k_tile_scatter, dim3(nblocks(c->ne)), dim3(kBlock), 0, c->st, tmap);
   hipLaunchKernelGGL(k_log_gs, dim3(1), dim3(64), 0, c->st, c->gctl, ny, nm, c->dlog, c->dlog_n);
   HIPCHK(hipGetLastError());
@@ -2481,6 +2506,40 @@ int cfdl_ctx_create(int ne, int nf, int nbf, const int *ef2nb_idx, const int *ef
     }
     levels(row, nbr, 0, ne, false, lev);
     CALL(make_sched(C, lev, ne, &C->gb));
+    // colour order of solve_gs: greedy colouring in global order (a cell
+    // takes the least colour none of its earlier interior neighbours has),
+    // cells renumbered by (colour, global order)
+    {
+      std::vector<int> col(ne, 0);
+      int ncol = 1;
+      for (int e = 0; e < ne; e++) {
+        uint64_t used = 0;
+        for (int k = row[e]; k < row[e + 1]; k++) {
+          const int d = nbr[k];
+          if (d < e && col[d] < 64) used |= uint64_t{1} << col[d];
+        }
+        int q = 0;
+        while (q < 64 && ((used >> q) & 1)) q++;
+        col[e] = q;
+        ncol = std::max(ncol, q + 1);
+      }
+      std::vector<int> n2o(ne), o2n(ne), rowc(ne + 1, 0), nbrc;
+      for (int e = 0; e < ne; e++) n2o[e] = e;
+      std::stable_sort(n2o.begin(), n2o.end(), [&](int x, int y) { return col[x] < col[y]; });
+      for (int i = 0; i < ne; i++) o2n[n2o[i]] = i;
+      C->gcol_ptr.assign(ncol + 1, 0);
+      for (int e = 0; e < ne; e++) C->gcol_ptr[col[e] + 1]++;
+      for (int q = 0; q < ncol; q++) C->gcol_ptr[q + 1] += C->gcol_ptr[q];
+      nbrc.reserve(L);
+      for (int i = 0; i < ne; i++) {
+        const int g = n2o[i];
+        rowc[i + 1] = rowc[i] + (row[g + 1] - row[g]);
+        for (int k = row[g]; k < row[g + 1]; k++) nbrc.push_back(nbr[k] < ne ? o2n[nbr[k]] : nbr[k]);
+      }
+      CALL(upload(C, &C->l2g_gc, n2o));
+      CALL(upload(C, &C->rowgc, rowc));
+      CALL(upload(C, &C->nbrgc, nbrc));
+    }
   }
   // --- subdomain systems (construct_subdomains, mod_subdomains.f90:18-160):
   // local cells in g2gf order, CSR rows in the global row order, halo slots
@@ -2774,6 +2833,25 @@ int cfdl_set_pc_order(cfdl_ctx *c, int order) {
     CALL(dalloc(c, &c->d32, static_cast<int64_t>(c->ne) + c->nh));
   }
   c->pc_order = order;
+  return CFDL_OK;
+}
+
+int cfdl_set_gs_order(cfdl_ctx *c, int order) {
+  CALL(check_ctx(c));
+  if (order != CFDL_GS_REFERENCE && order != CFDL_GS_COLORED) {
+    set_error("cfdl_set_gs_order: order must be CFDL_GS_REFERENCE or CFDL_GS_COLORED");
+    return CFDL_EINVAL;
+  }
+  if (order == CFDL_GS_COLORED && !c->ap_m) {  // the copies the tiled path would own
+    CALL(dalloc(c, &c->ap_m, c->ne));
+    CALL(dalloc(c, &c->rap_m, c->ne));
+    CALL(dalloc(c, &c->anb_m, c->L));
+    for (int y = 0; y < 3; y++) {
+      CALL(dalloc(c, &c->b_m[y], c->ne));
+      CALL(dalloc(c, &c->phi_m[y], int64_t{c->ne} + c->nbf));
+    }
+  }
+  c->gs_order = order;
   return CFDL_OK;
 }
 

@@ -12,6 +12,7 @@ module cfdl_c
   integer(c_int), parameter :: CFDL_BC_WALL = 0, CFDL_BC_LID = 1, CFDL_BC_SYMMETRY = 2
   integer(c_int), parameter :: CFDL_PC_REFERENCE = 0, CFDL_PC_COLORED = 1, &
                                 CFDL_PC_COLORED_MIXED = 2
+  integer(c_int), parameter :: CFDL_GS_REFERENCE = 0, CFDL_GS_COLORED = 1
   integer(c_int), parameter :: CFDL_U = 0, CFDL_V = 1, CFDL_W = 2, CFDL_P = 3, &
        CFDL_U0 = 4, CFDL_V0 = 5, CFDL_W0 = 6, CFDL_GU = 7, CFDL_GV = 8, CFDL_GW = 9, &
        CFDL_GP = 10, CFDL_GPC = 11, CFDL_MIP = 12, CFDL_MIP0 = 13, CFDL_BU = 14, &
@@ -125,6 +126,12 @@ module cfdl_c
       type(c_ptr), value :: c
       integer(c_int), value :: order
       integer(c_int) :: cfdl_set_pc_order
+    end function
+    function cfdl_set_gs_order(c, order) bind(C, name="cfdl_set_gs_order")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: c
+      integer(c_int), value :: order
+      integer(c_int) :: cfdl_set_gs_order
     end function
     function cfdl_set_properties(c, rho, mu) bind(C, name="cfdl_set_properties")
       import :: c_int, c_double, c_ptr

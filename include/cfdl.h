@@ -170,6 +170,17 @@ int cfdl_set_relaxation(cfdl_ctx *c, double alpha_u, double alpha_p);
 enum { CFDL_PC_REFERENCE = 0, CFDL_PC_COLORED = 1, CFDL_PC_COLORED_MIXED = 2 };
 int cfdl_set_pc_order(cfdl_ctx *c, int order);
 
+/* Visiting order of solve_gs (mod_solver.f90:255-327: the momentum solves,
+   the energy solve, and the pc solve with one subdomain).
+   CFDL_GS_REFERENCE (the default) is the reference's global order, bit for
+   bit.  CFDL_GS_COLORED has no counterpart in the reference: the global
+   system is coloured greedily in global order (a cell takes the least colour
+   none of its earlier interior neighbours has) and each sweep visits the
+   cells colour by colour, forward, and in reverse, backward -- the same
+   update and stop rule in another sequential order, one launch per colour. */
+enum { CFDL_GS_REFERENCE = 0, CFDL_GS_COLORED = 1 };
+int cfdl_set_gs_order(cfdl_ctx *c, int order);
+
 /* Host-side state a Fortran drop-in keeps authoritative (properties_t and
    phys_t%dt, mod_properties.f90:9-51, mod_physics.f90:15; the BC type the
    setters assign, mod_uvwp.f90:500,525,555: 'dirichlet' -> CFDL_BC_WALL,

@@ -93,6 +93,10 @@ typedef struct or_case {
      defect correction at every residual check (an extension as well) */
   int pc_order;
   int **pc_perm; /* per subdomain: its ne cells (1-based) in visiting order */
+  /* visiting order of solve_gs: 0 is the reference's (global cell order); 1
+     visits the cells colour by colour (or_set_gs_order), an extension */
+  int gs_order;
+  int *gs_perm; /* ne cells (1-based) in visiting order */
 } or_case;
 
 /* ---- setup ---- */
@@ -131,6 +135,7 @@ void or_solve(or_case *c, const char *name, double *phi);
    the cells by colour, then by local index; order 0 restores the
    reference's order. */
 void or_set_pc_order(or_case *c, int order);
+void or_set_gs_order(or_case *c, int order);
 
 /* ---- hot path (equation routines mirror mod_uvwp.f90) ---- */
 void or_update_boundaries(or_case *c);

@@ -690,6 +690,7 @@ void or_case_free(or_case *c) {
                   c->cp, c->tc, c->t, c->gt, c->h, c->h0, c->gh};
   for (size_t k = 0; k < sizeof(ptrs) / sizeof(ptrs[0]); k++) free(ptrs[k]);
   or_set_pc_order(c, 0);
+  or_set_gs_order(c, 0);
   if (c->sd) {
     for (int k = 0; k < c->nsubd; k++) {
       or_subd *s = &c->sd[k];

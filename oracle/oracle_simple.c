@@ -152,11 +152,43 @@ void or_calc_residual(const double *phi, const double *ap, const double *anb,
   *res_max = rm;
 }
 
-/* solve_gs, src/modules/mod_solver.f90:255-327 */
+/* Colour order of solve_gs (an extension, no counterpart in the reference):
+   a greedy colouring of the global system in global order, a cell taking the
+   least colour none of its earlier interior neighbours has; gs_perm lists
+   the cells by (colour, global index). */
+void or_set_gs_order(or_case *c, int order) {
+  free(c->gs_perm);
+  c->gs_perm = NULL;
+  c->gs_order = order;
+  if (order != 1) return;
+  int ne = c->ne, ncol = 0;
+  int *col = calloc((size_t)ne, sizeof(int));
+  for (int e = 1; e <= ne; e++) {
+    unsigned long long used = 0;
+    for (int idx = c->ef2nb_idx[e - 1]; idx <= c->ef2nb_idx[e] - 1; idx++) {
+      int d = PK_E(c->ef2nb[idx - 1]);
+      if (d <= ne && d < e && col[d - 1] < 64) used |= 1ull << col[d - 1];
+    }
+    int k = 0;
+    while (k < 64 && (used >> k & 1ull)) k++;
+    col[e - 1] = k;
+    if (k + 1 > ncol) ncol = k + 1;
+  }
+  c->gs_perm = malloc((size_t)ne * sizeof(int));
+  int n = 0;
+  for (int k = 0; k < ncol; k++)
+    for (int e = 1; e <= ne; e++)
+      if (col[e - 1] == k) c->gs_perm[n++] = e;
+  free(col);
+}
+
+/* solve_gs, src/modules/mod_solver.f90:255-327 (with gs_order 1 the sweeps
+   visit the cells in gs_perm order forward and reversed backward) */
 void or_solve_gs(or_case *c, const char *name, double *phi, const double *ap,
                  const double *anb, const double *b, const int *ef2nb_idx,
                  const int *ef2nb, int ne, int nf, int nbf, int nit) {
   (void)nf; (void)nbf;
+  const int *perm = (c && c->gs_order == 1 && ne == c->ne) ? c->gs_perm : NULL;
   double sor = sor_of(name);
   double res_i = 0.0;
   for (int e = 1; e <= ne; e++) {
@@ -168,11 +200,13 @@ void or_solve_gs(or_case *c, const char *name, double *phi, const double *ap,
   double res_f = res_i, res_max = 0.0, target = res_i / 10.0;
   while (it < nit && res_f > target) {
     it++;
-    for (int e = 1; e <= ne; e++) {
+    for (int k = 1; k <= ne; k++) {
+      int e = perm ? perm[k - 1] : k;
       double s = gs_sum(phi, anb, b, ef2nb_idx, ef2nb, e);
       phi[e - 1] = (s + (sor - 1.0) * ap[e - 1] * phi[e - 1]) / ap[e - 1] / sor;
     }
-    for (int e = ne; e >= 1; e--) {
+    for (int k = ne; k >= 1; k--) {
+      int e = perm ? perm[k - 1] : k;
       double s = gs_sum(phi, anb, b, ef2nb_idx, ef2nb, e);
       phi[e - 1] = (s + (sor - 1.0) * ap[e - 1] * phi[e - 1]) / ap[e - 1] / sor;
     }

@@ -49,7 +49,8 @@ class OrCase(C.Structure):
                 ("ap", PD), ("anb", PD), ("b", PD), ("phic", PD),
                 ("nlog", C.c_int), ("caplog", C.c_int), ("log", C.POINTER(OrLog)),
                 ("cp", PD), ("tc", PD), ("t", PD), ("gt", PD), ("h", PD), ("h0", PD), ("gh", PD),
-                ("pc_order", C.c_int), ("pc_perm", C.POINTER(PI))]
+                ("pc_order", C.c_int), ("pc_perm", C.POINTER(PI)),
+                ("gs_order", C.c_int), ("gs_perm", PI)]
 
 
 _lib = None
@@ -162,6 +163,16 @@ class OracleCase:
         L = lib()
         L.or_set_pc_order.argtypes = [C.c_void_p, C.c_int]
         L.or_set_pc_order(C.cast(self.p, C.c_void_p), {"reference": 0, "colored": 1, "mixed": 2}[order])
+
+    def set_gs_order(self, order="reference"):
+        """Colour order of solve_gs (or_set_gs_order)."""
+        L = lib()
+        L.or_set_gs_order.argtypes = [C.c_void_p, C.c_int]
+        L.or_set_gs_order(C.cast(self.p, C.c_void_p), {"reference": 0, "colored": 1}[order])
+
+    def gs_perm(self):
+        """Visiting order (1-based cells) of solve_gs under the colour order."""
+        return np.ctypeslib.as_array(self.c.gs_perm, shape=(self.c.ne,)).copy()
 
     def pc_perm(self, k):
         """Visiting order (1-based local cells) of subdomain k's pc smoother

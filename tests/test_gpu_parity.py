@@ -442,3 +442,23 @@ def test_serial_assembly_kernels_bitwise(monkeypatch):
     oc.run(2, 3)
     assert_fields_equal(eq, oc)
     assert_logs_match(eq.log(), oc.log())
+
+
+@pytest.mark.parametrize("n,jit,nsubd,pc", [(8, 0.1, 4, "reference"), (12, 0.15, 4, "mixed"),
+                                            (10, 0.1, 1, "reference"), (32, 0.0, 4, "colored")])
+def test_colored_gs_order_bitwise(n, jit, nsubd, pc):
+    """The colour order of solve_gs (an extension): one launch per colour of
+    the global system for the momentum solves (and, with one subdomain, the
+    pc solve), against the oracle's sequential solve_gs in the same order;
+    alone and with the pc smoother's orders.  Fields and solver lines match."""
+    eq, oc = make_pair(n, jit, nsubd, mu=0.0025)
+    eq.set_relaxation(0.7, 0.3)
+    oc.set_relaxation(0.7, 0.3)
+    eq.set_gs_order("colored")
+    oc.set_gs_order("colored")
+    eq.set_pc_order(pc)
+    oc.set_pc_order(pc)
+    eq.run(2, 3)
+    oc.run(2, 3)
+    assert_fields_equal(eq, oc, ["u", "v", "w", "p", "mip", "gp", "phic"])
+    assert_logs_match(eq.log(), oc.log())

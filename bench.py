@@ -211,6 +211,13 @@ def main():
     ap.add_argument("--gs-order", choices=("reference", "colored"), default="reference",
                     help="visiting order of solve_gs (the momentum solves) in the --pc-order "
                          "extension lines; the headline value always keeps the reference order")
+    ap.add_argument("--serving", action="store_true",
+                    help="also run a serving stream (Poisson arrivals of independent cavity "
+                         "solves of edge 16..96 and Re 100..1000, each the reference's 10 x 3 "
+                         "outer iterations) and report solves/s with p50/p99 latency")
+    ap.add_argument("--serving-requests", type=int, default=512)
+    ap.add_argument("--serving-rate", type=float, default=10.0, help="arrivals per second")
+    ap.add_argument("--serving-slots", type=int, default=4, help="requests in flight")
     ap.add_argument("--pcie", action="store_true",
                     help="also time outer iterations with the host transfers of the drop-in "
                          "Fortran solve_uvwp (state up, state and work arrays down)")
@@ -311,10 +318,34 @@ def main():
           "outer iteration %.1f ms" % (st["pc_launches"], st["pc_cells"] / max(st["pc_launches"], 1),
                                        st["pc_levels_per_sym_sweep"], 1e3 * secs / args.steps),
           file=sys.stderr)
+    if args.serving:
+        serving_line(args)
     if args.pc_order in ("colored", "mixed"):
         ref = colored_rate(cfdl, mesh, mu, ne, args, "colored")
     if args.pc_order == "mixed":
         colored_rate(cfdl, mesh, mu, ne, args, "mixed", ref)
+
+
+def serving_line(args):
+    """The serving stream (an extension; the reference solves one case per
+    run): seeded Poisson arrivals, each request a cavity solved from rest on
+    a context kept per edge, in the fastest mode (mixed pc solve, colour
+    order of solve_gs).  Printed as its own JSON line."""
+    from cfdl import serving
+    reqs = serving.make_requests(args.serving_requests, args.serving_rate, 20261202)
+    print("bench: serving %d requests" % len(reqs), file=sys.stderr, flush=True)
+    out = serving.serve(reqs, slots=args.serving_slots, pc_order="mixed", gs_order="colored")
+    cells = sum(n ** 3 for _, n, _ in reqs)
+    print(json.dumps({"mode": "serving", "metric": "solves/s with p50/p99 latency",
+                      "solves_per_s": round(out["solves_per_s"], 3),
+                      "p50_ms": round(1e3 * out["p50_s"], 1), "p99_ms": round(1e3 * out["p99_s"], 1),
+                      "wall_s": round(out["wall_s"], 2), "requests": len(reqs),
+                      "arrival_rate_per_s": args.serving_rate, "slots": args.serving_slots,
+                      "cell_iterations_per_s": round(30 * cells / out["wall_s"], 1),
+                      "case": "lid-driven cavity N^3, N in {16,...,96}, Re log-uniform in "
+                              "[100, 1000], 10 x 3 outer iterations from rest, 4 subdomains",
+                      "pc_order": "mixed", "gs_order": "colored",
+                      "dtype": "f32 pc sweeps, f64 otherwise", "data": "synthetic"}), flush=True)
 
 
 def colored_rate(cfdl, mesh, mu, ne, args, order, fp64=None):

@@ -1,0 +1,141 @@
+"""Serving: a stream of independent lid-driven cavity solves on one GPU.
+
+Each request is a cavity of edge N (N^3 hex cells, 4 subdomains) at Reynolds
+number Re, solved by the reference's loop (src/main.f90:50-63: ntstep pseudo
+time steps of ncoef outer iterations, from rest).  Requests are admitted in
+arrival order when a slot is free and their device footprint fits the HBM
+budget; each admitted request runs on its own context, whose HIP stream runs
+concurrently with the other slots'.  Contexts are kept per N and reset
+between requests (every field cleared, the viscosity set from Re), so a case
+gives the same bits whether it runs alone or in any mix.
+
+There is no counterpart in the reference, which solves one case per run.
+"""
+import concurrent.futures as cf
+import threading
+import time
+
+import numpy as np
+
+from .mesh import Mesh
+from .solver import FIELDS, UVWP
+
+# the fields of solve_uvwp's state and work arrays (the energy fields are not
+# used by these solves)
+STATE_FIELDS = [f for f in FIELDS if f not in ("t", "gt", "h", "h0", "gh")]
+# device bytes per cell of one context: fields, mesh, the pc system in two
+# numberings, the sweep schedules (a bound from the allocations, not measured)
+BYTES_PER_CELL = 3072
+
+
+class ContextPool:
+    """Contexts cached per box edge, each with its mesh."""
+
+    def __init__(self, pc_order="reference", gs_order="reference", nsubd=4):
+        self.pc_order, self.gs_order, self.nsubd = pc_order, gs_order, nsubd
+        self._free = {}
+        self._all = []
+        self._lock = threading.Lock()
+
+    def acquire(self, n):
+        with self._lock:
+            if self._free.get(n):
+                return self._free[n].pop()
+        mesh = Mesh.box(n).partition(self.nsubd)
+        eq = UVWP.cavity(mesh, rho=1.0, mu=0.01, dt=0.01, nit=100)
+        eq.set_pc_order(self.pc_order)
+        eq.set_gs_order(self.gs_order)
+        with self._lock:
+            self._all.append((mesh, eq))
+        return mesh, eq
+
+    def release(self, n, item):
+        with self._lock:
+            self._free.setdefault(n, []).append(item)
+
+    def close(self):
+        with self._lock:
+            for mesh, eq in self._all:
+                eq.close()
+                mesh.close()
+            self._all, self._free = [], {}
+
+
+def solve_case(pool, n, re, ntstep=10, ncoef=3):
+    """One request: reset a context for edge n to rest with mu = 1/re (rho 1,
+    lid speed 1), run ntstep x ncoef outer iterations, return u, v, w, p."""
+    mesh, eq = pool.acquire(n)
+    try:
+        ne = n ** 3
+        for f in STATE_FIELDS:
+            eq.set(f, np.zeros(eq.sizes[f]))
+        eq.set_properties(np.ones(ne), np.full(ne, 1.0 / re))
+        eq.clear_log()
+        eq.run(ntstep, ncoef)
+        return {k: eq.get(k)[:ne] for k in ("u", "v", "w", "p")}
+    finally:
+        pool.release(n, (mesh, eq))
+
+
+def make_requests(count, rate, seed, sizes=tuple(range(16, 97, 8)), re_range=(100.0, 1000.0)):
+    """Poisson arrivals at rate per second; N uniform over sizes; Re
+    log-uniform over re_range.  Returns [(arrival_s, n, re)]."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, count))
+    ns = rng.choice(np.asarray(sizes), count)
+    res = np.exp(rng.uniform(np.log(re_range[0]), np.log(re_range[1]), count))
+    return [(float(a), int(b), float(c)) for a, b, c in zip(t, ns, res)]
+
+
+def serve(requests, slots=4, hbm_budget=200e9, pc_order="reference", gs_order="reference",
+          ntstep=10, ncoef=3, keep_fields=False):
+    """Runs the requests ([(arrival_s, n, re)], arrival times from the start)
+    and returns a dict: wall seconds, solves per second, the p50 and p99
+    latencies (completion minus arrival, seconds), per-request latencies and,
+    with keep_fields, each request's fields."""
+    pool = ContextPool(pc_order, gs_order)
+    cond = threading.Condition()
+    state = {"bytes": 0.0, "busy": 0}
+    lat = [0.0] * len(requests)
+    fields = [None] * len(requests)
+    errors = []
+
+    def run(i, arrival, n, re, need):
+        try:
+            out = solve_case(pool, n, re, ntstep, ncoef)
+            if keep_fields:
+                fields[i] = out
+        except Exception as exc:  # noqa: BLE001 -- reported after the stream
+            errors.append((i, exc))
+        finally:
+            lat[i] = time.perf_counter() - t0 - arrival
+            with cond:
+                state["bytes"] -= need
+                state["busy"] -= 1
+                cond.notify_all()
+
+    try:
+        with cf.ThreadPoolExecutor(max_workers=slots) as ex:
+            t0 = time.perf_counter()
+            for i, (arrival, n, re) in enumerate(requests):
+                need = float(BYTES_PER_CELL) * n ** 3
+                if need > hbm_budget:
+                    raise ValueError("request %d (N=%d) needs more than the HBM budget" % (i, n))
+                wait = t0 + arrival - time.perf_counter()
+                if wait > 0:
+                    time.sleep(wait)
+                with cond:
+                    cond.wait_for(lambda: state["busy"] < slots and
+                                  state["bytes"] + need <= hbm_budget)
+                    state["busy"] += 1
+                    state["bytes"] += need
+                ex.submit(run, i, arrival, n, re, need)
+        wall = time.perf_counter() - t0
+    finally:
+        pool.close()
+    if errors:
+        raise RuntimeError("request %d failed: %s" % errors[0])
+    ms = np.asarray(lat)
+    return {"wall_s": wall, "solves_per_s": len(requests) / wall,
+            "p50_s": float(np.percentile(ms, 50)), "p99_s": float(np.percentile(ms, 99)),
+            "latency_s": lat, "fields": fields if keep_fields else None}

@@ -462,3 +462,24 @@ def test_colored_gs_order_bitwise(n, jit, nsubd, pc):
     oc.run(2, 3)
     assert_fields_equal(eq, oc, ["u", "v", "w", "p", "mip", "gp", "phic"])
     assert_logs_match(eq.log(), oc.log())
+
+
+def test_serving_alone_vs_mix():
+    """Requests served three at a time from reused contexts (every field
+    cleared, viscosity from Re) give the bits of the same requests solved
+    alone on fresh contexts, in the fastest mode (mixed pc solve, colour
+    order of solve_gs)."""
+    from cfdl import serving
+    reqs = [(0.0, 8, 150.0), (0.0, 10, 400.0), (0.0, 8, 900.0), (0.01, 12, 250.0),
+            (0.02, 10, 120.0), (0.03, 8, 600.0)]
+    out = serving.serve(reqs, slots=3, pc_order="mixed", gs_order="colored", ntstep=2, ncoef=3,
+                        keep_fields=True)
+    assert out["solves_per_s"] > 0 and out["p99_s"] >= out["p50_s"] > 0
+    for i, (_, n, re) in enumerate(reqs):
+        pool = serving.ContextPool("mixed", "colored")
+        try:
+            alone = serving.solve_case(pool, n, re, 2, 3)
+        finally:
+            pool.close()
+        for k in ("u", "v", "w", "p"):
+            assert np.array_equal(out["fields"][i][k], alone[k]), (i, k)

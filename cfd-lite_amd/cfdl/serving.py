@@ -23,9 +23,12 @@ from .solver import FIELDS, UVWP
 # the fields of solve_uvwp's state and work arrays (the energy fields are not
 # used by these solves)
 STATE_FIELDS = [f for f in FIELDS if f not in ("t", "gt", "h", "h0", "gh")]
-# device bytes per cell of one context: fields, mesh, the pc system in two
-# numberings, the sweep schedules (a bound from the allocations, not measured)
-BYTES_PER_CELL = 3072
+# device footprint of one context (fields, mesh, the pc system in two
+# numberings, sweep schedules, the FP32 copies of the mixed mode): measured
+# on MI355X from the free device memory around context creation, 375 MB at
+# 64^3 and 1.11 GB at 96^3, i.e. about 64 MiB plus 1.2 KB per cell
+BYTES_PER_CONTEXT = 64 << 20
+BYTES_PER_CELL = 1280
 
 
 class ContextPool:
@@ -118,7 +121,7 @@ def serve(requests, slots=4, hbm_budget=200e9, pc_order="reference", gs_order="r
         with cf.ThreadPoolExecutor(max_workers=slots) as ex:
             t0 = time.perf_counter()
             for i, (arrival, n, re) in enumerate(requests):
-                need = float(BYTES_PER_CELL) * n ** 3
+                need = float(BYTES_PER_CONTEXT + BYTES_PER_CELL * n ** 3)
                 if need > hbm_budget:
                     raise ValueError("request %d (N=%d) needs more than the HBM budget" % (i, n))
                 wait = t0 + arrival - time.perf_counter()

@@ -31,19 +31,48 @@ BYTES_PER_CONTEXT = 64 << 20
 BYTES_PER_CELL = 1280
 
 
+def footprint(n):
+    """Device bytes of one context for edge n."""
+    return BYTES_PER_CONTEXT + BYTES_PER_CELL * n ** 3
+
+
 class ContextPool:
-    """Contexts cached per box edge, each with its mesh."""
+    """Contexts cached per box edge, each with its mesh; bytes counts the
+    footprint of every context held, idle or in use."""
 
     def __init__(self, pc_order="reference", gs_order="reference", nsubd=4):
         self.pc_order, self.gs_order, self.nsubd = pc_order, gs_order, nsubd
+        self.bytes = 0
         self._free = {}
         self._all = []
         self._lock = threading.Lock()
 
-    def acquire(self, n):
+    def take(self, n):
+        """An idle context for edge n, or None."""
         with self._lock:
-            if self._free.get(n):
-                return self._free[n].pop()
+            return self._free[n].pop() if self._free.get(n) else None
+
+    def reserve(self, n, budget):
+        """Counts a new context for edge n if it fits the budget, closing idle
+        contexts of other edges to make room; False (nothing counted) if it
+        does not fit even then."""
+        need = footprint(n)
+        with self._lock:
+            while self.bytes + need > budget:
+                victim = next(((k, v) for k, v in self._free.items() if v), None)
+                if victim is None:
+                    return False
+                k, items = victim
+                mesh, eq = items.pop()
+                eq.close()
+                mesh.close()
+                self._all.remove((mesh, eq))
+                self.bytes -= footprint(k)
+            self.bytes += need
+            return True
+
+    def create(self, n):
+        """A new context for edge n (its footprint already reserved)."""
         mesh = Mesh.box(n).partition(self.nsubd)
         eq = UVWP.cavity(mesh, rho=1.0, mu=0.01, dt=0.01, nit=100)
         eq.set_pc_order(self.pc_order)
@@ -51,6 +80,14 @@ class ContextPool:
         with self._lock:
             self._all.append((mesh, eq))
         return mesh, eq
+
+    def acquire(self, n, budget=float("inf")):
+        item = self.take(n)
+        if item is None:
+            if not self.reserve(n, budget):
+                raise ValueError("a context for N=%d does not fit the HBM budget" % n)
+            item = self.create(n)
+        return item
 
     def release(self, n, item):
         with self._lock:
@@ -61,13 +98,14 @@ class ContextPool:
             for mesh, eq in self._all:
                 eq.close()
                 mesh.close()
-            self._all, self._free = [], {}
+            self._all, self._free, self.bytes = [], {}, 0
 
 
-def solve_case(pool, n, re, ntstep=10, ncoef=3):
-    """One request: reset a context for edge n to rest with mu = 1/re (rho 1,
-    lid speed 1), run ntstep x ncoef outer iterations, return u, v, w, p."""
-    mesh, eq = pool.acquire(n)
+def solve_case(pool, n, re, ntstep=10, ncoef=3, item=None):
+    """One request: reset a context for edge n (item, or one from the pool)
+    to rest with mu = 1/re (rho 1, lid speed 1), run ntstep x ncoef outer
+    iterations, return u, v, w, p."""
+    mesh, eq = item if item is not None else pool.acquire(n)
     try:
         ne = n ** 3
         for f in STATE_FIELDS:
@@ -95,17 +133,21 @@ def serve(requests, slots=4, hbm_budget=200e9, pc_order="reference", gs_order="r
     """Runs the requests ([(arrival_s, n, re)], arrival times from the start)
     and returns a dict: wall seconds, solves per second, the p50 and p99
     latencies (completion minus arrival, seconds), per-request latencies and,
-    with keep_fields, each request's fields."""
+    with keep_fields, each request's fields.  A request is admitted, in
+    arrival order, when a slot is free and either an idle context of its edge
+    exists or a new one fits hbm_budget with every context held."""
     pool = ContextPool(pc_order, gs_order)
     cond = threading.Condition()
-    state = {"bytes": 0.0, "busy": 0}
+    busy = [0]
     lat = [0.0] * len(requests)
     fields = [None] * len(requests)
     errors = []
 
-    def run(i, arrival, n, re, need):
+    def run(i, arrival, n, re, item):
         try:
-            out = solve_case(pool, n, re, ntstep, ncoef)
+            if item is None:
+                item = pool.create(n)
+            out = solve_case(pool, n, re, ntstep, ncoef, item)
             if keep_fields:
                 fields[i] = out
         except Exception as exc:  # noqa: BLE001 -- reported after the stream
@@ -113,26 +155,33 @@ def serve(requests, slots=4, hbm_budget=200e9, pc_order="reference", gs_order="r
         finally:
             lat[i] = time.perf_counter() - t0 - arrival
             with cond:
-                state["bytes"] -= need
-                state["busy"] -= 1
+                busy[0] -= 1
                 cond.notify_all()
 
     try:
         with cf.ThreadPoolExecutor(max_workers=slots) as ex:
             t0 = time.perf_counter()
             for i, (arrival, n, re) in enumerate(requests):
-                need = float(BYTES_PER_CONTEXT + BYTES_PER_CELL * n ** 3)
-                if need > hbm_budget:
+                if footprint(n) > hbm_budget:
                     raise ValueError("request %d (N=%d) needs more than the HBM budget" % (i, n))
                 wait = t0 + arrival - time.perf_counter()
                 if wait > 0:
                     time.sleep(wait)
+                got = []
+
+                def admit():
+                    if busy[0] >= slots:
+                        return False
+                    item = pool.take(n)
+                    if item is None and not pool.reserve(n, hbm_budget):
+                        return False
+                    got.append(item)  # None: a new context, its bytes reserved
+                    return True
+
                 with cond:
-                    cond.wait_for(lambda: state["busy"] < slots and
-                                  state["bytes"] + need <= hbm_budget)
-                    state["busy"] += 1
-                    state["bytes"] += need
-                ex.submit(run, i, arrival, n, re, need)
+                    cond.wait_for(admit)
+                    busy[0] += 1
+                ex.submit(run, i, arrival, n, re, got[0])
         wall = time.perf_counter() - t0
     finally:
         pool.close()

@@ -475,6 +475,10 @@ def test_serving_alone_vs_mix():
     out = serving.serve(reqs, slots=3, pc_order="mixed", gs_order="colored", ntstep=2, ncoef=3,
                         keep_fields=True)
     assert out["solves_per_s"] > 0 and out["p99_s"] >= out["p50_s"] > 0
+    # a budget of two contexts: admissions wait and idle contexts are evicted
+    tight = serving.footprint(12) + serving.footprint(10)
+    out2 = serving.serve(reqs, slots=3, hbm_budget=tight, pc_order="mixed", gs_order="colored",
+                         ntstep=2, ncoef=3, keep_fields=True)
     for i, (_, n, re) in enumerate(reqs):
         pool = serving.ContextPool("mixed", "colored")
         try:
@@ -483,3 +487,4 @@ def test_serving_alone_vs_mix():
             pool.close()
         for k in ("u", "v", "w", "p"):
             assert np.array_equal(out["fields"][i][k], alone[k]), (i, k)
+            assert np.array_equal(out2["fields"][i][k], alone[k]), (i, k)

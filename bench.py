@@ -217,7 +217,7 @@ def main():
                          "outer iterations) and report solves/s with p50/p99 latency")
     ap.add_argument("--serving-requests", type=int, default=512)
     ap.add_argument("--serving-rate", type=float, default=10.0, help="arrivals per second")
-    ap.add_argument("--serving-slots", type=int, default=4, help="requests in flight")
+    ap.add_argument("--serving-slots", type=int, default=8, help="requests in flight")
     ap.add_argument("--pcie", action="store_true",
                     help="also time outer iterations with the host transfers of the drop-in "
                          "Fortran solve_uvwp (state up, state and work arrays down)")

@@ -488,3 +488,16 @@ def test_serving_alone_vs_mix():
         for k in ("u", "v", "w", "p"):
             assert np.array_equal(out["fields"][i][k], alone[k]), (i, k)
             assert np.array_equal(out2["fields"][i][k], alone[k]), (i, k)
+
+
+def test_energy_equation_colored_gs_bitwise():
+    """The energy solve goes through solve_gs too, so the colour order of
+    solve_gs applies to it: momentum and energy in colour order, against the
+    oracle in the same order, energy fields and solver lines bit for bit."""
+    eq, oc = make_pair(9, 0.15, 4)
+    eq.set_gs_order("colored")
+    oc.set_gs_order("colored")
+    eq.run(2, 2, energy=True)
+    oc.run(2, 2, energy=True)
+    assert_fields_equal(eq, oc, ["t", "gt", "h", "h0", "gh", "u", "v", "w", "p"])
+    assert_logs_match(eq.log(), oc.log())

@@ -501,3 +501,22 @@ def test_energy_equation_colored_gs_bitwise():
     oc.run(2, 2, energy=True)
     assert_fields_equal(eq, oc, ["t", "gt", "h", "h0", "gh", "u", "v", "w", "p"])
     assert_logs_match(eq.log(), oc.log())
+
+
+@pytest.mark.parametrize("dims,lens,jit,nsubd,order", [
+    ((7, 5, 9), (1.4, 1.0, 2.0), 0.1, 4, "reference"), ((10, 4, 6), (2.0, 0.5, 1.0), 0.0, 2, "reference"),
+    ((12, 6, 10), (1.2, 0.6, 1.0), 0.1, 4, "mixed")])
+def test_non_cubic_box_bitwise(dims, lens, jit, nsubd, order):
+    """Boxes with unequal cell counts and side lengths (the oracle matches the
+    reference build on such boxes): another partition, other tile shapes and
+    level structures.  Fields and solver lines match the oracle."""
+    mesh = cfdl.Mesh.box(*dims, lx=lens[0], ly=lens[1], lz=lens[2], jitter=jit).partition(nsubd)
+    eq = cfdl.UVWP.cavity(mesh, rho=1.0, mu=0.01, dt=0.02)
+    g = meshgen.box_mesh(*dims, lx=lens[0], ly=lens[1], lz=lens[2], jitter=jit)
+    oc = oracle_lib.OracleCase(g, meshgen.cavity_bc(g), nsubd=nsubd, rho=1.0, mu=0.01, dt=0.02)
+    eq.set_pc_order(order)
+    oc.set_pc_order(order)
+    eq.run(2, 3)
+    oc.run(2, 3)
+    assert_fields_equal(eq, oc)
+    assert_logs_match(eq.log(), oc.log())

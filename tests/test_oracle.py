@@ -622,3 +622,27 @@ def test_serving_request_stream_is_seeded_and_bounded():
     re = np.array([r[2] for r in a])
     assert re.min() >= 100.0 and re.max() <= 1000.0
     assert abs(np.median(np.log(re)) - 0.5 * (np.log(100.0) + np.log(1000.0))) < 0.2
+
+
+@pytest.mark.skipif(not refsim_io.have_refsim("O2"), reason="reference build absent")
+@pytest.mark.parametrize("dims,lens,jit,nsubd", [((7, 5, 9), (1.4, 1.0, 2.0), 0.1, 4),
+                                                 ((10, 4, 6), (2.0, 0.5, 1.0), 0.0, 2)])
+def test_oracle_matches_reference_non_cubic_box(dims, lens, jit, nsubd):
+    """Boxes with unequal cell counts and side lengths: the bisection splits
+    the longest axis of each group's bounding box (mod_agglomeration.f90:
+    380-561), so the subdomains, their local order and the interface pairs
+    differ from a cube's.  Set-up arrays, fields and solver lines equal the
+    reference's bit for bit."""
+    g = meshgen.box_mesh(*dims, lx=lens[0], ly=lens[1], lz=lens[2], jitter=jit)
+    oc = oracle_lib.OracleCase(g, meshgen.cavity_bc(g), nsubd=nsubd, rho=1.0, mu=0.01, dt=0.02)
+    with tempfile.TemporaryDirectory() as td:
+        mf, of = os.path.join(td, "m.bin"), os.path.join(td, "o.bin")
+        meshgen.write_refsim_mesh(g, mf)
+        log, _ = refsim_io.run_refsim(mf, of, nsubd, 1.0, 0.01, 0.02, 2, 3)
+        d = refsim_io.read_dump(of)
+    for k in ("gf2g", "g2gf", "g2gf_idx"):
+        assert np.array_equal(oc.arr(k), d[k]), k
+    oc.run(2, 3)
+    for k in FIELDS + SETUP:
+        assert np.array_equal(oc.arr(k), d[k]), k
+    assert [l[:2] for l in oc.log()] == [l[:2] for l in log]

@@ -6,6 +6,10 @@ iterations and routine by routine.  The residual logs must have the same
 solver iteration counts; residual values come from fixed-order parallel
 reductions and agree to 1e-12 relative.
 """
+import hashlib
+import json
+import os
+
 import numpy as np
 import pytest
 
@@ -520,3 +524,22 @@ def test_non_cubic_box_bitwise(dims, lens, jit, nsubd, order):
     oc.run(2, 3)
     assert_fields_equal(eq, oc)
     assert_logs_match(eq.log(), oc.log())
+
+
+def test_cavity128_bench_configuration_digests():
+    """The benchmark's own configuration (128^3 cavity, Re 400, 4 subdomains,
+    under-relaxation 0.7/0.3), two outer iterations: every field's SHA-256
+    digest equals the oracle's (tests/golden/make_digest128.py), so the
+    headline workload itself is bit-identical, and the solver lines match."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                        "cavity128_relaxed_digests.json")
+    ref = json.load(open(path))
+    c = ref["case"]
+    mesh = cfdl.Mesh.box(c["n"]).partition(c["nsubd"])
+    eq = cfdl.UVWP.cavity(mesh, rho=c["rho"], mu=c["mu"], dt=c["dt"], nit=c["nit"])
+    eq.set_relaxation(c["alpha_u"], c["alpha_p"])
+    eq.run(c["ntstep"], c["ncoef"])
+    for k, h in ref["digests"].items():
+        got = hashlib.sha256(np.ascontiguousarray(eq.get(k), dtype="<f8").tobytes()).hexdigest()
+        assert got == h, k
+    assert_logs_match(eq.log(), [tuple(l) for l in ref["log"]])

@@ -543,3 +543,26 @@ def test_cavity128_bench_configuration_digests():
         got = hashlib.sha256(np.ascontiguousarray(eq.get(k), dtype="<f8").tobytes()).hexdigest()
         assert got == h, k
     assert_logs_match(eq.log(), [tuple(l) for l in ref["log"]])
+
+
+def test_cavity128_reference_mode_matches_the_reference_build():
+    """The 128^3 cavity in the reference's own algorithm (Re 400, 4
+    subdomains, no under-relaxation), two outer iterations: every field's
+    SHA-256 digest equals that of the reference build's fields
+    (tests/golden/make_digest128_ref.py runs oracle/_ref/refsim_O2), and the
+    solver lines agree (iteration counts exactly, residuals to the four
+    digits the reference prints)."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                        "cavity128_reference_digests.json")
+    ref = json.load(open(path))
+    c = ref["case"]
+    mesh = cfdl.Mesh.box(c["n"]).partition(c["nsubd"])
+    eq = cfdl.UVWP.cavity(mesh, rho=c["rho"], mu=c["mu"], dt=c["dt"], nit=c["nit"])
+    eq.run(c["ntstep"], c["ncoef"])
+    for k, h in ref["digests"].items():
+        got = hashlib.sha256(np.ascontiguousarray(eq.get(k), dtype="<f8").tobytes()).hexdigest()
+        assert got == h, k
+    log = eq.log()
+    assert [(l[0], l[1]) for l in log] == [(l[0], l[1]) for l in ref["log"]]
+    for a, b in zip(log, ref["log"]):
+        assert np.allclose(a[2:], b[2:], rtol=6e-4, atol=0), (a, b)

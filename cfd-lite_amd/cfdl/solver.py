@@ -88,7 +88,8 @@ class UVWP:
     def set_pc_order(self, order="reference"):
         """Visiting order of the multi-subdomain pressure-correction smoother:
         "reference" (the reference's order, the default), "colored" (greedy
-        colouring of each subdomain, swept colour by colour; an extension) or
+        colouring of each subdomain in global cell order, swept colour by
+        colour; an extension) or
         "mixed" (the colour order with FP32 sweeps on the correction system
         under FP64 defect correction at every residual check; an extension)."""
         if order not in self.PC_ORDERS:

@@ -2632,16 +2632,23 @@ int cfdl_ctx_create(int ne, int nf, int nbf, const int *ef2nb_idx, const int *ef
     }
     const std::vector<int> xsrc_old = xsrc;
     for (auto &v : xsrc) v = o2n[v];
-    // colour order (extension): greedy colouring of each subdomain in its
-    // local order, a cell taking the least colour that none of its earlier
-    // interior neighbours has; colour + 1 serves as the cell's level
-    std::vector<int> colq(ne, 0), coln(ne);
+    // colour order (extension): greedy colouring of each subdomain in global
+    // cell order, a cell taking the least colour that none of its interior
+    // neighbours earlier in that order has (two colours on a structured hex
+    // box, where the subdomain-local quicksort order needs four); colour + 1
+    // serves as the cell's level
+    std::vector<int> colq(ne, 0), coln(ne), byg(ne);
+    for (int q = 0; q < ne; q++) byg[q] = q;
     for (int s = 0; s < nsubd; s++)
-      for (int q = C->co[s]; q < C->co[s + 1]; q++) {
+      std::sort(byg.begin() + C->co[s], byg.begin() + C->co[s + 1],
+                [&](int x, int y) { return l2g[x] < l2g[y]; });
+    for (int s = 0; s < nsubd; s++)
+      for (int i = C->co[s]; i < C->co[s + 1]; i++) {
+        const int q = byg[i];
         uint64_t used = 0;
         for (int k = rowo[q]; k < rowo[q + 1]; k++) {
           const int d = nbro[k];
-          if (d < ne && d < q && colq[d] < 64) used |= uint64_t{1} << colq[d];
+          if (d < ne && l2g[d] < l2g[q] && colq[d] < 64) used |= uint64_t{1} << colq[d];
         }
         int col = 0;
         while (col < 64 && ((used >> col) & 1)) col++;

@@ -152,8 +152,9 @@ int cfdl_set_relaxation(cfdl_ctx *c, double alpha_u, double alpha_p);
    (smoother_gs inside multi_subdomain_solver, mod_solver.f90:124-228).
    CFDL_PC_REFERENCE (the default) is the reference's order, bit for bit.
    CFDL_PC_COLORED has no counterpart in the reference: every subdomain is
-   coloured greedily in its local order (a cell takes the least colour none
-   of its earlier interior neighbours has) and the sweeps visit the cells
+   coloured greedily in global cell order (a cell takes the least colour
+   none of its interior neighbours earlier in that order has; two colours on
+   a structured box) and the sweeps visit the cells
    colour by colour (forward) and in reverse (backward) -- the same
    Gauss-Seidel update in another sequential order, with every stop rule
    unchanged, in a few fully parallel launches per sweep.

@@ -258,7 +258,8 @@ static void update_halos(or_intf *t, or_subd *s1, or_subd *s2) {
 
 /* smoother_gs (src/modules/mod_solver.f90:191-228) visiting the cells in the
    order perm[0..ne-1] (1-based cells) forward and in reverse backward: the
-   same update, a different sequential order (colour order, extension) */
+   same update, a different sequential order (colour order, extension; the
+   colours come from a greedy colouring in global cell order) */
 static void smoother_gs_perm(const char *name, double *phi, const double *ap, const double *anb,
                              const double *b, const int *ef2nb_idx, const int *ef2nb, int ne,
                              const int *perm, int nit) {
@@ -277,6 +278,12 @@ static void smoother_gs_perm(const char *name, double *phi, const double *ap, co
   }
 }
 
+/* qsort order of int pairs by their first element (distinct here) */
+static int cmp_first(const void *a, const void *b) {
+  int x = ((const int *)a)[0], y = ((const int *)b)[0];
+  return (x > y) - (x < y);
+}
+
 void or_set_pc_order(or_case *c, int order) {
   if (c->pc_perm) {
     for (int k = 0; k < c->nsubd; k++) free(c->pc_perm[k]);
@@ -288,19 +295,26 @@ void or_set_pc_order(or_case *c, int order) {
   c->pc_perm = calloc((size_t)c->nsubd, sizeof(int *));
   for (int cc = 0; cc < c->nsubd; cc++) {
     const or_subd *s = &c->sd[cc];
+    /* global index of each local cell, and the local cells in global order */
+    const int *gl = c->g2gf + c->g2gf_idx[cc] - 1;
     int *col = calloc((size_t)s->ne, sizeof(int));
     int ncol = 0;
-    for (int e = 1; e <= s->ne; e++) {
+    int (*byg)[2] = malloc((size_t)s->ne * sizeof(*byg)); /* (global index, local cell) */
+    for (int e = 1; e <= s->ne; e++) { byg[e - 1][0] = gl[e - 1]; byg[e - 1][1] = e; }
+    qsort(byg, (size_t)s->ne, sizeof(*byg), cmp_first);
+    for (int i = 0; i < s->ne; i++) {
+      int e = byg[i][1];
       unsigned long long used = 0; /* a hex cell has at most 6 neighbours */
       for (int idx = s->ef2nb_idx[e - 1]; idx <= s->ef2nb_idx[e] - 1; idx++) {
         int d = PK_E(s->ef2nb[idx - 1]);
-        if (d <= s->ne && d < e && col[d - 1] < 64) used |= 1ull << col[d - 1];
+        if (d <= s->ne && gl[d - 1] < gl[e - 1] && col[d - 1] < 64) used |= 1ull << col[d - 1];
       }
       int k = 0;
       while (k < 64 && (used >> k & 1ull)) k++;
       col[e - 1] = k;
       if (k + 1 > ncol) ncol = k + 1;
     }
+    free(byg);
     int *perm = malloc((size_t)s->ne * sizeof(int)), n = 0;
     for (int k = 0; k < ncol; k++)
       for (int e = 1; e <= s->ne; e++)

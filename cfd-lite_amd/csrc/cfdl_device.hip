@@ -1450,8 +1450,8 @@ __global__ void k_pc_init(PcCtl *c, const double *red, SubCounts sc, int nit) {
   c->it = 0;
   c->active = (0 < nit && c->rf > c->target) ? 1 : 0;
 }
-__global__ void k_pc_block_end(PcCtl *c, const double *red, SubCounts sc, int nit, int has_res) {
-  if (threadIdx.x != 0 || !c->active) return;
+__device__ void pc_block_end(PcCtl *c, const double *red, SubCounts sc, int nit, int has_res) {
+  if (!c->active) return;
   c->it = c->it + 2;
   if (has_res) {
     double rf = c->rf, rmt = c->rmt;
@@ -1464,6 +1464,21 @@ __global__ void k_pc_block_end(PcCtl *c, const double *red, SubCounts sc, int ni
     c->rmt = rmt;
   }
   c->active = (c->it < nit && c->rf > c->target) ? 1 : 0;
+}
+__global__ void k_pc_block_end(PcCtl *c, const double *red, SubCounts sc, int nit, int has_res) {
+  if (threadIdx.x == 0) pc_block_end(c, red, sc, nit, has_res);
+}
+// A block without a residual check: the halo exchange and the block's stop
+// test in one launch.  The exchange does not read the active flag the stop
+// test writes: once a solve has stopped its values no longer change, so the
+// copies it makes are the ones already made (or, stopped on entry, the
+// gathered halo values themselves).
+template <typename T>
+__global__ void k_exchange_end(int n, const int *dst, const int *src, T *phic, PcCtl *c,
+                               const double *red, SubCounts sc, int nit) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) phic[dst[i]] = phic[src[i]];
+  if (i == 0) pc_block_end(c, red, sc, nit, 0);
 }
 __global__ void k_log_pc(const PcCtl *c, LogLine *log, int *nlog) {
   if (threadIdx.x != 0) return;
@@ -2041,13 +2056,22 @@ static int multi_subdomain_solver(cfdl_ctx *c) {
         CALL(pc_sweep(c, c->pcl, s1, false, act));
       }
     }
+    const int has_res = ((2 * blk) % 10 == 0) ? 1 : 0;
+    if (!has_res && c->nx > 0) {  // exchange and stop test in one launch
+      if (mixed)
+        hipLaunchKernelGGL(k_exchange_end<float>, dim3(nblocks(c->nx)), dim3(kBlock), 0, c->st,
+                           c->nx, c->xdst, xsrc, c->d32, c->pctl, c->red, sc, c->nit);
+      else
+        hipLaunchKernelGGL(k_exchange_end<double>, dim3(nblocks(c->nx)), dim3(kBlock), 0, c->st,
+                           c->nx, c->xdst, xsrc, c->phic_p, c->pctl, c->red, sc, c->nit);
+      continue;
+    }
     if (c->nx > 0 && mixed)
       hipLaunchKernelGGL(k_exchange<float>, dim3(nblocks(c->nx)), dim3(kBlock), 0, c->st, c->nx,
                          c->xdst, xsrc, c->d32, act);
     else if (c->nx > 0)
       hipLaunchKernelGGL(k_exchange<double>, dim3(nblocks(c->nx)), dim3(kBlock), 0, c->st, c->nx,
                          c->xdst, xsrc, c->phic_p, act);
-    const int has_res = ((2 * blk) % 10 == 0) ? 1 : 0;
     if (has_res && mixed)
       hipLaunchKernelGGL(k_mixed_update, dim3(nblocks(nall)), dim3(kBlock), 0, c->st, nall,
                          c->phic_p, c->d32, act);

@@ -2743,11 +2743,11 @@ int cfdl_ctx_create(int ne, int nf, int nbf, const int *ef2nb_idx, const int *ef
     CALL(dalloc(C, &C->phic_p, int64_t{ne} + nh));
     int maxsd = 0;
     for (int s = 0; s < nsubd; s++) maxsd = std::max(maxsd, C->co[s + 1] - C->co[s]);
-    C->chunk_p = std::max(kBlock * 8, (maxsd + 1023) / 1024);
+    C->chunk_p = std::max(kBlock * 4, (maxsd + 2047) / 2048);
     C->nblk_p = (maxsd + C->chunk_p - 1) / C->chunk_p;
   }
   trace("ctx_create: subdomains");
-  C->chunk_g = std::max(kBlock * 8, (ne + 1023) / 1024);
+  C->chunk_g = std::max(kBlock * 4, (ne + 2047) / 2048);
   C->nblk_g = (ne + C->chunk_g - 1) / C->chunk_g;
   const int nblk = std::max(C->nblk_g * 3, C->nblk_p * nsubd);
   CALL(dalloc(C, &C->part, 2 * int64_t{nblk}));

@@ -202,7 +202,11 @@ def main():
                     help="run from rest to a 1e-6 scaled residual (or --max-outer) instead")
     ap.add_argument("--max-outer", type=int, default=500)
     ap.add_argument("--tol", type=float, default=1e-6)
-    ap.add_argument("--pc-order", choices=("reference", "colored", "mixed"), default="reference",
+    ap.add_argument("--steady", action="store_true",
+                    help="dt = infinity: steady SIMPLE (no pseudo-time term), the mode that "
+                         "converges to the 1e-6 scaled residual")
+    ap.add_argument("--pc-order", choices=("reference", "colored", "mixed", "mgcg", "mgcg-mixed"),
+                    default="reference",
                     help="visiting order of the pressure-correction smoother; 'colored' is an "
                          "extension (greedy colouring, one launch per colour) reported on its "
                          "own line, not as the headline value; 'mixed' also runs the colour "
@@ -240,7 +244,10 @@ def main():
     print("bench: setup %.1f s, %d cells" % (time.perf_counter() - t0, ne), file=sys.stderr, flush=True)
 
     if args.converge:
+        if args.steady:
+            eq.set_dt(float("inf"))
         eq.set_pc_order(args.pc_order)
+        eq.set_gs_order(args.gs_order)
         k, secs, worst = run_to_convergence(eq, ne, args.tstep_every, args.max_outer, args.tol)
         print(json.dumps({
             "metric": METRIC, "mode": "converge", "pc_order": args.pc_order,

@@ -28,10 +28,13 @@ _SIGS = {
                                 C.c_double, C.c_uint64, C.POINTER(VP)]),
     "cfdl_mesh_from_elements": (C.c_int, [C.c_int, PD, PD, PD, C.c_int, PI, PI, PI, C.POINTER(VP)]),
     "cfdl_mesh_partition": (C.c_int, [VP, C.c_int]),
+    "cfdl_mesh_renumber": (C.c_int, [VP, C.c_uint64]),
     "cfdl_mesh_info": (C.c_int, [VP, PI, PI, PI, PI, PI, PI]),
     "cfdl_mesh_get_i32": (C.c_int, [VP, C.c_char_p, PI, C.c_int64]),
     "cfdl_mesh_get_f64": (C.c_int, [VP, C.c_char_p, PD, C.c_int64]),
     "cfdl_mesh_free": (None, [VP]),
+    "cfdl_calc_residual": (C.c_int, [C.c_int, VP, VP, VP, VP, VP, VP, PD, PD, VP]),
+    "cfdl_mesh_mg_get": (C.c_int, [VP, C.c_int, C.c_char_p, PI, C.c_int64, C.POINTER(C.c_int64)]),
     "cfdl_mesh_set_section_names": (C.c_int, [VP, C.c_int, C.POINTER(C.c_char_p)]),
     "cfdl_mesh_write_vtu": (C.c_int, [VP, C.c_char_p, C.c_int, C.POINTER(C.c_char_p), PI,
                                       C.POINTER(PD)]),
@@ -40,6 +43,10 @@ _SIGS = {
                                   C.POINTER(VP)]),
     "cfdl_ctx_create_cavity": (C.c_int, [VP, C.c_double, C.c_double, C.c_double, C.c_int,
                                          C.POINTER(VP)]),
+    "cfdl_ctx_create_channel": (C.c_int, [VP, C.c_double, C.c_double, C.c_double, C.c_int,
+                                          C.c_double, C.POINTER(VP)]),
+    "cfdl_set_inlet_profile": (C.c_int, [VP, C.c_double, C.c_double, C.c_double, C.c_double,
+                                         C.c_double]),
     "cfdl_ctx_destroy": (None, [VP]),
     "cfdl_set_relaxation": (C.c_int, [VP, C.c_double, C.c_double]),
     "cfdl_set_pc_order": (C.c_int, [VP, C.c_int]),

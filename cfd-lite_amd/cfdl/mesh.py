@@ -51,6 +51,12 @@ class Mesh:
             check(lib.cfdl_mesh_set_section_names(h, len(names), arr))
         return cls(h, names)
 
+    def renumber(self, seed):
+        """Seeded random cell permutation, then reverse Cuthill-McKee
+        (cfdl_mesh_renumber); before partition()."""
+        check(self._lib.cfdl_mesh_renumber(self._h, seed))
+        return self
+
     def partition(self, nsubd):
         """Oct-tree bisection + subdomain-local order (cell_input.f90:131-151)."""
         check(self._lib.cfdl_mesh_partition(self._h, nsubd))
@@ -103,6 +109,19 @@ class Mesh:
         else:
             raise KeyError(name)
         return a
+
+    def mg_get(self, level, name):
+        """One array of the agglomeration multigrid hierarchy that the "mgcg"
+        pc solve builds for this mesh (cfdl_mesh_mg_get); None when the
+        level or array does not exist."""
+        cnt = C.c_int64()
+        check(self._lib.cfdl_mesh_mg_get(self._h, level, name.encode(), None, 0, C.byref(cnt)))
+        if cnt.value < 0:
+            return None
+        a = np.empty(max(cnt.value, 1), np.int32)
+        check(self._lib.cfdl_mesh_mg_get(self._h, level, name.encode(), a.ctypes.data_as(PI),
+                                         cnt.value, C.byref(cnt)))
+        return a[:cnt.value]
 
     @property
     def handle(self):

@@ -12,7 +12,7 @@ import numpy as np
 
 from ._lib import PD, PI, VP, CfdlError, check, load_library
 
-BC_WALL, BC_LID, BC_SYMMETRY = 0, 1, 2
+BC_WALL, BC_LID, BC_SYMMETRY, BC_INLET, BC_OUTLET = 0, 1, 2, 3, 4
 
 FIELDS = {name: k for k, name in enumerate(
     ["u", "v", "w", "p", "u0", "v0", "w0", "gu", "gv", "gw", "gp", "gpc", "mip", "mip0",
@@ -63,6 +63,17 @@ class UVWP:
         return cls(h, cls._sizes(i["ne"], i["nf"], i["nbf"]))
 
     @classmethod
+    def channel(cls, mesh, rho=1.0, mu=0.005, dt=0.01, nit=100, umax=1.0):
+        """Channel along x on a partitioned box Mesh (an extension): 'west'
+        is a parabolic inlet of peak umax, 'east' a zero-gradient outlet
+        scaled to the inflow, the other sections no-slip walls."""
+        lib = load_library()
+        h = VP()
+        check(lib.cfdl_ctx_create_channel(mesh.handle, rho, mu, dt, nit, umax, C.byref(h)))
+        i = mesh.info()
+        return cls(h, cls._sizes(i["ne"], i["nf"], i["nbf"]))
+
+    @classmethod
     def from_arrays(cls, ne, nf, nbf, ef2nb_idx, ef2nb, s2g, bs, xc, yc, zc, vol, aip, rip,
                     esec, bckind, nsubd, gf2g, g2gf, g2gf_idx, rho, mu, dt, nit=100):
         """Context from the reference's own arrays (geometry_t, meshds_t,
@@ -83,7 +94,7 @@ class UVWP:
                                   C.byref(h)))
         return cls(h, cls._sizes(ne, nf, nbf))
 
-    PC_ORDERS = {"reference": 0, "colored": 1, "mixed": 2}
+    PC_ORDERS = {"reference": 0, "colored": 1, "mixed": 2, "mgcg": 3, "mgcg-mixed": 4}
 
     def set_pc_order(self, order="reference"):
         """Visiting order of the multi-subdomain pressure-correction smoother:
@@ -91,7 +102,10 @@ class UVWP:
         colouring of each subdomain in global cell order, swept colour by
         colour; an extension) or
         "mixed" (the colour order with FP32 sweeps on the correction system
-        under FP64 defect correction at every residual check; an extension)."""
+        under FP64 defect correction at every residual check; an extension),
+        "mgcg" (conjugate gradients on the global system preconditioned by an
+        agglomeration-multigrid V-cycle, the reference's stop rule; an
+        extension) or "mgcg-mixed" (the same with the V-cycle in FP32)."""
         if order not in self.PC_ORDERS:
             raise CfdlError("pc order must be one of %s" % sorted(self.PC_ORDERS))
         check(self._lib.cfdl_set_pc_order(self._h, self.PC_ORDERS[order]))
@@ -107,7 +121,7 @@ class UVWP:
             raise CfdlError("gs order must be one of %s" % sorted(self.GS_ORDERS))
         check(self._lib.cfdl_set_gs_order(self._h, self.GS_ORDERS[order]))
 
-    BC_KINDS = {"wall": 0, "lid": 1, "symmetry": 2}
+    BC_KINDS = {"wall": 0, "lid": 1, "symmetry": 2, "inlet": 3, "outlet": 4}
 
     def set_bc_kind(self, first, last, kind):
         """Boundary type of the halos first..last (1-based element numbers), as
@@ -124,6 +138,16 @@ class UVWP:
         esec = mesh.get("esec").reshape(-1, 2)
         s = mesh.names.index(name)
         self.set_bc_kind(int(esec[s][0]), int(esec[s][1]), kind)
+
+    def set_dt(self, dt):
+        """Pseudo-time step (phys_t%dt, mod_physics.f90:15); float("inf") is
+        the steady mode (an extension: transient terms exactly zero, needs
+        alpha_u < 1)."""
+        check(self._lib.cfdl_set_dt(self._h, dt))
+
+    def set_inlet_profile(self, umax=1.0, y0=0.0, y1=1.0, z0=0.0, z1=1.0):
+        """Parabolic inlet of the "inlet" halos (channel extension)."""
+        check(self._lib.cfdl_set_inlet_profile(self._h, umax, y0, y1, z0, z1))
 
     def set_relaxation(self, alpha_u=1.0, alpha_p=1.0):
         """Config-mode under-relaxation (1, 1 = the reference algorithm)."""

@@ -89,7 +89,9 @@ static void trace(const char *stage) {
 
 constexpr int kBlock = 256;
 constexpr int kMaxSys = 16;
-constexpr int kLogCap = 1 << 15;
+// multigrid hierarchy of the MGCG pc solve: at most kMgAgg^l cells per
+// level-l aggregate; levels are added while more than kMgCoarse remain
+constexpr int kMgAgg = 8, kMgCoarse = 64;
 
 struct MeshDev {
   int ne, nf, nbf, L;
@@ -124,11 +126,6 @@ struct PcCtl {
   double ri, rf, rmt, target;
 };
 
-struct LogLine {
-  char name[8];
-  int it, pad;
-  double res_i, res_f, res_max;
-};
 
 // A linear system for the residual / sweep kernels: ny right-hand sides or
 // subdomains, each a contiguous cell range sharing row/nbr/ap/anb.
@@ -214,11 +211,42 @@ __device__ inline void centre(const MeshDev &m, int e, double *r) {
 // ---------------------------------------------------------------------------
 // BC setters lid / dirichlet0 / symmetry (mod_uvwp.f90:493-570), per halo.
 // ---------------------------------------------------------------------------
-__global__ void k_bc_update(MeshDev m, FieldsDev f) {
+// Inlet profile of the channel extension: umax, y0, y1, z0, z1
+struct Inlet {
+  double q[5];
+};
+
+__global__ void k_bc_update(MeshDev m, FieldsDev f, Inlet in) {
   const int h = blockIdx.x * blockDim.x + threadIdx.x;
   if (h >= m.nbf) return;
   const int e = m.hcell[h], H = m.ne + h, kind = m.hkind[h];
   const int fg = abs(m.fsg[m.hslot[h]]) - 1;
+  if (kind == CFDL_BC_INLET || kind == CFDL_BC_OUTLET) {
+    // channel extension: inlet velocity along the inward normal with the
+    // parabolic profile at the face centre, outlet velocity and pressure
+    // from the owner; the face mass flux rho_P (v.n) A, the outlet's later
+    // scaled to the inflow (k_outlet_flux)
+    const double *a = m.aip + 3 * fg;
+    const double area = area_of(a);
+    const double nrm[3] = {a[0] / area, a[1] / area, a[2] / area};
+    double vel[3];
+    if (kind == CFDL_BC_INLET) {
+      const double sy = (m.yc[H] - in.q[1]) / (in.q[2] - in.q[1]);
+      const double sz = (m.zc[H] - in.q[3]) / (in.q[4] - in.q[3]);
+      const double un = in.q[0] * 16.0 * sy * (1.0 - sy) * sz * (1.0 - sz);
+      for (int i = 0; i < 3; i++) vel[i] = un * -nrm[i];
+    } else {
+      vel[0] = f.u[e];
+      vel[1] = f.v[e];
+      vel[2] = f.w[e];
+    }
+    f.u[H] = vel[0];
+    f.v[H] = vel[1];
+    f.w[H] = vel[2];
+    f.p[H] = f.p[e];
+    f.mip[fg] = dot3(vel, nrm) * m.rho[e] * area;
+    return;
+  }
   if (kind == CFDL_BC_SYMMETRY) {
     const double *a = m.aip + 3 * fg;
     const double area = area_of(a);
@@ -240,6 +268,61 @@ __global__ void k_bc_update(MeshDev m, FieldsDev f) {
   }
   f.p[H] = f.p[e];
   f.mip[fg] = 0.0;
+}
+
+// Outlet mass flux of the channel extension: Q_in (inlet faces) and the
+// extrapolated Q_out (outlet faces), each summed in the blocked tree order
+// (blocks of ceil(n / nblk) faces, nblk = min(1024, ceil(n / 4096)); lane t
+// of 256 sums its strided faces, lanes combined pairwise s[t] += s[t + h],
+// h = 128 .. 1, block sums combined the same way), here all in one
+// workgroup that walks the blocks in turn; then the outlet fluxes are
+// scaled by -Q_in / Q_out, or, when Q_out is not positive, replaced by the
+// inflow spread by face area.  Face lists are in halo order.
+constexpr int kOutThreads = 256;
+__device__ double wg_tree(double v, double *s) {
+  const int t = threadIdx.x;
+  s[t] = v;
+  __syncthreads();
+  for (int h = kOutThreads / 2; h >= 1; h >>= 1) {
+    if (t < h) s[t] = s[t] + s[t + h];
+    __syncthreads();
+  }
+  const double r = s[0];
+  __syncthreads();
+  return r;
+}
+// sum of v(k) for k < n in the blocked order; part holds 1024 block sums
+template <typename F>
+__device__ double wg_blocked_sum(int n, F v, double *s, double *part) {
+  int nblk = (n + 4095) / 4096;
+  nblk = nblk < 1 ? 1 : (nblk > 1024 ? 1024 : nblk);
+  const int chunk = (n + nblk - 1) / nblk;
+  for (int x = 0; x < nblk; x++) {
+    const int lo = x * chunk, hi = min(lo + chunk, n);
+    double acc = 0.0;
+    for (int k = lo + threadIdx.x; k < hi; k += kOutThreads) acc = acc + v(k);
+    const double bs = wg_tree(acc, s);
+    if (threadIdx.x == 0) part[x] = bs;
+  }
+  __syncthreads();
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += kOutThreads) acc = acc + part[i];
+  return wg_tree(acc, s);
+}
+__global__ void __launch_bounds__(kOutThreads)
+k_outlet_flux(MeshDev m, double *mip, const int *fin, int nin, const int *fout, int nout) {
+  __shared__ double s[kOutThreads], part[1024];
+  if (nout == 0) return;
+  const double qin = wg_blocked_sum(nin, [&](int k) { return mip[fin[k]]; }, s, part);
+  const double qout = wg_blocked_sum(nout, [&](int k) { return mip[fout[k]]; }, s, part);
+  const double aout = wg_blocked_sum(nout, [&](int k) { return area_of(m.aip + 3 * fout[k]); }, s, part);
+  if (qout > 0.0) {
+    const double sc = -qin / qout;
+    for (int k = threadIdx.x; k < nout; k += kOutThreads) mip[fout[k]] = mip[fout[k]] * sc;
+  } else {
+    for (int k = threadIdx.x; k < nout; k += kOutThreads)
+      mip[fout[k]] = -qin * area_of(m.aip + 3 * fout[k]) / aout;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -319,7 +402,21 @@ __global__ void k_coef_uvw(MeshDev m, FieldsDev f, double dt, double alpha_u) {
     const double dr[3] = {m.xc[H] - rp[0], m.yc[H] - rp[1], m.zc[H] - rp[2]};
     const double ds = sqrt(dot3(dr, dr));
     const double fz = 0.0, d = m.mu[e] * area / ds;
-    if (m.hkind[H - m.ne] != CFDL_BC_SYMMETRY) {  // 'dirichlet', :255-265
+    const int kind = m.hkind[H - m.ne];
+    if (kind == CFDL_BC_INLET || kind == CFDL_BC_OUTLET) {
+      // channel extension: the face's convective flux as on interior faces
+      // (anb += max(f, 0), f * u_P into the right-hand side as sumf),
+      // diffusion at the inlet only (zero-gradient outlet)
+      const double fl = -f.mip[fg], fnb = fmax(fl, 0.0);
+      const double dd = kind == CFDL_BC_INLET ? d : 0.0;
+      bu = bu + fl * f.u[e];
+      bv = bv + fl * f.v[e];
+      bw = bw + fl * f.w[e];
+      ap = ap + dd + fnb;
+      f.anb[k] = f.anb[k] + dd + fnb;
+      continue;
+    }
+    if (kind != CFDL_BC_SYMMETRY) {  // 'dirichlet', :255-265
       const double ue = f.u[e], ve = f.v[e], we = f.w[e];
       double vr[3] = {ue, ve, we};
       const double t = dot3(vr, nrm);
@@ -459,7 +556,19 @@ k_coef_uvw6(MeshDev m, FieldsDev f, double dt, double alpha_u) {
     const double dr[3] = {m.xc[H] - rp[0], m.yc[H] - rp[1], m.zc[H] - rp[2]};
     const double ds = sqrt(dot3(dr, dr));
     const double fz = 0.0, d = m.mu[e] * area / ds;
-    if (m.hkind[H - m.ne] != CFDL_BC_SYMMETRY) {  // 'dirichlet', :255-265
+    const int kind = m.hkind[H - m.ne];
+    if (kind == CFDL_BC_INLET || kind == CFDL_BC_OUTLET) {  // channel extension, as above
+      const double fl = -f.mip[fg], fnb = fmax(fl, 0.0);
+      const double dd = kind == CFDL_BC_INLET ? d : 0.0;
+      bu = bu + fl * f.u[e];
+      bv = bv + fl * f.v[e];
+      bw = bw + fl * f.w[e];
+      ap = ap + dd + fnb;
+      anbk[k - 6 * e] = anbk[k - 6 * e] + dd + fnb;
+      f.anb[k] = anbk[k - 6 * e];
+      continue;
+    }
+    if (kind != CFDL_BC_SYMMETRY) {  // 'dirichlet', :255-265
       const double ue = f.u[e], ve = f.v[e], we = f.w[e];
       double vr[3] = {ue, ve, we};
       const double t = dot3(vr, nrm);
@@ -1618,6 +1727,15 @@ struct cfdl_ctx {
   // ap, anb, residual and correction (cells and halos)
   float *a32 = nullptr, *anb32 = nullptr, *r32 = nullptr, *d32 = nullptr;
   int pc_nlev = 0;
+  // channel extension: inlet profile, the host copy of the halo BC kinds and
+  // the inlet / outlet faces (halo order) whose fluxes k_outlet_flux balances
+  Inlet inlet{{1.0, 0.0, 1.0, 0.0, 1.0}};
+  std::vector<int> hkind_h, hslot_h, fsg_h;
+  int *fin = nullptr, *fout = nullptr;
+  int nin = 0, nout = 0;
+  // multigrid-preconditioned CG pc solve (CFDL_PC_MGCG[_MIXED], an
+  // extension): the hierarchy's device copy, FP64 and FP32 preconditioner
+  cfdl::MgCgSolver *mg[2] = {nullptr, nullptr};
   // space-time tiled schedule of the pc sweeps (the backward sweep runs it
   // in reverse)
   struct TileSched {
@@ -2323,6 +2441,23 @@ extern "C" {
 const char *cfdl_last_error(void) { return g_err.c_str(); }
 int cfdl_version(void) { return 100; }
 
+// the inlet and outlet faces (0-based, halo order) of the channel extension
+static int channel_lists(cfdl_ctx *c) {
+  std::vector<int> fin, fout;
+  for (int h = 0; h < c->nbf; h++) {
+    const int fg = std::abs(c->fsg_h[c->hslot_h[h]]) - 1;
+    if (c->hkind_h[h] == CFDL_BC_INLET) fin.push_back(fg);
+    if (c->hkind_h[h] == CFDL_BC_OUTLET) fout.push_back(fg);
+  }
+  c->nin = static_cast<int>(fin.size());
+  c->nout = static_cast<int>(fout.size());
+  if (!fin.empty())
+    HIPCHK(hipMemcpy(c->fin, fin.data(), sizeof(int) * fin.size(), hipMemcpyHostToDevice));
+  if (!fout.empty())
+    HIPCHK(hipMemcpy(c->fout, fout.data(), sizeof(int) * fout.size(), hipMemcpyHostToDevice));
+  return CFDL_OK;
+}
+
 int cfdl_ctx_create(int ne, int nf, int nbf, const int *ef2nb_idx, const int *ef2nb,
                     const int *s2g, const int *bs, const double *xc, const double *yc,
                     const double *zc, const double *vol, const double *aip, const double *rip,
@@ -2399,7 +2534,14 @@ int cfdl_ctx_create(int ne, int nf, int nbf, const int *ef2nb_idx, const int *ef
   }
   for (int s = 0; s < nsec; s++)
     for (int el = esec[2 * s]; el <= esec[2 * s + 1]; el++)
-      if (el > ne && el <= n) hkind[el - ne - 1] = bckind[s];
+      if (el > ne && el <= n) {
+        if (bckind[s] < CFDL_BC_WALL || bckind[s] > CFDL_BC_OUTLET) {
+          set_error("cfdl_ctx_create: bckind of section " + std::to_string(s + 1) +
+                    " is not a CFDL_BC_* kind");
+          return CFDL_EINVAL;
+        }
+        hkind[el - ne - 1] = bckind[s];
+      }
   for (int e = 0; e < ne; e++) bcrow[e + 1] += bcrow[e];
   {
     std::vector<int> pos(bcrow.begin(), bcrow.end() - 1);
@@ -2418,6 +2560,12 @@ int cfdl_ctx_create(int ne, int nf, int nbf, const int *ef2nb_idx, const int *ef
   CALL(upload(C, &d_hcell, hcell));
   CALL(upload(C, &d_hslot, hslot));
   CALL(upload(C, &d_hkind, hkind));
+  C->hkind_h = hkind;
+  C->hslot_h = hslot;
+  C->fsg_h = fsg;
+  CALL(dalloc(C, &C->fin, nbf));
+  CALL(dalloc(C, &C->fout, nbf));
+  CALL(channel_lists(C));
   auto up64 = [&](double **p, const double *src, int64_t cnt) -> int {
     CALL(dalloc(C, p, cnt));
     HIPCHK(hipMemcpy(*p, src, 8 * cnt, hipMemcpyHostToDevice));
@@ -2771,6 +2919,7 @@ int cfdl_ctx_create(int ne, int nf, int nbf, const int *ef2nb_idx, const int *ef
 void cfdl_ctx_destroy(cfdl_ctx *c) {
   if (!c) return;
   (void)hipStreamSynchronize(c->st);
+  for (auto &m : c->mg) cfdl::mgcg_destroy(m);
   if (c->host_flags) (void)hipHostFree(c->host_flags);
   for (auto &x : c->samples) {
     (void)hipEventDestroy(x.a);
@@ -2808,6 +2957,40 @@ int cfdl_ctx_create_cavity(const cfdl_mesh *m, double rho, double mu, double dt,
                          m->g2gf_idx.data(), r.data(), u.data(), dt, nit, out);
 }
 
+int cfdl_ctx_create_channel(const cfdl_mesh *m, double rho, double mu, double dt, int nit,
+                            double umax, cfdl_ctx **out) {
+  if (!m || m->nsubd < 1) {
+    set_error("cfdl_ctx_create_channel: mesh is not partitioned (call cfdl_mesh_partition)");
+    return CFDL_ESTATE;
+  }
+  std::vector<int> kind(m->nsec, CFDL_BC_WALL);
+  int found = 0;
+  for (int s = 0; s < m->nsec && s < static_cast<int>(m->secname.size()); s++) {
+    if (m->secname[s] == "west") kind[s] = CFDL_BC_INLET, found |= 1;
+    if (m->secname[s] == "east") kind[s] = CFDL_BC_OUTLET, found |= 2;
+  }
+  if (found != 3) {
+    set_error("cfdl_ctx_create_channel: sections 'west' (inlet) and 'east' (outlet) needed");
+    return CFDL_EINVAL;
+  }
+  // the cross-section from the vertex bounding box
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int v = 0; v < m->nvx; v++) {
+    const double p[3] = {m->x[v], m->y[v], m->z[v]};
+    for (int i = 0; i < 3; i++) {
+      lo[i] = std::min(lo[i], p[i]);
+      hi[i] = std::max(hi[i], p[i]);
+    }
+  }
+  std::vector<double> r(m->ne, rho), u(m->ne, mu);
+  CALL(cfdl_ctx_create(m->ne, m->nf, m->nbf, m->ef2nb_idx.data(), m->ef2nb.data(), m->s2g.data(),
+                       m->bs.data(), m->xc.data(), m->yc.data(), m->zc.data(), m->vol.data(),
+                       m->aip.data(), m->rip.data(), m->nsec, m->esec.data(), kind.data(),
+                       m->nsubd, m->gf2g.data(), m->g2gf.data(), m->g2gf_idx.data(), r.data(),
+                       u.data(), dt, nit, out));
+  return cfdl_set_inlet_profile(*out, umax, lo[1], hi[1], lo[2], hi[2]);
+}
+
 int cfdl_set_properties(cfdl_ctx *c, const double *rho, const double *mu) {
   CALL(check_ctx(c));
   if (!rho || !mu) {
@@ -2832,15 +3015,26 @@ int cfdl_set_dt(cfdl_ctx *c, double dt) {
 
 int cfdl_set_bc_kind(cfdl_ctx *c, int first, int last, int kind) {
   CALL(check_ctx(c));
-  if (first <= c->ne || last > c->ne + c->nbf || first > last ||
-      (kind != CFDL_BC_WALL && kind != CFDL_BC_LID && kind != CFDL_BC_SYMMETRY)) {
+  if (first <= c->ne || last > c->ne + c->nbf || first > last || kind < CFDL_BC_WALL ||
+      kind > CFDL_BC_OUTLET) {
     set_error("cfdl_set_bc_kind: halo range or kind out of range");
     return CFDL_EINVAL;
   }
   std::vector<int> k(last - first + 1, kind);
-  HIPCHK(hipMemcpyAsync(const_cast<int *>(c->md.hkind) + (first - c->ne - 1), k.data(),
-                        sizeof(int) * k.size(), hipMemcpyHostToDevice, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
+  HIPCHK(hipMemcpy(const_cast<int *>(c->md.hkind) + (first - c->ne - 1), k.data(),
+                   sizeof(int) * k.size(), hipMemcpyHostToDevice));
+  for (int el = first; el <= last; el++) c->hkind_h[el - c->ne - 1] = kind;
+  return channel_lists(c);
+}
+
+int cfdl_set_inlet_profile(cfdl_ctx *c, double umax, double y0, double y1, double z0, double z1) {
+  CALL(check_ctx(c));
+  if (!(y1 > y0) || !(z1 > z0) || !std::isfinite(umax)) {
+    set_error("cfdl_set_inlet_profile: need y1 > y0, z1 > z0 and a finite umax");
+    return CFDL_EINVAL;
+  }
+  c->inlet = Inlet{{umax, y0, y1, z0, z1}};
   return CFDL_OK;
 }
 
@@ -2855,13 +3049,34 @@ int cfdl_set_relaxation(cfdl_ctx *c, double alpha_u, double alpha_p) {
   return CFDL_OK;
 }
 
+// the multigrid hierarchy of the mesh CSR and its device copy for the
+// FP64 (fp32 = 0) or FP32 (fp32 = 1) preconditioner, built once
+static int mg_ensure(cfdl_ctx *c, int fp32) {
+  if (c->mg[fp32]) return CFDL_OK;
+  const int ne = c->ne, n = ne + c->nbf;
+  std::vector<int> row(ne + 1), nbr(c->L);
+  std::vector<double> x(n), y(n), z(n);
+  HIPCHK(hipStreamSynchronize(c->st));
+  HIPCHK(hipMemcpy(row.data(), c->md.row, sizeof(int) * row.size(), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(nbr.data(), c->md.nbr, sizeof(int) * nbr.size(), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(x.data(), c->md.xc, sizeof(double) * n, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(y.data(), c->md.yc, sizeof(double) * n, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(z.data(), c->md.zc, sizeof(double) * n, hipMemcpyDeviceToHost));
+  cfdl::MgHier h;
+  CALL(cfdl::mg_build(ne, row.data(), nbr.data(), x.data(), y.data(), z.data(), kMgAgg,
+                      kMgCoarse, &h));
+  return cfdl::mgcg_create(h, c->nbf, fp32, &c->mg[fp32]);
+}
+
 int cfdl_set_pc_order(cfdl_ctx *c, int order) {
   CALL(check_ctx(c));
-  if (order != CFDL_PC_REFERENCE && order != CFDL_PC_COLORED && order != CFDL_PC_COLORED_MIXED) {
-    set_error("cfdl_set_pc_order: order must be CFDL_PC_REFERENCE, CFDL_PC_COLORED or "
-              "CFDL_PC_COLORED_MIXED");
+  if (order < CFDL_PC_REFERENCE || order > CFDL_PC_MGCG_MIXED) {
+    set_error("cfdl_set_pc_order: order must be CFDL_PC_REFERENCE, CFDL_PC_COLORED, "
+              "CFDL_PC_COLORED_MIXED, CFDL_PC_MGCG or CFDL_PC_MGCG_MIXED");
     return CFDL_EINVAL;
   }
+  if (order == CFDL_PC_MGCG || order == CFDL_PC_MGCG_MIXED)
+    CALL(mg_ensure(c, order == CFDL_PC_MGCG_MIXED ? 1 : 0));
   if (order == CFDL_PC_COLORED_MIXED && !c->d32) {
     CALL(dalloc(c, &c->a32, c->ne));
     CALL(dalloc(c, &c->anb32, c->Lp));
@@ -2924,7 +3139,12 @@ int cfdl_synchronize(cfdl_ctx *c) {
 int cfdl_update_boundaries(cfdl_ctx *c) {
   CALL(check_ctx(c));
   if (c->nbf > 0) {
-    hipLaunchKernelGGL(k_bc_update, dim3(nblocks(c->nbf)), dim3(kBlock), 0, c->st, c->md, c->fd);
+    hipLaunchKernelGGL(k_bc_update, dim3(nblocks(c->nbf)), dim3(kBlock), 0, c->st, c->md, c->fd,
+                       c->inlet);
+    if (c->nout > 0)
+      hipLaunchKernelGGL(k_outlet_flux, dim3(1), dim3(kOutThreads), 0, c->st, c->md, c->fd.mip,
+                         static_cast<const int *>(c->fin), c->nin,
+                         static_cast<const int *>(c->fout), c->nout);
     hipLaunchKernelGGL(k_bc_energy, dim3(nblocks(c->nbf)), dim3(kBlock), 0, c->st, c->md, c->en);
   }
   HIPCHK(hipGetLastError());
@@ -2985,6 +3205,13 @@ int cfdl_solve_energy(cfdl_ctx *c) {
 
 int cfdl_calc_coef_uvw(cfdl_ctx *c) {
   CALL(check_ctx(c));
+  // steady mode (dt = infinity): the transient terms vanish and, without
+  // under-relaxation, ap - sum(anb) = 0 on interior cells, so dc = V / 0
+  if (std::isinf(c->dt) && c->alpha_u == 1.0) {
+    set_error("cfdl_calc_coef_uvw: dt = infinity (steady) needs momentum under-relaxation "
+              "alpha_u < 1 (cfdl_set_relaxation)");
+    return CFDL_ESTATE;
+  }
   if (c->mhex)
     hipLaunchKernelGGL(k_coef_uvw6, dim3((c->ne + kFaceCells - 1) / kFaceCells), dim3(kFaceThreads),
                        0, c->st, c->md, c->fd, c->dt, c->alpha_u);
@@ -3061,6 +3288,22 @@ int cfdl_set_a_0(cfdl_ctx *c, int field) {
 
 int cfdl_solve(cfdl_ctx *c, const char *name) {
   CALL(check_ctx(c));
+  if (c->pc_order == CFDL_PC_MGCG || c->pc_order == CFDL_PC_MGCG_MIXED) {
+    cfdl::MgCgSystem sys{};
+    sys.ne = c->ne;
+    sys.nbf = c->nbf;
+    sys.nit = c->nit;
+    sys.row = c->md.row;
+    sys.nbr = c->md.nbr;
+    sys.ap = c->fd.ap;
+    sys.anb = c->fd.anb;
+    sys.b = c->fd.b;
+    sys.phi = c->fd.phic;
+    std::strncpy(sys.name, name ? name : "pc", 7);
+    sys.log = c->dlog;
+    sys.nlog = c->dlog_n;
+    return cfdl::mgcg_solve(c->mg[c->pc_order == CFDL_PC_MGCG_MIXED ? 1 : 0], sys, c->st);
+  }
   if (c->nsubd <= 1) {
     double *phi = c->fd.phic;
     const double *b = c->fd.b;
@@ -3194,6 +3437,69 @@ int cfdl_log_clear(cfdl_ctx *c) {
   CALL(drain_log(c));
   c->hlog.clear();
   return CFDL_OK;
+}
+
+// calc_residual (src/modules/mod_solver.f90:230-253) on a system in device
+// memory, outside any context: the residual kernels of the solves
+// (k_resid_partial + k_resid_final, signed max as in the reference) with
+// block partials in a buffer of 2 * nblk + 2 doubles (partials, then the sum
+// of squares and the max).
+constexpr int kResidualMaxCells = 1 << 30;
+static void resid_shape(int ne, int *nblk, int *chunk) {
+  *chunk = std::max(kBlock * 4, (ne + 2047) / 2048);
+  *nblk = (ne + *chunk - 1) / *chunk;
+}
+static int resid_args(int ne, const int *row, const int *nbr, const double *ap, const double *anb,
+                      const double *b, const double *phi, double *res, double *res_max,
+                      const char *who) {
+  if (ne < 1 || ne > kResidualMaxCells || !row || !nbr || !ap || !anb || !b || !phi || !res ||
+      !res_max) {
+    set_error(std::string(who) + ": need 1 <= ne <= 2^30 and non-null arrays");
+    return CFDL_EINVAL;
+  }
+  return CFDL_OK;
+}
+static int resid_run(int ne, const int *row, const int *nbr, const double *ap, const double *anb,
+                     const double *b, const double *phi, double *buf, hipStream_t st,
+                     double *res, double *res_max) {
+  int nblk = 0, chunk = 0;
+  resid_shape(ne, &nblk, &chunk);
+  SysDesc s{};
+  s.ny = 1;
+  s.beg[0] = 0;
+  s.end[0] = ne;
+  s.phi[0] = const_cast<double *>(phi);
+  s.b[0] = b;
+  s.row = row;
+  s.nbr = nbr;
+  s.ap = ap;
+  s.anb = anb;
+  double *part = buf, *red = buf + 2 * nblk;
+  hipLaunchKernelGGL(k_resid_partial, dim3(nblk, 1), dim3(kBlock), 0, st, s, chunk, nblk, 0, part,
+                     static_cast<float *>(nullptr));
+  hipLaunchKernelGGL(k_resid_final, dim3(1), dim3(kBlock), 0, st, static_cast<const double *>(part),
+                     nblk, red);
+  HIPCHK(hipGetLastError());
+  double h[2];
+  HIPCHK(hipMemcpyAsync(h, red, sizeof(h), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  *res = std::sqrt(h[0] / static_cast<double>(ne));
+  *res_max = h[1];
+  return CFDL_OK;
+}
+
+int cfdl_calc_residual(int ne, const int *row, const int *nbr, const double *ap, const double *anb,
+                       const double *b, const double *phi, double *res, double *res_max,
+                       void *stream) {
+  CALL(resid_args(ne, row, nbr, ap, anb, b, phi, res, res_max, "cfdl_calc_residual"));
+  int nblk = 0, chunk = 0;
+  resid_shape(ne, &nblk, &chunk);
+  double *buf = nullptr;
+  HIPCHK(hipMalloc(&buf, sizeof(double) * (2 * static_cast<size_t>(nblk) + 2)));
+  const int rc = resid_run(ne, row, nbr, ap, anb, b, phi, buf, static_cast<hipStream_t>(stream),
+                           res, res_max);
+  HIPCHK(hipFree(buf));
+  return rc;
 }
 
 int cfdl_set_profile(cfdl_ctx *c, int sample_every) {

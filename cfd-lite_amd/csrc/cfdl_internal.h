@@ -35,4 +35,54 @@ struct cfdl_mesh {
 
 namespace cfdl {
 int mesh_build(cfdl_mesh *m);  // connectivity + geometry
-}
+
+// Agglomeration multigrid hierarchy of a CSR system (cfdl_mg.cpp).  Level 0
+// is the given CSR (neighbours >= ne are boundary halos); lists named for
+// the next level (agg, mptr/mem, iptr/ilist, gptr/glist) are empty on the
+// coarsest level.
+constexpr int kMgMaxLevels = 16;
+struct MgLevel {
+  int n = 0, nnz = 0, ncol = 0;
+  std::vector<int> row, col;    // CSR, neighbours in increasing order (level >= 1)
+  std::vector<int> cptr, cperm;  // greedy colours: cperm[cptr[c] .. cptr[c+1])
+  std::vector<int> agg;          // node -> next-level aggregate
+  std::vector<int> mptr, mem;    // next-level aggregate -> its nodes, increasing
+  std::vector<int> iptr, ilist;  // next-level aggregate -> slots inside it
+  std::vector<int> gptr, glist;  // next-level slot -> slots summed into it
+};
+struct MgHier {
+  std::vector<MgLevel> lv;
+};
+int mg_build(int ne, const int *row, const int *nbr, const double *xc, const double *yc,
+             const double *zc, int maxagg, int ncoarse, MgHier *out);
+
+// one line of the device residual log (a linear solve's name, iterations,
+// initial, final and maximum residual)
+struct LogLine {
+  char name[8];
+  int it, pad;
+  double res_i, res_f, res_max;
+};
+constexpr int kLogCap = 1 << 15;
+
+// Multigrid-preconditioned conjugate gradients on a CSR system held in
+// device memory (cfdl_mgcg.hip).  The solver owns the hierarchy's device
+// arrays and its work vectors; the system's arrays belong to the caller.
+struct MgCgSolver;
+struct MgCgSystem {
+  int ne, nbf, nit;
+  const int *row, *nbr;               // device CSR, neighbours >= ne are halos
+  const double *ap, *anb, *b;          // device coefficients (ne, L, ne)
+  double *phi;                         // device values (ne + nbf), updated on cells
+  char name[8];                        // log label
+  LogLine *log;                        // device log and its count
+  int *nlog;
+};
+int mgcg_create(const MgHier &h, int nbf, int fp32, MgCgSolver **out);
+void mgcg_destroy(MgCgSolver *s);
+// enqueues the solve on stream (a hipStream_t); synchronises with the host
+// once per iteration to stop at the stop rule
+int mgcg_solve(MgCgSolver *s, const MgCgSystem &sys, void *stream);
+// iterations of the last solve and launches it made (diagnostics)
+void mgcg_stats(const MgCgSolver *s, int *launches, int *levels);
+}  // namespace cfdl

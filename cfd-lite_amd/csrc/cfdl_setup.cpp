@@ -801,6 +801,92 @@ extern "C" int cfdl_mesh_box(int nx, int ny, int nz, double lx, double ly, doubl
   return CFDL_OK;
 }
 
+// Cell renumbering for the distorted-channel input (no counterpart in the
+// reference, which keeps the element order of its CGNS file): a seeded
+// uniform permutation of the cells (Fisher-Yates from the last position:
+// position i swaps with j = splitmix64 % (i + 1)), then the reverse
+// Cuthill-McKee order of the interior-face graph of the shuffled mesh.  RCM
+// here: every component starts at its lowest-numbered cell of least degree
+// (interior neighbours); a breadth-first walk appends each visited cell's
+// unvisited neighbours by increasing (degree, number); the whole sequence is
+// then reversed.  Boundary elements keep their order, so the halos keep
+// theirs.
+namespace {
+void rcm_order(int ne, const std::vector<int> &idx, const std::vector<int> &ef2nb,
+               std::vector<int> &order) {
+  std::vector<int> deg(ne, 0);
+  for (int e = 0; e < ne; e++)
+    for (int k = idx[e] - 1; k < idx[e + 1] - 1; k++)
+      if (pk_elem(ef2nb[k]) <= ne) deg[e]++;
+  std::vector<int> start(ne);
+  for (int e = 0; e < ne; e++) start[e] = e;
+  std::stable_sort(start.begin(), start.end(), [&](int a, int b) { return deg[a] < deg[b]; });
+  std::vector<char> seen(ne, 0);
+  order.clear();
+  order.reserve(ne);
+  std::vector<int> nb;
+  for (int s0 : start) {
+    if (seen[s0]) continue;
+    seen[s0] = 1;
+    size_t head = order.size();
+    order.push_back(s0);
+    while (head < order.size()) {
+      const int v = order[head++];
+      nb.clear();
+      for (int k = idx[v] - 1; k < idx[v + 1] - 1; k++) {
+        const int j = pk_elem(ef2nb[k]) - 1;
+        if (j < ne && !seen[j]) nb.push_back(j);
+      }
+      std::sort(nb.begin(), nb.end(),
+                [&](int a, int b) { return deg[a] != deg[b] ? deg[a] < deg[b] : a < b; });
+      for (int j : nb) {
+        seen[j] = 1;
+        order.push_back(j);
+      }
+    }
+  }
+  std::reverse(order.begin(), order.end());
+}
+
+// cell k of the new numbering is cell from[k] of the current one
+int permute_cells(cfdl_mesh *m, const std::vector<int> &from) {
+  const int ne = m->ne;
+  std::vector<int> ev(m->e2vx.begin(), m->e2vx.begin() + 8 * static_cast<size_t>(ne));
+  for (int k = 0; k < ne; k++)
+    std::memcpy(m->e2vx.data() + 8 * static_cast<size_t>(k), ev.data() + 8 * static_cast<size_t>(from[k]),
+                8 * sizeof(int));
+  return mesh_build(m);
+}
+}  // namespace
+
+extern "C" int cfdl_mesh_renumber(cfdl_mesh *m, uint64_t seed) {
+  if (!m || m->ne <= 0) {
+    set_error("cfdl_mesh_renumber: null mesh");
+    return CFDL_EINVAL;
+  }
+  if (m->nsubd != 0) {
+    set_error("cfdl_mesh_renumber: renumber before cfdl_mesh_partition");
+    return CFDL_ESTATE;
+  }
+  if (m->nsec < 1 || m->etype[0] != kHexa8 || m->esec[0] != 1 || m->esec[1] != m->ne) {
+    set_error("cfdl_mesh_renumber: the cells must be the first section");
+    return CFDL_EINVAL;
+  }
+  const int ne = m->ne;
+  std::vector<int> from(ne);
+  for (int k = 0; k < ne; k++) from[k] = k;
+  uint64_t st = seed;
+  for (int i = ne - 1; i >= 1; i--) {
+    const int j = static_cast<int>(splitmix64(st) % static_cast<uint64_t>(i + 1));
+    std::swap(from[i], from[j]);
+  }
+  int rc = permute_cells(m, from);
+  if (rc) return rc;
+  std::vector<int> order;
+  rcm_order(ne, m->ef2nb_idx, m->ef2nb, order);
+  return permute_cells(m, order);
+}
+
 extern "C" int cfdl_mesh_info(const cfdl_mesh *m, int *ne, int *nf, int *nbf, int *nsec,
                               int *nsubd, int *nvx) {
   if (!m) {

@@ -43,7 +43,22 @@ enum cfdl_status {
 enum cfdl_bc_kind {
   CFDL_BC_WALL = 0,     /* dirichlet0, mod_uvwp.f90:493-515 */
   CFDL_BC_LID = 1,      /* lid (u=1), mod_uvwp.f90:548-570 */
-  CFDL_BC_SYMMETRY = 2  /* symmetry (zero_flux), mod_uvwp.f90:517-546 */
+  CFDL_BC_SYMMETRY = 2, /* symmetry (zero_flux), mod_uvwp.f90:517-546 */
+  /* Channel boundaries, no counterpart that runs in the reference (its inlet
+     and outlet setters, mod_uvwp.f90:572-606, are bound nowhere and the BC
+     loops of calc_coef_uvw / calc_coef_p handle only 'dirichlet' and
+     'zero_flux', :254-270, 344-366):
+     INLET: velocity along the inward face normal with the parabolic profile
+       of cfdl_set_inlet_profile at the face centre, p = p(owner), mass flux
+       rho(owner) (v.n) A; momentum: diffusion mu A / ds and upwind
+       convection of that flux (anb += d + max(f, 0), b += f u_P);
+     OUTLET: zero gradient, u, v, w, p = the owner's; mass flux rho (v.n) A
+       from the owner's velocity, then all outlet fluxes scaled so that the
+       outflow equals the inflow (by face area if the extrapolated outflow is
+       not positive); momentum: convection only.
+     The pc system keeps every boundary flux fixed (Neumann). */
+  CFDL_BC_INLET = 3,
+  CFDL_BC_OUTLET = 4
 };
 
 /* fields of uvwp_t (mod_uvwp.f90:6-16) and the shared work arrays of phys_t
@@ -97,6 +112,15 @@ int cfdl_mesh_set_section_names(cfdl_mesh *m, int nsec, const char *const *names
 int cfdl_mesh_partition(cfdl_mesh *m, int nsubd);
 int cfdl_mesh_info(const cfdl_mesh *m, int *ne, int *nf, int *nbf, int *nsec, int *nsubd,
                    int *nvx);
+/* Cell renumbering of the distorted-channel input (no counterpart in the
+   reference, which keeps its file's element order): a seeded random
+   permutation of the cells (Fisher-Yates, position i = ne-1 .. 1 swaps with
+   splitmix64 % (i+1)), then reverse Cuthill-McKee on the interior-face graph
+   (each component from its lowest-numbered least-degree cell, neighbours by
+   increasing degree then number, sequence reversed).  Connectivity and
+   geometry are rebuilt; boundary elements and halos keep their order.  Call
+   before cfdl_mesh_partition. */
+int cfdl_mesh_renumber(cfdl_mesh *m, uint64_t seed);
 /* Copy out one setup array by name: "ef2nb_idx" (ne+1), "ef2nb" (2(2nf-nbf),
    column-major), "s2g" (nf), "bs" (nbf), "esec" (2nsec), "etype" (nsec),
    "gf2g" (ne), "g2gf" (ne), "g2gf_idx" (nsubd+1), "e2vx" (8(ne+nbf));
@@ -104,6 +128,15 @@ int cfdl_mesh_info(const cfdl_mesh *m, int *ne, int *nf, int *nbf, int *nsec, in
 int cfdl_mesh_get_i32(const cfdl_mesh *m, const char *name, int *dst, int64_t n);
 int cfdl_mesh_get_f64(const cfdl_mesh *m, const char *name, double *dst, int64_t n);
 void cfdl_mesh_free(cfdl_mesh *m);
+/* The agglomeration multigrid hierarchy the CFDL_PC_MGCG pc solve builds
+   for this mesh (no counterpart in the reference; see cfdl_set_pc_order):
+   copies level `level`'s array `name` -- "n", "row", "col", "cptr", "cperm",
+   and towards the next level "agg", "mptr", "mem", "iptr", "ilist", "gptr",
+   "glist" -- up to cap values into dst and its length into *count (-1 when
+   the level or name does not exist).  Level 0 is the mesh CSR (0-based,
+   neighbours >= ne are halos). */
+int cfdl_mesh_mg_get(const cfdl_mesh *m, int level, const char *name, int *dst, int64_t cap,
+                     int64_t *count);
 /* VTU file of cell arrays on the mesh, byte for byte the layout the
    reference's write_vtubin / vtu_data writes (src/modules/mod_vtu_output.f90:
    6-326): nvar Float64 cell arrays (names[v], ncomp[v] components, ne*ncomp
@@ -139,6 +172,12 @@ int cfdl_ctx_create(int ne, int nf, int nbf, const int *ef2nb_idx, const int *ef
    mod_uvwp.f90:73-78). */
 int cfdl_ctx_create_cavity(const cfdl_mesh *m, double rho, double mu, double dt, int nit,
                            cfdl_ctx **out);
+/* Convenience: a channel along x from a cfdl_mesh with uniform rho, mu:
+   section 'west' -> CFDL_BC_INLET with u_max = umax over the y-z extent of
+   the mesh's vertices, 'east' -> CFDL_BC_OUTLET, every other section a
+   no-slip wall (dirichlet0). */
+int cfdl_ctx_create_channel(const cfdl_mesh *m, double rho, double mu, double dt, int nit,
+                            double umax, cfdl_ctx **out);
 void cfdl_ctx_destroy(cfdl_ctx *c);
 
 /* Config mode (no counterpart in the reference, which relaxes only through
@@ -167,8 +206,26 @@ int cfdl_set_relaxation(cfdl_ctx *c, double alpha_u, double alpha_p);
    the stop test's norms and the next right-hand side.  Assembly, residuals,
    stop rules and the fields outside the pc solve stay FP64.
    Neither colour order has an effect with one subdomain (the pc solve is
-   then solve_gs on the global system). */
-enum { CFDL_PC_REFERENCE = 0, CFDL_PC_COLORED = 1, CFDL_PC_COLORED_MIXED = 2 };
+   then solve_gs on the global system).
+   CFDL_PC_MGCG and CFDL_PC_MGCG_MIXED (no counterpart in the reference, whose
+   agglomeration levels, src/setup/mod_agglomeration.f90:331-561, no solver
+   uses) replace the subdomain smoother by conjugate gradients on the global
+   system, preconditioned by one V-cycle of agglomeration multigrid (cells
+   grouped by recursive longest-axis bisection of their centres, at most 8^l
+   per level-l aggregate; Galerkin coarse operators; one forward colour
+   Gauss-Seidel sweep before and one backward sweep after each coarse
+   correction).  The stop rule is the reference's (rms residual at most a
+   tenth of the initial one, or nit iterations) and the log line keeps the
+   solve's name.  In CFDL_PC_MGCG_MIXED the V-cycle runs in FP32; CG, the
+   residual, the norms and the stop test stay FP64.  The hierarchy is built
+   on the first call that selects either mode. */
+enum {
+  CFDL_PC_REFERENCE = 0,
+  CFDL_PC_COLORED = 1,
+  CFDL_PC_COLORED_MIXED = 2,
+  CFDL_PC_MGCG = 3,
+  CFDL_PC_MGCG_MIXED = 4
+};
 int cfdl_set_pc_order(cfdl_ctx *c, int order);
 
 /* Visiting order of solve_gs (mod_solver.f90:255-327: the momentum solves,
@@ -188,8 +245,19 @@ int cfdl_set_gs_order(cfdl_ctx *c, int order);
    'zero_flux' -> CFDL_BC_SYMMETRY) for the halos first..last (1-based
    element numbers, ne < first <= last <= ne+nbf). */
 int cfdl_set_properties(cfdl_ctx *c, const double *rho, const double *mu);
+/* dt = +infinity (here or at context creation) is the steady mode (no
+   counterpart in the reference, whose loop is pseudo-transient, main.f90:
+   50-63 with dt = 0.01, mod_physics.f90:15): rho V / dt and the Rhie-Chow
+   transient term (mod_uvwp.f90:233-238, 479-484) are then exactly zero and
+   the outer iterations are steady SIMPLE, which needs momentum
+   under-relaxation (cfdl_set_relaxation with alpha_u < 1; without it
+   cfdl_calc_coef_uvw returns CFDL_ESTATE). */
 int cfdl_set_dt(cfdl_ctx *c, double dt);
 int cfdl_set_bc_kind(cfdl_ctx *c, int first, int last, int kind);
+/* Inlet profile of CFDL_BC_INLET halos (channel extension): normal velocity
+   umax*16*s*(1-s)*t*(1-t) at the face centre, s = (y-y0)/(y1-y0),
+   t = (z-z0)/(z1-z0); default umax = 1 over the unit square. */
+int cfdl_set_inlet_profile(cfdl_ctx *c, double umax, double y0, double y1, double z0, double z1);
 
 /* host <-> device copies of one field (n = element count of that field) */
 int cfdl_set_field(cfdl_ctx *c, int field, const double *src, int64_t n);
@@ -246,6 +314,17 @@ int cfdl_log_get(cfdl_ctx *c, int k, char name[8], int *it, double *res_i, doubl
                  double *res_max);
 int cfdl_log_clear(cfdl_ctx *c);
 
+/* calc_residual (src/modules/mod_solver.f90:230-253) on a system already in
+   device memory, outside any context: r = b + sum(anb * phi_nb) - ap * phi
+   over the cells, *res = sqrt(sum(r^2) / ne) and *res_max = max(r, 0)
+   (signed, as the reference's).  Device arrays: row (ne+1, 0-based CSR),
+   nbr (row[ne] neighbour indices, 0-based; halos >= ne), ap, b (ne), anb
+   (row[ne]), phi (every index nbr names).  res and res_max are host
+   pointers.  stream is a hipStream_t (NULL: the default stream); the call
+   synchronises it.  1 <= ne <= 2^30.  The sum is a fixed-order tree. */
+int cfdl_calc_residual(int ne, const int *row, const int *nbr, const double *ap,
+                       const double *anb, const double *b, const double *phi, double *res,
+                       double *res_max, void *stream);
 /* Launch counters and sampled kernel timing (no counterpart in the
    reference, which has no timers).  cfdl_set_profile(c, k) brackets every
    k-th pressure-correction sweep (all of its launches, back to back) with

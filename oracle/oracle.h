@@ -30,6 +30,11 @@ extern "C" {
 
 enum { OR_HEXA_8 = 17, OR_QUAD_4 = 7 };
 enum { OR_BC_WALL = 0, OR_BC_LID = 1, OR_BC_SYMMETRY = 2 };
+/* channel boundaries (an extension: the reference's inlet/outlet setters,
+   src/equations/mod_uvwp.f90:572-606, are bound nowhere and its assembly
+   cannot use them, mod_uvwp.f90:254-270): a parabolic velocity inlet and a
+   zero-gradient outlet whose mass flux is scaled to the inflow */
+enum { OR_BC_INLET = 3, OR_BC_OUTLET = 4 };
 
 typedef struct {
   int ne, nf, nbf;
@@ -97,6 +102,14 @@ typedef struct or_case {
      visits the cells colour by colour (or_set_gs_order), an extension */
   int gs_order;
   int *gs_perm; /* ne cells (1-based) in visiting order */
+  /* multigrid hierarchy of the pc solve in pc_order 3 and 4 (an extension:
+     multigrid-preconditioned conjugate gradients on the global system, FP64
+     or with an FP32 preconditioner), built on first use */
+  struct or_mg *mg;
+  /* inlet profile (OR_BC_INLET): umax, y0, y1, z0, z1; the halo velocity is
+     umax*16*s*(1-s)*t*(1-t) along the inward normal, s = (y-y0)/(y1-y0),
+     t = (z-z0)/(z1-z0) at the face centre */
+  double inlet[5];
 } or_case;
 
 /* ---- setup ---- */
@@ -136,9 +149,29 @@ void or_solve(or_case *c, const char *name, double *phi);
    reference's order. */
 void or_set_pc_order(or_case *c, int order);
 void or_set_gs_order(or_case *c, int order);
+/* pc_order 3 / 4 (extension, oracle_mgcg.c): the pc solve is multigrid-
+   preconditioned conjugate gradients on the global system, with the
+   preconditioner in FP64 (3) or FP32 (4) */
+typedef struct or_mg or_mg;
+or_mg *or_mg_build(int ne, const int *ef2nb_idx, const int *ef2nb, const double *xc,
+                   const double *yc, const double *zc, int maxagg, int ncoarse);
+void or_mg_free(or_mg *M);
+int or_mg_nlev(const or_mg *M);
+int or_mg_get(const or_mg *M, int l, const char *name, int *dst, int n);
+or_mg *or_case_mg(or_case *c);
+void or_mg_apply(or_case *c, const double *ap, const double *anb, const double *r, double *z,
+                 int fp32);
+int or_mg_operator(or_case *c, const double *ap, const double *anb, int l, double *apl,
+                   double *anbl);
+void or_mgcg_solve(or_case *c, const char *name, double *phi, int fp32);
+double or_tree_dot(const double *a, const double *b, int n);
+double or_tree_sum(const double *a, int n);
+void or_tree_blocks(int n, int *nblk, int *chunk);
+void or_log_line(or_case *c, const char *name, int it, double ri, double rf, double rm);
 
 /* ---- hot path (equation routines mirror mod_uvwp.f90) ---- */
 void or_update_boundaries(or_case *c);
+void or_scale_outlet(or_case *c); /* channel extension, called by or_update_boundaries */
 void or_calc_coef_uvw(or_case *c);
 void or_calc_mip(or_case *c, int rhie_chow);
 void or_calc_coef_p(or_case *c);

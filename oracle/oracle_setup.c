@@ -663,6 +663,7 @@ void or_init_physics(or_case *c, double rho, double mu, double dt, int nit) {
   int ne = c->ne, n = c->ne + c->nbf, nf = c->nf;
   c->dt = dt; c->nit = nit;
   c->alpha_u = 1.0; c->alpha_p = 1.0;
+  c->inlet[0] = 1.0; c->inlet[1] = 0.0; c->inlet[2] = 1.0; c->inlet[3] = 0.0; c->inlet[4] = 1.0;
   c->rho = xcalloc(ne, 8); c->mu = xcalloc(ne, 8);
   for (int e = 0; e < ne; e++) { c->rho[e] = rho; c->mu[e] = mu; }
   double **f1[] = {&c->u, &c->v, &c->w, &c->p, &c->u0, &c->v0, &c->w0, &c->phic};
@@ -691,6 +692,7 @@ void or_case_free(or_case *c) {
   for (size_t k = 0; k < sizeof(ptrs) / sizeof(ptrs[0]); k++) free(ptrs[k]);
   or_set_pc_order(c, 0);
   or_set_gs_order(c, 0);
+  or_mg_free(c->mg);
   if (c->sd) {
     for (int k = 0; k < c->nsubd; k++) {
       or_subd *s = &c->sd[k];

@@ -41,6 +41,12 @@ static void log_line(or_case *c, const char *name, int it, double ri, double rf,
   l->it = it; l->res_i = ri; l->res_f = rf; l->res_max = rm;
 }
 
+static int bc_kind_of_halo(const or_case *c, int enb);
+
+void or_log_line(or_case *c, const char *name, int it, double ri, double rf, double rm) {
+  log_line(c, name, it, ri, rf, rm);
+}
+
 /* vec_weight, src/modules/mod_util.f90:729-743 */
 double or_vec_weight(const double *r0, const double *r1, const double *r2) {
   double ra[3], rb[3];
@@ -495,7 +501,9 @@ static void multi_subdomain_solver(or_case *c, const char *name, double *phi) {
 
 /* solve, src/modules/mod_solver.f90:329-344 */
 void or_solve(or_case *c, const char *name, double *phi) {
-  if (c->nsubd <= 1)
+  if (c->pc_order == 3 || c->pc_order == 4)
+    or_mgcg_solve(c, name, phi, c->pc_order == 4);
+  else if (c->nsubd <= 1)
     or_solve_gs(c, name, phi, c->ap, c->anb, c->b, c->ef2nb_idx, c->ef2nb, c->ne, c->nf, c->nbf,
                 c->nit);
   else
@@ -526,6 +534,31 @@ void or_update_boundaries(or_case *c) {
         c->w[e - 1] = vt[2] - 2.0 * vn[2];
         c->p[e - 1] = c->p[enb - 1];
         c->mip[fg - 1] = 0.0;
+      } else if (kind == OR_BC_INLET || kind == OR_BC_OUTLET) {
+        /* channel extension: inlet velocity along the inward normal with the
+           parabolic profile, outlet velocity and pressure from the owner;
+           the face mass flux rho_P (v.n) A (the outlet's is rescaled below) */
+        int fg = g2[c->ef2nb_idx[enb - 1] + lfnb - 2];
+        const double *a = c->aip + 3 * (fg - 1);
+        double area = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+        double nrm[3] = {a[0] / area, a[1] / area, a[2] / area};
+        double vel[3];
+        if (kind == OR_BC_INLET) {
+          const double *q = c->inlet;
+          double sy = (c->yc[e - 1] - q[1]) / (q[2] - q[1]);
+          double sz = (c->zc[e - 1] - q[3]) / (q[4] - q[3]);
+          double un = q[0] * 16.0 * sy * (1.0 - sy) * sz * (1.0 - sz);
+          for (int i = 0; i < 3; i++) vel[i] = un * -nrm[i];
+        } else {
+          vel[0] = c->u[enb - 1];
+          vel[1] = c->v[enb - 1];
+          vel[2] = c->w[enb - 1];
+        }
+        c->u[e - 1] = vel[0];
+        c->v[e - 1] = vel[1];
+        c->w[e - 1] = vel[2];
+        c->p[e - 1] = c->p[enb - 1];
+        c->mip[fg - 1] = dot3(vel, nrm) * c->rho[enb - 1] * area;
       } else {
         c->u[e - 1] = kind == OR_BC_LID ? 1.0 : 0.0;
         c->v[e - 1] = 0.0;
@@ -536,6 +569,48 @@ void or_update_boundaries(or_case *c) {
       }
     }
   }
+  or_scale_outlet(c);
+}
+
+/* Outlet mass flux of the channel extension: the extrapolated fluxes are
+   scaled so that the outflow equals the inflow, scale = -Q_in / Q_out with
+   both sums (inlet, then outlet halos in halo order) in the blocked tree
+   order of or_tree_sum; if Q_out is not positive the inflow is spread by
+   face area instead, -Q_in * A_f / A_out.  The pc system stays Neumann with
+   zero net boundary flux. */
+void or_scale_outlet(or_case *c) {
+  const int *g2 = c->ef2nb + c->L;
+  int nin = 0, nout = 0;
+  int *fin = malloc(sizeof(int) * (size_t)(c->nbf + 1)), *fout = malloc(sizeof(int) * (size_t)(c->nbf + 1));
+  for (int h = 0; h < c->nbf; h++) {
+    const int kind = bc_kind_of_halo(c, c->ne + 1 + h);
+    if (kind != OR_BC_INLET && kind != OR_BC_OUTLET) continue;
+    const int pk = abs(c->bs[h]);
+    const int fg = g2[c->ef2nb_idx[PK_E(pk) - 1] + PK_F(pk) - 2];
+    if (kind == OR_BC_INLET) fin[nin++] = fg; else fout[nout++] = fg;
+  }
+  if (nout > 0) {
+    double *v = malloc(sizeof(double) * (size_t)(nin > nout ? nin : nout));
+    double *ar = malloc(sizeof(double) * (size_t)nout);
+    for (int k = 0; k < nin; k++) v[k] = c->mip[fin[k] - 1];
+    const double qin = or_tree_sum(v, nin);
+    for (int k = 0; k < nout; k++) {
+      const double *a = c->aip + 3 * (fout[k] - 1);
+      v[k] = c->mip[fout[k] - 1];
+      ar[k] = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    }
+    const double qout = or_tree_sum(v, nout), aout = or_tree_sum(ar, nout);
+    if (qout > 0.0) {
+      const double sc = -qin / qout;
+      for (int k = 0; k < nout; k++) c->mip[fout[k] - 1] = c->mip[fout[k] - 1] * sc;
+    } else {
+      for (int k = 0; k < nout; k++) c->mip[fout[k] - 1] = -qin * ar[k] / aout;
+    }
+    free(v);
+    free(ar);
+  }
+  free(fin);
+  free(fout);
 }
 
 static int bc_kind_of_halo(const or_case *c, int enb) {
@@ -620,7 +695,22 @@ void or_calc_coef_uvw(or_case *c) {
                     c->zc[enb - 1] - c->zc[e - 1]};
     double ds = sqrt(dot3(dr, dr));
     double f = 0.0, d = c->mu[e - 1] * area / ds;
-    if (bc_kind_of_halo(c, enb) != OR_BC_SYMMETRY) {
+    const int kind = bc_kind_of_halo(c, enb);
+    if (kind == OR_BC_INLET || kind == OR_BC_OUTLET) {
+      /* channel extension: the face's convective flux as on interior faces
+         (anb += max(f, 0) upwind, f * u_P into the right-hand side as the
+         cell loop's sumf), diffusion d at the inlet and none at the
+         zero-gradient outlet */
+      const double fl = -c->mip[fg - 1], fnb = fmax(fl, 0.0);
+      const double dd = kind == OR_BC_INLET ? d : 0.0;
+      c->bu[e - 1] = c->bu[e - 1] + fl * c->u[e - 1];
+      c->bv[e - 1] = c->bv[e - 1] + fl * c->v[e - 1];
+      c->bw[e - 1] = c->bw[e - 1] + fl * c->w[e - 1];
+      c->ap[e - 1] = c->ap[e - 1] + dd + fnb;
+      c->anb[idx - 1] = c->anb[idx - 1] + dd + fnb;
+      continue;
+    }
+    if (kind != OR_BC_SYMMETRY) {
       double vb[3] = {c->u[enb - 1], c->v[enb - 1], c->w[enb - 1]};
       double vr[3] = {c->u[e - 1], c->v[e - 1], c->w[e - 1]};
       double t = dot3(vr, nrm);

@@ -15,7 +15,7 @@ import numpy as np
 
 HEXA_8 = 17
 QUAD_4 = 7
-BC_WALL, BC_LID, BC_SYMMETRY = 0, 1, 2
+BC_WALL, BC_LID, BC_SYMMETRY, BC_INLET, BC_OUTLET = 0, 1, 2, 3, 4
 SECTION_NAMES = ["bottom", "south", "east", "north", "west", "top"]
 
 _M64 = (1 << 64) - 1
@@ -105,12 +105,87 @@ def box_mesh(nx, ny=None, nz=None, lx=1.0, ly=1.0, lz=1.0, jitter=0.0, seed=2026
                 ne=ne, nbf=nbf)
 
 
+def cell_graph(mesh):
+    """Interior-face adjacency of the cells (sets of 0-based cell ids): two
+    hexahedra are neighbours when they share the four vertices of a face
+    (CGNS HEXA_8 face order, src/modules/mod_util.f90:78-85)."""
+    ne = mesh["ne"]
+    ev = mesh["e2vx"].reshape(-1, 8)[:ne]
+    faces = ((0, 3, 2, 1), (0, 1, 5, 4), (1, 2, 6, 5), (2, 3, 7, 6), (0, 4, 7, 3), (4, 5, 6, 7))
+    owner = {}
+    adj = [set() for _ in range(ne)]
+    for e in range(ne):
+        for f in faces:
+            key = tuple(sorted(int(ev[e, q]) for q in f))
+            o = owner.get(key)
+            if o is None:
+                owner[key] = e
+            else:
+                adj[e].add(o)
+                adj[o].add(e)
+    return adj
+
+
+def rcm(adj):
+    """Reverse Cuthill-McKee: each component from its lowest-numbered cell of
+    least degree, a breadth-first walk appending each cell's unvisited
+    neighbours by increasing (degree, number), the sequence reversed."""
+    n = len(adj)
+    deg = [len(a) for a in adj]
+    seen = [False] * n
+    order = []
+    for s in sorted(range(n), key=lambda i: (deg[i], i)):
+        if seen[s]:
+            continue
+        seen[s] = True
+        head = len(order)
+        order.append(s)
+        while head < len(order):
+            v = order[head]
+            head += 1
+            for j in sorted((j for j in adj[v] if not seen[j]), key=lambda j: (deg[j], j)):
+                seen[j] = True
+                order.append(j)
+    return order[::-1]
+
+
+def renumber(mesh, seed):
+    """The cell renumbering of the distorted-channel input: a seeded
+    permutation (Fisher-Yates from the last position, position i swapping
+    with splitmix64 % (i + 1)), then RCM of the shuffled mesh's cells.
+    Boundary elements keep their order.  Returns a new mesh dict."""
+    ne = mesh["ne"]
+    ev = mesh["e2vx"].reshape(-1, 8).copy()
+    frm = list(range(ne))
+    r = splitmix64(seed, max(ne - 1, 0))
+    for t, i in enumerate(range(ne - 1, 0, -1)):
+        j = int(r[t] % np.uint64(i + 1))
+        frm[i], frm[j] = frm[j], frm[i]
+    ev[:ne] = ev[:ne][frm]
+    out = dict(mesh)
+    out["e2vx"] = ev.reshape(-1).copy()
+    order = rcm(cell_graph(out))
+    ev2 = ev.copy()
+    ev2[:ne] = ev[:ne][order]
+    out["e2vx"] = ev2.reshape(-1)
+    return out
+
+
 def cavity_bc(mesh, symmetry=()):
     """Lid-driven cavity binding, src/equations/mod_uvwp.f90:73-78; the
     sections named in symmetry take the symmetry setter (:517-546) instead."""
     kinds = [-1]
     for name in mesh["names"][1:]:
         kinds.append(BC_SYMMETRY if name in symmetry else BC_LID if name == "top" else BC_WALL)
+    return np.array(kinds, np.int32)
+
+
+def channel_bc(mesh):
+    """Channel along x (an extension): inlet on west (x-), outlet on east
+    (x+), no-slip walls on the other sections."""
+    kinds = [-1]
+    for name in mesh["names"][1:]:
+        kinds.append(BC_INLET if name == "west" else BC_OUTLET if name == "east" else BC_WALL)
     return np.array(kinds, np.int32)
 
 

@@ -50,7 +50,8 @@ class OrCase(C.Structure):
                 ("nlog", C.c_int), ("caplog", C.c_int), ("log", C.POINTER(OrLog)),
                 ("cp", PD), ("tc", PD), ("t", PD), ("gt", PD), ("h", PD), ("h0", PD), ("gh", PD),
                 ("pc_order", C.c_int), ("pc_perm", C.POINTER(PI)),
-                ("gs_order", C.c_int), ("gs_perm", PI)]
+                ("gs_order", C.c_int), ("gs_perm", PI), ("mg", C.c_void_p),
+                ("inlet", C.c_double * 5)]
 
 
 _lib = None
@@ -162,7 +163,8 @@ class OracleCase:
         """Colour order of the subdomain pc smoother (or_set_pc_order)."""
         L = lib()
         L.or_set_pc_order.argtypes = [C.c_void_p, C.c_int]
-        L.or_set_pc_order(C.cast(self.p, C.c_void_p), {"reference": 0, "colored": 1, "mixed": 2}[order])
+        L.or_set_pc_order(C.cast(self.p, C.c_void_p),
+                          {"reference": 0, "colored": 1, "mixed": 2, "mgcg": 3, "mgcg-mixed": 4}[order])
 
     def set_gs_order(self, order="reference"):
         """Colour order of solve_gs (or_set_gs_order)."""
@@ -178,6 +180,10 @@ class OracleCase:
         """Visiting order (1-based local cells) of subdomain k's pc smoother
         under the colour order."""
         return np.ctypeslib.as_array(self.c.pc_perm[k], shape=(self.c.sd[k].ne,)).copy()
+
+    def set_inlet_profile(self, umax=1.0, y0=0.0, y1=1.0, z0=0.0, z1=1.0):
+        for k, v in enumerate((umax, y0, y1, z0, z1)):
+            self.c.inlet[k] = v
 
     def set_relaxation(self, alpha_u=1.0, alpha_p=1.0):
         self.c.alpha_u = alpha_u

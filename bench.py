@@ -1,14 +1,15 @@
 """SIMPLE throughput on one MI355X: cell-iterations per second.
 
-Workload (the GPU FP64 configuration): lid-driven cavity on a uniform
-128x128x128 hex box (2,097,152 cells, unstructured face lists), Re=400
-(rho=1, mu=0.0025), lid u=1, fields start at zero; the reference SIMPLE(C)
-outer iteration (solve_uvwp) with its exact-order Gauss-Seidel solvers and
-4-subdomain pressure-correction solver, FP64, pseudo-time step dt=0.01
-with 3 outer iterations per step, plus config-mode under-relaxation 0.7 (u)
-and 0.3 (p).  One step = one outer iteration (update_boundaries +
-solve_uvwp; update_time after every third).  Inputs are resident in HBM when
-the timed region starts.
+Headline workload (the GPU FP64 configuration): lid-driven cavity on a
+uniform 128x128x128 hex box (2,097,152 cells, unstructured face lists),
+Re=400 (rho=1, mu=0.0025), lid u=1, fields start at zero; steady SIMPLE(C)
+(dt = infinity: no pseudo-time term, the loop that converges to the 1e-6
+scaled residual) with under-relaxation 0.7 (u) and 0.3 (p), the reference's
+exact-order Gauss-Seidel solvers and 4-subdomain pressure-correction solver,
+FP64.  One step = one outer iteration (update_boundaries + solve_uvwp).
+Inputs are resident in HBM when the timed region starts.  The timed steps
+are a prefix of the run (residual_target_met false); the "converge" block
+holds runs to the 1e-6 scaled residual.
 
 Prints one JSON line.  value = 2,097,152 * K / seconds for K timed steps.
 roofline: the pressure-correction Gauss-Seidel sweep kernel k_gs_tiles_lds (the dominant
@@ -23,7 +24,16 @@ device copy gives), reported for that workload only; --traffic-kib gives
 another measured figure.
 cpu_baseline: the reference Fortran (flang -O2 build in oracle/_ref) on one
 host core, 64^3 sample (see --cpu-*), or the oracle's C restatement when the
-reference build is absent.
+reference build is absent; "port" beside it: the oracle's restatement of the
+headline algorithm itself (steady, relaxed) on one core, 64^3 sample.
+Extras in the same line (--no-extras skips them): "converge" (runs to the
+1e-6 scaled residual: the 32^3 Re=100 cavity in the reference order and in
+the multigrid mode, the 128^3 workload in the multigrid mode), "modes_128"
+and "modes_256" (the extension solvers on the 128^3 workload and on the
+256^3 Re=1000 cavity: cell-iterations/s, whole-iteration HBM fraction from
+the algorithmic bytes of every kernel, field error against the FP64 runs),
+"channel_128" (the distorted channel: 128^3 jittered, renumbered and RCM
+ordered, Re=200).
 """
 import argparse
 import json
@@ -41,6 +51,52 @@ sys.path.insert(0, os.path.join(REPO, "cfd-lite_amd"))
 METRIC = "SIMPLE cell-iters/s at 1e-6 residual; batched solves/s + p50/p99 latency"
 BYTES_PER_CELL_SWEEP = 108.0
 HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per cell of one outer iteration (each array element a
+# kernel touches counted once; FP64 8 B, FP32 and int32 4 B; hexahedral
+# interior cell): 2108 B for assembly, gradients, face fluxes and the update,
+# 100 + 316 n per momentum solve of n Gauss-Seidel iterations, and for the
+# pc solve of n iterations:
+#   reference / colour order: 100 + 226 n  (two FP64 sweeps of 108 B and a
+#     residual every tenth iteration)
+#   "mixed" (FP32 colour sweeps): 184 + 146 n  (FP32 copies of ap and anb,
+#     two 68 B sweeps, an FP64 residual every tenth iteration)
+#   "mgcg" (FP64 V-cycle): 190 + 580 n  (initial residual and the Galerkin
+#     sums; per CG iteration a forward and a backward colour sweep of 108 B,
+#     the restriction 104 B and prolongation 20 B on the mesh, 8/7 of that
+#     over the coarse levels, plus 196 B of CG vectors and the 100 B product)
+#   "mgcg-mixed" (FP32 V-cycle): 274 + 442 n
+BYTES_FIXED = 2108.0
+PC_BYTES = {"reference": (100.0, 226.0), "colored": (100.0, 226.0), "mixed": (184.0, 146.0),
+            "mgcg": (190.0, 580.0), "mgcg-mixed": (274.0, 442.0)}
+
+
+def iteration_bytes(log, pc_order):
+    """Algorithmic bytes per cell of the outer iterations in log (one line
+    per linear solve, u v w pc per iteration): the sum over them."""
+    b0, b1 = PC_BYTES[pc_order]
+    total = 0.0
+    for name, it, ri, rf, rm in log:
+        if name == "pc":
+            total += BYTES_FIXED + b0 + b1 * it
+        else:
+            total += 100.0 + 316.0 * it
+    return total
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def progress(msg):
+    print("bench: " + msg, file=sys.stderr, flush=True)
 
 
 def scaled_residual(log):
@@ -183,6 +239,210 @@ def cpu_baseline(n, nouter, mu):
                       "restatement on one host core; %.1f s" % (n, 1.0 / mu, nouter, secs)}
 
 
+def port_baseline(n, nouter, mu):
+    """The oracle's C restatement of the headline algorithm itself (steady,
+    relaxation 0.7 / 0.3, reference-order solvers) on one host core, on an
+    n^3 sample: the like-for-like CPU rate beside the reference's own."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import oracle_lib  # test infrastructure: the oracle as checker/baseline only
+    import cfdl
+    mesh = cfdl.Mesh.box(n)
+    g = {"nvx": mesh.info()["nvx"], "x": mesh.get("x"), "y": mesh.get("y"), "z": mesh.get("z"),
+         "etype": mesh.get("etype"), "esec": mesh.get("esec"), "e2vx": mesh.get("e2vx")}
+    kind = np.array([-1, 0, 0, 0, 0, 0, 1], np.int32)
+    oc = oracle_lib.OracleCase(g, kind, nsubd=4, rho=1.0, mu=mu, dt=float("inf"))
+    oc.set_relaxation(0.7, 0.3)
+    oc.run(1, 1)
+    t = time.perf_counter()
+    oc.run(1, nouter)
+    secs = time.perf_counter() - t
+    return {"value": n ** 3 * nouter / secs, "unit": "cell-iterations/s", "cores": 1,
+            "kind": "port", "cpu": cpu_model(),
+            "sample": "%d^3 lid-driven cavity, Re=%g, steady, relaxation 0.7/0.3, %d outer "
+                      "iterations of the oracle's C restatement (the headline algorithm, "
+                      "reference-order solvers) on one host core; %.1f s" % (n, 1.0 / mu, nouter, secs)}
+
+
+def run_mode(cfdl, mesh, mu, order, gs, steps, warmup, channel=False, keep=False):
+    """Steady outer iterations of one solver mode from rest on a fresh
+    context: warmup untimed, then steps timed.  Returns the rate, the whole
+    iteration's HBM fraction (algorithmic bytes of every kernel over the wall
+    time) and, with keep, the fields."""
+    ne = mesh.info()["ne"]
+    mk = cfdl.UVWP.channel if channel else cfdl.UVWP.cavity
+    eq = mk(mesh, rho=1.0, mu=mu, dt=float("inf"))
+    eq.set_relaxation(0.7, 0.3)
+    eq.set_pc_order(order)
+    eq.set_gs_order(gs)
+    for _ in range(warmup):
+        eq.update_boundaries()
+        eq.solve_uvwp()
+    eq.clear_log()
+    eq.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        eq.update_boundaries()
+        eq.solve_uvwp()
+    eq.synchronize()
+    secs = time.perf_counter() - t
+    log = eq.log()
+    nbytes = ne * iteration_bytes(log, order)
+    out = {"pc_order": order, "gs_order": gs, "value": round(ne * steps / secs, 1),
+           "unit": "cell-iterations/s", "ms_per_step": round(1e3 * secs / steps, 3),
+           "steps": steps, "warmup": warmup,
+           "hbm_frac_whole_iteration": round(nbytes / secs / (HBM_PEAK_GBS * 1e9), 4),
+           "pc_iterations": [l[1] for l in log if l[0] == "pc"][-3:],
+           "dtype": "f32 pc preconditioner / sweeps, f64 otherwise" if "mixed" in order else "f64"}
+    if keep:
+        out["fields"] = {k: eq.get(k)[:ne] for k in ("u", "v", "w", "p")}
+    eq.close()
+    return out
+
+
+def field_error(a, b):
+    """Relative L2 difference per field (velocities against the velocity
+    norm, p against its own)."""
+    vel = np.linalg.norm(np.concatenate([b[k] for k in ("u", "v", "w")]))
+    return {k: float(np.linalg.norm(a[k] - b[k]) / (vel if k != "p" else np.linalg.norm(b[k])))
+            for k in ("u", "v", "w", "p")}
+
+
+def converge(cfdl, mesh, mu, order, gs, max_outer, tol, channel=False, keep=False):
+    """Steady outer iterations from rest to the scaled residual tol or
+    max_outer iterations (the configurations' criterion: 1e-6 or 500)."""
+    ne = mesh.info()["ne"]
+    mk = cfdl.UVWP.channel if channel else cfdl.UVWP.cavity
+    eq = mk(mesh, rho=1.0, mu=mu, dt=float("inf"))
+    eq.set_relaxation(0.7, 0.3)
+    eq.set_pc_order(order)
+    eq.set_gs_order(gs)
+    k, secs, worst = run_to_convergence(eq, ne, 1 << 30, max_outer, tol)
+    out = {"pc_order": order, "gs_order": gs, "outer_iterations": k, "seconds": round(secs, 3),
+           "converged": bool(worst <= tol), "scaled_residual": worst, "tol": tol,
+           "cell_iterations_per_s": round(ne * k / secs, 1)}
+    if keep:
+        out["fields"] = {f: eq.get(f)[:ne] for f in ("u", "v", "w", "p")}
+    eq.close()
+    return out
+
+
+def same_residual(cfdl, mesh, mu, plain, fast_modes, max_outer):
+    """The fast modes from rest to the scaled residual the plain FP64 run
+    reached (within max_outer): their time to that residual against the
+    plain run's."""
+    r = {}
+    for order, gs in fast_modes:
+        f = converge(cfdl, mesh, mu, order, gs, max_outer, max(plain["scaled_residual"], 1e-12))
+        if f["converged"] and f["seconds"] > 0:
+            f["time_ratio_vs_plain_fp64"] = round(plain["seconds"] / f["seconds"], 2)
+        r[order + "/" + gs] = f
+    return r
+
+
+def extras(cfdl, args, base_mesh, headline):
+    """The extension modes and the runs to the 1e-6 residual (see the module
+    docstring); each section records its error instead of failing the line."""
+    out = {}
+
+    def section(name, fn):
+        progress("section %s" % name)
+        t = time.perf_counter()
+        try:
+            out[name] = fn()
+        except Exception as e:  # noqa: BLE001 - recorded in the line
+            out[name] = {"error": "%s: %s" % (type(e).__name__, e)}
+        progress("section %s done in %.1f s" % (name, time.perf_counter() - t))
+
+    def conv32():
+        m = cfdl.Mesh.box(32).partition(4)
+        # to 1e-6 (past the configuration's cap of 500, which stops it at
+        # about 5e-6) for the time to solution
+        ref = converge(cfdl, m, 0.01, "reference", "reference", 1000, args.tol, keep=True)
+        fast = converge(cfdl, m, 0.01, "mgcg-mixed", "colored", 1000, args.tol, keep=True)
+        fast["rel_l2_vs_fp64_reference_order_converged"] = field_error(fast.pop("fields"),
+                                                                       ref.pop("fields"))
+        r = {"case": "32^3 cavity, Re=100, steady, relaxation 0.7/0.3 (the CPU configuration), "
+                     "to the 1e-6 scaled residual",
+             "fp64_reference_order": ref, "mgcg_mixed": fast}
+        if ref["converged"] and fast["converged"]:
+            r["time_to_solution_ratio"] = round(ref["seconds"] / fast["seconds"], 2)
+        # the oracle is bitwise the reference-order GPU path, so it needs the
+        # same outer iterations: its time to solution from a timed sample
+        sys.path.insert(0, os.path.join(REPO, "tests"))
+        import oracle_lib  # test infrastructure: the CPU rate only
+        g = {"nvx": m.info()["nvx"], "x": m.get("x"), "y": m.get("y"), "z": m.get("z"),
+             "etype": m.get("etype"), "esec": m.get("esec"), "e2vx": m.get("e2vx")}
+        oc = oracle_lib.OracleCase(g, np.array([-1, 0, 0, 0, 0, 0, 1], np.int32), nsubd=4,
+                                   rho=1.0, mu=0.01, dt=float("inf"))
+        oc.set_relaxation(0.7, 0.3)
+        t = time.perf_counter()
+        oc.run(1, 20)
+        per = (time.perf_counter() - t) / 20
+        r["cpu_port_seconds_to_solution"] = round(per * ref["outer_iterations"], 2)
+        r["cpu_port_seconds_per_iteration"] = round(per, 4)
+        return r
+
+    def modes128():
+        r = {"case": "128^3 cavity, Re=%g, steady, relaxation 0.7/0.3, 10 outer iterations "
+                     "after 2" % args.re,
+             "reference": {"value": headline["value"], "hbm_frac_whole_iteration":
+                           headline["hbm_frac_whole_iteration"]}}
+        for order in ("colored", "mixed", "mgcg", "mgcg-mixed"):
+            r[order] = run_mode(cfdl, base_mesh, 1.0 / args.re, order, "colored", 10, 2)
+        return r
+
+    def conv128():
+        mu = 1.0 / args.re
+        plain = converge(cfdl, base_mesh, mu, "reference", "reference", 500, args.tol)
+        progress("128^3 plain FP64: %d outer iterations, scaled residual %.3e, %.1f s"
+                 % (plain["outer_iterations"], plain["scaled_residual"], plain["seconds"]))
+        return {"case": "128^3 cavity, Re=%g, steady, relaxation 0.7/0.3, 1e-6 or 500 outer "
+                        "iterations (the configuration's criterion); the fast modes run to the "
+                        "residual the plain FP64 path reached" % args.re,
+                "plain_fp64_reference_order": plain,
+                "same_residual": same_residual(cfdl, base_mesh, mu, plain,
+                                               (("mixed", "colored"), ("mgcg-mixed", "colored")),
+                                               2000)}
+
+    def modes256():
+        m = cfdl.Mesh.box(256).partition(4)
+        K = 4
+        runs = {}
+        for order, gs in (("reference", "reference"), ("colored", "colored"),
+                          ("mixed", "colored"), ("mgcg", "colored"), ("mgcg-mixed", "colored")):
+            runs[order] = run_mode(cfdl, m, 1e-3, order, gs, K, 0, keep=True)
+            progress("256^3 %s: %.1f M cell-iterations/s" % (order, runs[order]["value"] / 1e6))
+        for order in ("colored", "mixed", "mgcg", "mgcg-mixed"):
+            runs[order]["rel_l2_vs_fp64_reference_order"] = field_error(
+                runs[order]["fields"], runs["reference"]["fields"])
+        for order, ref in (("mixed", "colored"), ("mgcg-mixed", "mgcg")):
+            runs[order]["rel_l2_vs_fp64_same_solver"] = field_error(
+                runs[order]["fields"], runs[ref]["fields"])
+        for v in runs.values():
+            v.pop("fields")
+        base = runs["reference"]["value"]
+        for order in runs:
+            runs[order]["x_fp64_reference_order"] = round(runs[order]["value"] / base, 2)
+        # the mixed-precision configuration's run: 1e-6 or 500 outer iterations
+        runs["mixed_500"] = converge(cfdl, m, 1e-3, "mixed", "colored", 500, args.tol)
+        return dict(case="256^3 cavity, Re=1000, steady, relaxation 0.7/0.3, %d outer "
+                         "iterations from rest (mixed_500: to 1e-6 or 500)" % K, **runs)
+
+    def channel128():
+        m = cfdl.Mesh.box(128, jitter=0.2, seed=20261202).renumber(20261202).partition(4)
+        r = run_mode(cfdl, m, 1.0 / 200.0, "mgcg-mixed", "colored", 10, 2, channel=True)
+        return dict(case="128^3 channel, interior vertices jittered +-0.2h, cells shuffled then "
+                         "RCM ordered, parabolic inlet u_max=1, zero-gradient outlet, Re=200, "
+                         "steady", **r)
+
+    section("converge_cavity32", conv32)
+    section("modes_128", modes128)
+    section("converge_cavity128", conv128)
+    section("modes_256", modes256)
+    section("channel_128", channel128)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -194,31 +454,28 @@ def main():
     ap.add_argument("--alpha-p", type=float, default=0.3)
     ap.add_argument("--cpu-n", type=int, default=64)
     ap.add_argument("--cpu-iters", type=int, default=12)
+    ap.add_argument("--port-iters", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="only the headline line (no converge / modes / channel blocks)")
     ap.add_argument("--sample-every", type=int, default=7, help="time every k-th pc sweep")
-    ap.add_argument("--tstep-every", type=int, default=3,
-                    help="outer iterations per pseudo-time step (update_time after each group)")
+    ap.add_argument("--pseudo-transient", action="store_true",
+                    help="the reference's pseudo-time loop (dt=0.01, update_time after every "
+                         "--tstep-every outer iterations) instead of steady SIMPLE")
+    ap.add_argument("--tstep-every", type=int, default=3)
     ap.add_argument("--converge", action="store_true",
-                    help="run from rest to a 1e-6 scaled residual (or --max-outer) instead")
+                    help="only run from rest to a 1e-6 scaled residual (or --max-outer)")
     ap.add_argument("--max-outer", type=int, default=500)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--steady", action="store_true",
-                    help="dt = infinity: steady SIMPLE (no pseudo-time term), the mode that "
-                         "converges to the 1e-6 scaled residual")
+                    help="with --converge: dt = infinity (the default elsewhere)")
     ap.add_argument("--pc-order", choices=("reference", "colored", "mixed", "mgcg", "mgcg-mixed"),
                     default="reference",
-                    help="visiting order of the pressure-correction smoother; 'colored' is an "
-                         "extension (greedy colouring, one launch per colour) reported on its "
-                         "own line, not as the headline value; 'mixed' also runs the colour "
-                         "order with FP32 sweeps under FP64 defect correction and reports its "
-                         "field error against the FP64 colour order")
+                    help="pc solver of --converge runs")
     ap.add_argument("--gs-order", choices=("reference", "colored"), default="reference",
-                    help="visiting order of solve_gs (the momentum solves) in the --pc-order "
-                         "extension lines; the headline value always keeps the reference order")
+                    help="order of solve_gs of --converge runs")
     ap.add_argument("--serving", action="store_true",
-                    help="also run a serving stream (Poisson arrivals of independent cavity "
-                         "solves of edge 16..96 and Re 100..1000, each the reference's 10 x 3 "
-                         "outer iterations) and report solves/s with p50/p99 latency")
+                    help="also run a serving stream and report solves/s with p50/p99 latency")
     ap.add_argument("--serving-requests", type=int, default=512)
     ap.add_argument("--serving-rate", type=float, default=10.0, help="arrivals per second")
     ap.add_argument("--serving-slots", type=int, default=8, help="requests in flight")
@@ -238,10 +495,12 @@ def main():
     mu = 1.0 / args.re
     t0 = time.perf_counter()
     mesh = cfdl.Mesh.box(args.n).partition(4)
-    eq = cfdl.UVWP.cavity(mesh, rho=1.0, mu=mu, dt=0.01, nit=100)
+    steady = not args.pseudo_transient
+    dt = float("inf") if steady else 0.01
+    eq = cfdl.UVWP.cavity(mesh, rho=1.0, mu=mu, dt=dt, nit=100)
     eq.set_relaxation(args.alpha_u, args.alpha_p)
     ne = mesh.info()["ne"]
-    print("bench: setup %.1f s, %d cells" % (time.perf_counter() - t0, ne), file=sys.stderr, flush=True)
+    progress("setup %.1f s, %d cells" % (time.perf_counter() - t0, ne))
 
     if args.converge:
         if args.steady:
@@ -251,6 +510,7 @@ def main():
         k, secs, worst = run_to_convergence(eq, ne, args.tstep_every, args.max_outer, args.tol)
         print(json.dumps({
             "metric": METRIC, "mode": "converge", "pc_order": args.pc_order,
+            "gs_order": args.gs_order, "steady": bool(args.steady or steady),
             "value": round(ne * k / secs, 1),
             "unit": "cell-iterations/s", "outer_iterations": k, "seconds": round(secs, 3),
             "converged": worst <= args.tol, "scaled_residual": worst, "tol": args.tol,
@@ -266,7 +526,7 @@ def main():
         eq.update_boundaries()
         eq.solve_uvwp()
         step_no[0] += 1
-        if step_no[0] % args.tstep_every == 0:
+        if not steady and step_no[0] % args.tstep_every == 0:
             eq.update_time()
 
     if args.pcie:
@@ -275,14 +535,14 @@ def main():
     for _ in range(args.warmup):
         step()
     eq.synchronize()
-    print("bench: warmup done", file=sys.stderr, flush=True)
+    progress("warmup done")
     eq.reset_stats()
     eq.set_profile(args.sample_every)
     eq.synchronize()
     t = time.perf_counter()
     for k in range(args.steps):
         step()
-        print("bench: step %d queued" % (k + 1), file=sys.stderr, flush=True)
+        progress("step %d queued" % (k + 1))
     eq.synchronize()
     secs = time.perf_counter() - t
     eq.set_profile(0)
@@ -304,33 +564,45 @@ def main():
                 "kernel": "k_gs_tiles_lds<true, true, 1> (pressure-correction Gauss-Seidel, space-time tiles in LDS)",
                 "launch_ms": round(kernel_ms, 5), "cells_per_launch": round(cells, 1),
                 "sampled_launches": int(st["sampled_launches"])}
-    base = None if args.no_cpu else cpu_baseline(args.cpu_n, args.cpu_iters, mu)
+    hbm_whole = ne * iteration_bytes(log[-4 * args.steps:], "reference") / secs / (HBM_PEAK_GBS * 1e9)
+    progress("pc level launches %d (%.0f cells each), %d levels per symmetric sweep; "
+             "outer iteration %.1f ms" % (st["pc_launches"], st["pc_cells"] / max(st["pc_launches"], 1),
+                                          st["pc_levels_per_sym_sweep"], 1e3 * secs / args.steps))
+    base = None
+    if not args.no_cpu:
+        base = cpu_baseline(args.cpu_n, args.cpu_iters, mu)
+        base["cpu"] = cpu_model()
+        progress("cpu baseline %.3g cell-iterations/s" % base["value"])
+        try:
+            base["port_like_for_like"] = port_baseline(args.cpu_n, args.port_iters, mu)
+        except Exception as e:  # noqa: BLE001 - recorded in the line
+            base["port_like_for_like"] = {"error": str(e)}
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "cell-iterations/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * secs / args.steps, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "lid-driven cavity %d^3 hex (%d cells), Re=%g, FP64 SIMPLE(C) with "
-                               "exact-order Gauss-Seidel and 4-subdomain pc solver, "
-                               "under-relaxation %g/%g" % (args.n, ne, args.re, args.alpha_u,
-                                                          args.alpha_p),
-                   "mesh": "%dx%dx%d" % (args.n, args.n, args.n), "subdomains": 4, "dt": 0.01,
+        "config": {"workload": "lid-driven cavity %d^3 hex (%d cells), Re=%g, %s FP64 SIMPLE(C) "
+                               "with exact-order Gauss-Seidel and 4-subdomain pc solver, "
+                               "under-relaxation %g/%g" % (
+                                   args.n, ne, args.re, "steady" if steady else "pseudo-transient",
+                                   args.alpha_u, args.alpha_p),
+                   "mesh": "%dx%dx%d" % (args.n, args.n, args.n), "subdomains": 4,
+                   "dt": "inf" if steady else 0.01,
                    "outer_iterations_timed": args.steps,
-                   "scaled_residual_reached": scaled_residual(log)},
+                   "scaled_residual_reached": scaled_residual(log),
+                   "residual_target_met": False,
+                   "hbm_frac_whole_iteration": round(hbm_whole, 4)},
         "roofline": roofline,
         "cpu_baseline": base,
     }
+    if not args.no_extras:
+        eq.close()
+        out["extras"] = extras(cfdl, args, mesh, {"value": out["value"],
+                                                  "hbm_frac_whole_iteration": round(hbm_whole, 4)})
     print(json.dumps(out), flush=True)
-    print("bench: pc level launches %d (%.0f cells each), %d levels per symmetric sweep; "
-          "outer iteration %.1f ms" % (st["pc_launches"], st["pc_cells"] / max(st["pc_launches"], 1),
-                                       st["pc_levels_per_sym_sweep"], 1e3 * secs / args.steps),
-          file=sys.stderr)
     if args.serving:
         serving_line(args)
-    if args.pc_order in ("colored", "mixed"):
-        ref = colored_rate(cfdl, mesh, mu, ne, args, "colored")
-    if args.pc_order == "mixed":
-        colored_rate(cfdl, mesh, mu, ne, args, "mixed", ref)
 
 
 def serving_line(args):
@@ -353,45 +625,6 @@ def serving_line(args):
                               "[100, 1000], 10 x 3 outer iterations from rest, 4 subdomains",
                       "pc_order": "mixed", "gs_order": "colored",
                       "dtype": "f32 pc sweeps, f64 otherwise", "data": "synthetic"}), flush=True)
-
-
-def colored_rate(cfdl, mesh, mu, ne, args, order, fp64=None):
-    """The same outer iterations with a colour-order pressure-correction
-    smoother (extensions; the headline value keeps the reference order),
-    from rest on a fresh context.  Printed as its own JSON line.  With fp64
-    (the FP64 colour order's fields after the same iterations) the line also
-    gives each field's relative L2 difference from them.  Returns the
-    fields."""
-    eq = cfdl.UVWP.cavity(mesh, rho=1.0, mu=mu, dt=0.01, nit=100)
-    eq.set_relaxation(args.alpha_u, args.alpha_p)
-    eq.set_pc_order(order)
-    eq.set_gs_order(args.gs_order)
-    n = 0
-    for k in range(args.warmup + args.steps):
-        if k == args.warmup:
-            eq.synchronize()
-            t = time.perf_counter()
-        eq.update_boundaries()
-        eq.solve_uvwp()
-        n += 1
-        if n % args.tstep_every == 0:
-            eq.update_time()
-    eq.synchronize()
-    secs = time.perf_counter() - t
-    fields = {k: eq.get(k)[:ne] for k in ("u", "v", "w", "p")}
-    mode = "pc-" + order + ("/gs-colored" if args.gs_order == "colored" else "")
-    line = {"mode": mode, "metric": METRIC, "value": round(ne * args.steps / secs, 1),
-            "unit": "cell-iterations/s", "ms_per_step": round(1e3 * secs / args.steps, 3),
-            "steps": args.steps, "outer_iterations_from_rest": args.warmup + args.steps,
-            "dtype": "f32 pc sweeps, f64 otherwise" if order == "mixed" else "f64",
-            "scaled_residual_reached": scaled_residual(eq.log())}
-    if fp64 is not None:
-        vel = np.linalg.norm(np.concatenate([fp64[k] for k in ("u", "v", "w")]))
-        line["rel_l2_vs_fp64_colored"] = {
-            k: float(np.linalg.norm(fields[k] - fp64[k]) /
-                     (vel if k != "p" else np.linalg.norm(fp64[k]))) for k in fields}
-    print(json.dumps(line), flush=True)
-    return fields
 
 
 if __name__ == "__main__":

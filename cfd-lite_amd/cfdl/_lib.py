@@ -50,6 +50,7 @@ _SIGS = {
     "cfdl_ctx_destroy": (None, [VP]),
     "cfdl_set_relaxation": (C.c_int, [VP, C.c_double, C.c_double]),
     "cfdl_set_pc_order": (C.c_int, [VP, C.c_int]),
+    "cfdl_set_assembly_precision": (C.c_int, [VP, C.c_int]),
     "cfdl_set_gs_order": (C.c_int, [VP, C.c_int]),
     "cfdl_set_properties": (C.c_int, [VP, PD, PD]),
     "cfdl_set_dt": (C.c_int, [VP, C.c_double]),

@@ -110,6 +110,11 @@ class UVWP:
             raise CfdlError("pc order must be one of %s" % sorted(self.PC_ORDERS))
         check(self._lib.cfdl_set_pc_order(self._h, self.PC_ORDERS[order]))
 
+    def set_assembly_precision(self, bits=64):
+        """64 (the reference's FP64 assembly) or 32 (FP32 coefficients under
+        the FP64 outer loop; an extension)."""
+        check(self._lib.cfdl_set_assembly_precision(self._h, bits))
+
     GS_ORDERS = {"reference": 0, "colored": 1}
 
     def set_gs_order(self, order="reference"):

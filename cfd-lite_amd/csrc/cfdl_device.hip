@@ -98,7 +98,26 @@ struct MeshDev {
   const int *row, *nbr, *fsg, *own, *fcell, *bcrow, *bcslot, *hcell, *hslot, *hkind;
   const double *xc, *yc, *zc, *vol, *aip, *rip, *rho, *mu;
   const double *cp, *tc;  // thermal properties of the energy equation
+  // FP32 copies of the geometry and properties for the FP32 assembly (null
+  // until cfdl_set_assembly_precision selects it)
+  const float *xc32, *yc32, *zc32, *vol32, *aip32, *rip32, *rho32, *mu32;
 };
+
+// the geometry of the arithmetic type T: the mesh's FP64 arrays, or their
+// FP32 copies
+struct Geo64 {
+  const double *xc, *yc, *zc, *vol, *aip, *rip, *rho, *mu;
+};
+struct Geo32 {
+  const float *xc, *yc, *zc, *vol, *aip, *rip, *rho, *mu;
+};
+template <typename T>
+__device__ inline auto geo_of(const MeshDev &m) {
+  if constexpr (sizeof(T) == 8)
+    return Geo64{m.xc, m.yc, m.zc, m.vol, m.aip, m.rip, m.rho, m.mu};
+  else
+    return Geo32{m.xc32, m.yc32, m.zc32, m.vol32, m.aip32, m.rip32, m.rho32, m.mu32};
+}
 
 // energy equation state (energy_t, mod_energy.f90:4-10): temperature t and
 // its gradient gt, the enthalpy h it solves for (phi), h0, gradient gh
@@ -181,25 +200,31 @@ __device__ inline double rcp_of(double a) {
   return (m >= kRcpDivMin && m <= kRcpDivMax) ? 1.0 / a : __builtin_nan("");
 }
 
-__device__ inline double dot3(const double *a, const double *b) {
+// The helpers are templates on the arithmetic type: FP64 everywhere, FP32 in
+// the FP32 assembly (cfdl_set_assembly_precision), with the same operations
+// in the same order.
+template <typename T>
+__device__ inline T dot3(const T *a, const T *b) {
   return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
 }
 
-__device__ inline double area_of(const double *a) {
+template <typename T>
+__device__ inline T area_of(const T *a) {
   return sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
 }
 
 // vec_weight, src/modules/mod_util.f90:729-743 (r0 = face centre)
-__device__ inline double vec_weight(const double *r0, const double *r1, const double *r2) {
-  double ra[3], rb[3];
+template <typename T>
+__device__ inline T vec_weight(const T *r0, const T *r1, const T *r2) {
+  T ra[3], rb[3];
   for (int i = 0; i < 3; i++) {
     ra[i] = r1[i] - r0[i];
     rb[i] = r2[i] - r0[i];
   }
-  const double la = sqrt(ra[0] * ra[0] + ra[1] * ra[1] + ra[2] * ra[2]);
-  const double lb = sqrt(rb[0] * rb[0] + rb[1] * rb[1] + rb[2] * rb[2]);
-  if (la + lb > 0.0) return la / (la + lb);
-  return 0.5;
+  const T la = sqrt(ra[0] * ra[0] + ra[1] * ra[1] + ra[2] * ra[2]);
+  const T lb = sqrt(rb[0] * rb[0] + rb[1] * rb[1] + rb[2] * rb[2]);
+  if (la + lb > static_cast<T>(0)) return la / (la + lb);
+  return static_cast<T>(0.5);
 }
 
 __device__ inline void centre(const MeshDev &m, int e, double *r) {
@@ -458,60 +483,67 @@ __global__ void k_coef_uvw(MeshDev m, FieldsDev f, double dt, double alpha_u) {
 // flux, secondary-stress and deferred-correction sums) and finishes the cell.
 // Interior anb are written by their entry's thread, boundary anb by the cell.
 constexpr int kFaceThreads = 256, kFaceCells = kFaceThreads / 6;
+template <typename T>
 __global__ void __launch_bounds__(kFaceThreads)
-k_coef_uvw6(MeshDev m, FieldsDev f, double dt, double alpha_u) {
-  __shared__ double st[9][kFaceThreads];  // d, fnb, fl, secondary stress, deferred correction
-  __shared__ int sin_[kFaceThreads];      // the entry is interior
+k_coef_uvw6(MeshDev m, FieldsDev f, double dt_, double alpha_u_) {
+  // T = double: the reference's FP64 assembly bit for bit.  T = float: the
+  // same operations in FP32 on the FP32 geometry copies and on the fields
+  // rounded to FP32 (the FP32 assembly; results stored as FP64).
+  const auto G = geo_of<T>(m);
+  const T one = static_cast<T>(1), zero = static_cast<T>(0);
+  const T dt = static_cast<T>(dt_), alpha_u = static_cast<T>(alpha_u_);
+  auto ld = [](double x) { return static_cast<T>(x); };
+  __shared__ T st[9][kFaceThreads];  // d, fnb, fl, secondary stress, deferred correction
+  __shared__ int sin_[kFaceThreads];  // the entry is interior
   const int lc = threadIdx.x / 6, j = threadIdx.x - 6 * lc;
   const int e = blockIdx.x * kFaceCells + lc;
   const bool valid = lc < kFaceCells && e < m.ne;
   if (valid) {
     const int k = 6 * e + j;
     const int nb = m.nbr[k];
-    double fnb = 0.0, d = 0.0, fl = 0.0, ss[3] = {0.0, 0.0, 0.0}, dfc[3] = {0.0, 0.0, 0.0};
+    T fnb = zero, d = zero, fl = zero, ss[3] = {zero, zero, zero}, dfc[3] = {zero, zero, zero};
     if (nb < m.ne) {
-      double rp[3];
-      centre(m, e, rp);
+      const T rp[3] = {G.xc[e], G.yc[e], G.zc[e]};
       const int fs = m.fsg[k];
-      const double sg = fs >= 0 ? 1.0 : -1.0;
+      const T sg = fs >= 0 ? one : -one;
       const int fg = abs(fs) - 1;
-      const double *a = m.aip + 3 * fg, *rip = m.rip + 3 * fg;
-      const double area = area_of(a);
-      double nrm[3], rpnb[3], dr[3], drip[3], rpp[3], rnp[3], drp[3];
+      const T *a = G.aip + 3 * fg, *rip = G.rip + 3 * fg;
+      const T area = area_of(a);
+      T nrm[3], dr[3], drip[3], rpp[3], rnp[3], drp[3];
       for (int i = 0; i < 3; i++) nrm[i] = sg * a[i] / area;
-      centre(m, nb, rpnb);
+      const T rpnb[3] = {G.xc[nb], G.yc[nb], G.zc[nb]};
       for (int i = 0; i < 3; i++) dr[i] = rpnb[i] - rp[i];
-      const double ds = sqrt(dot3(dr, dr));
-      const double wt = vec_weight(rip, rp, rpnb);
+      const T ds = sqrt(dot3(dr, dr));
+      const T wt = vec_weight(rip, rp, rpnb);
       for (int i = 0; i < 3; i++) drip[i] = rip[i] - rp[i];
-      double t = dot3(drip, nrm);
+      T t = dot3(drip, nrm);
       for (int i = 0; i < 3; i++) rpp[i] = rip[i] - t * nrm[i];
       for (int i = 0; i < 3; i++) drip[i] = rip[i] - rpnb[i];
       t = dot3(drip, nrm);
       for (int i = 0; i < 3; i++) rnp[i] = rip[i] - t * nrm[i];
       for (int i = 0; i < 3; i++) drp[i] = rnp[i] - rpp[i];
-      const double dsp = sqrt(dot3(drp, drp));
-      fl = -(sg * f.mip[fg]);
-      fnb = fmax(fl, 0.0);
-      const double muip = (1.0 - wt) * m.mu[e] + wt * m.mu[nb];
+      const T dsp = sqrt(dot3(drp, drp));
+      fl = -(sg * ld(f.mip[fg]));
+      fnb = fmax(fl, zero);
+      const T muip = (one - wt) * G.mu[e] + wt * G.mu[nb];
       d = muip * area / ds;
-      const double ge[3][3] = {{f.gu[3 * e], f.gu[3 * e + 1], f.gu[3 * e + 2]},
-                               {f.gv[3 * e], f.gv[3 * e + 1], f.gv[3 * e + 2]},
-                               {f.gw[3 * e], f.gw[3 * e + 1], f.gw[3 * e + 2]}};
-      const double gn[3][3] = {{f.gu[3 * nb], f.gu[3 * nb + 1], f.gu[3 * nb + 2]},
-                               {f.gv[3 * nb], f.gv[3 * nb + 1], f.gv[3 * nb + 2]},
-                               {f.gw[3 * nb], f.gw[3 * nb + 1], f.gw[3 * nb + 2]}};
+      const T ge[3][3] = {{ld(f.gu[3 * e]), ld(f.gu[3 * e + 1]), ld(f.gu[3 * e + 2])},
+                          {ld(f.gv[3 * e]), ld(f.gv[3 * e + 1]), ld(f.gv[3 * e + 2])},
+                          {ld(f.gw[3 * e]), ld(f.gw[3 * e + 1]), ld(f.gw[3 * e + 2])}};
+      const T gn[3][3] = {{ld(f.gu[3 * nb]), ld(f.gu[3 * nb + 1]), ld(f.gu[3 * nb + 2])},
+                          {ld(f.gv[3 * nb]), ld(f.gv[3 * nb + 1]), ld(f.gv[3 * nb + 2])},
+                          {ld(f.gw[3 * nb]), ld(f.gw[3 * nb + 1]), ld(f.gw[3 * nb + 2])}};
       for (int mm = 0; mm < 3; mm++) {
-        double gip[3];
-        for (int q = 0; q < 3; q++) gip[q] = (1.0 - wt) * ge[q][mm] + wt * gn[q][mm];
+        T gip[3];
+        for (int q = 0; q < 3; q++) gip[q] = (one - wt) * ge[q][mm] + wt * gn[q][mm];
         ss[mm] = muip * area * dot3(gip, dr) / ds;
       }
       for (int q = 0; q < 3; q++) {
-        double gip[3];
-        for (int i = 0; i < 3; i++) gip[i] = (1.0 - wt) * ge[q][i] + wt * gn[q][i];
+        T gip[3];
+        for (int i = 0; i < 3; i++) gip[i] = (one - wt) * ge[q][i] + wt * gn[q][i];
         dfc[q] = muip * area * (dot3(gip, drp) / dsp - dot3(gip, dr) / ds);
       }
-      f.anb[k] = d + fnb;
+      f.anb[k] = static_cast<double>(d + fnb);
     }
     st[0][threadIdx.x] = d;
     st[1][threadIdx.x] = fnb;
@@ -524,13 +556,12 @@ k_coef_uvw6(MeshDev m, FieldsDev f, double dt, double alpha_u) {
   }
   __syncthreads();
   if (!valid || j != 0) return;
-  double rp[3];
-  centre(m, e, rp);
-  double ap = 0.0, sumf = 0.0, sumss[3] = {0.0, 0.0, 0.0}, sumdefc[3] = {0.0, 0.0, 0.0};
-  double anbk[6];
+  const T rp[3] = {G.xc[e], G.yc[e], G.zc[e]};
+  T ap = zero, sumf = zero, sumss[3] = {zero, zero, zero}, sumdefc[3] = {zero, zero, zero};
+  T anbk[6];
   for (int q = 0; q < 6; q++) {
     const int t = threadIdx.x + q;
-    const double d = st[0][t], fnb = st[1][t];
+    const T d = st[0][t], fnb = st[1][t];
     if (sin_[t]) {
       sumf = sumf + st[2][t];
       for (int i = 0; i < 3; i++) sumss[i] = sumss[i] + st[3 + i][t];
@@ -539,66 +570,69 @@ k_coef_uvw6(MeshDev m, FieldsDev f, double dt, double alpha_u) {
     anbk[q] = d + fnb;
     ap = ap + d + fnb;
   }
-  const double vol = m.vol[e];
-  const double ap0 = m.rho[e] * vol / dt;
+  const T vol = G.vol[e];
+  const T ap0 = G.rho[e] * vol / dt;
   ap = ap + ap0;
-  double bu = ap0 * f.u0[e] + sumf * f.u[e] - vol * f.gp[3 * e] + sumss[0] + sumdefc[0];
-  double bv = ap0 * f.v0[e] + sumf * f.v[e] - vol * f.gp[3 * e + 1] + sumss[1] + sumdefc[1];
-  double bw = ap0 * f.w0[e] + sumf * f.w[e] - vol * f.gp[3 * e + 2] + sumss[2] + sumdefc[2];
+  const T ue = ld(f.u[e]), ve = ld(f.v[e]), we = ld(f.w[e]);
+  T bu = ap0 * ld(f.u0[e]) + sumf * ue - vol * ld(f.gp[3 * e]) + sumss[0] + sumdefc[0];
+  T bv = ap0 * ld(f.v0[e]) + sumf * ve - vol * ld(f.gp[3 * e + 1]) + sumss[1] + sumdefc[1];
+  T bw = ap0 * ld(f.w0[e]) + sumf * we - vol * ld(f.gp[3 * e + 2]) + sumss[2] + sumdefc[2];
   for (int q = 0; q < 6; q++)
-    if (!sin_[threadIdx.x + q]) f.anb[6 * e + q] = anbk[q];
+    if (!sin_[threadIdx.x + q]) f.anb[6 * e + q] = static_cast<double>(anbk[q]);
   for (int q = m.bcrow[e]; q < m.bcrow[e + 1]; q++) {
     const int k = m.bcslot[q], H = m.nbr[k];
     const int fg = m.fsg[k] - 1;  // boundary faces are always owner side
-    const double *a = m.aip + 3 * fg;
-    const double area = area_of(a);
-    const double nrm[3] = {a[0] / area, a[1] / area, a[2] / area};
-    const double dr[3] = {m.xc[H] - rp[0], m.yc[H] - rp[1], m.zc[H] - rp[2]};
-    const double ds = sqrt(dot3(dr, dr));
-    const double fz = 0.0, d = m.mu[e] * area / ds;
+    const T *a = G.aip + 3 * fg;
+    const T area = area_of(a);
+    const T nrm[3] = {a[0] / area, a[1] / area, a[2] / area};
+    const T dr[3] = {G.xc[H] - rp[0], G.yc[H] - rp[1], G.zc[H] - rp[2]};
+    const T ds = sqrt(dot3(dr, dr));
+    const T fz = zero, d = G.mu[e] * area / ds;
     const int kind = m.hkind[H - m.ne];
-    if (kind == CFDL_BC_INLET || kind == CFDL_BC_OUTLET) {  // channel extension, as above
-      const double fl = -f.mip[fg], fnb = fmax(fl, 0.0);
-      const double dd = kind == CFDL_BC_INLET ? d : 0.0;
-      bu = bu + fl * f.u[e];
-      bv = bv + fl * f.v[e];
-      bw = bw + fl * f.w[e];
+    if (kind == CFDL_BC_INLET || kind == CFDL_BC_OUTLET) {
+      // channel extension: the face's convective flux as on interior faces
+      // (anb += max(f, 0), f * u_P into the right-hand side as sumf),
+      // diffusion at the inlet only (zero-gradient outlet)
+      const T fl = -ld(f.mip[fg]), fnb = fmax(fl, zero);
+      const T dd = kind == CFDL_BC_INLET ? d : zero;
+      bu = bu + fl * ue;
+      bv = bv + fl * ve;
+      bw = bw + fl * we;
       ap = ap + dd + fnb;
       anbk[k - 6 * e] = anbk[k - 6 * e] + dd + fnb;
-      f.anb[k] = anbk[k - 6 * e];
+      f.anb[k] = static_cast<double>(anbk[k - 6 * e]);
       continue;
     }
     if (kind != CFDL_BC_SYMMETRY) {  // 'dirichlet', :255-265
-      const double ue = f.u[e], ve = f.v[e], we = f.w[e];
-      double vr[3] = {ue, ve, we};
-      const double t = dot3(vr, nrm);
+      T vr[3] = {ue, ve, we};
+      const T t = dot3(vr, nrm);
       for (int i = 0; i < 3; i++) vr[i] = vr[i] - t * nrm[i];
-      vr[0] = f.u[H] - vr[0];
-      vr[1] = f.v[H] - vr[1];
-      vr[2] = f.w[H] - vr[2];
+      vr[0] = ld(f.u[H]) - vr[0];
+      vr[1] = ld(f.v[H]) - vr[1];
+      vr[2] = ld(f.w[H]) - vr[2];
       bu = bu + d * vr[0] - d * ue;
       bv = bv + d * vr[1] - d * ve;
       bw = bw + d * vr[2] - d * we;
     }
     ap = ap + d + fz;
     anbk[k - 6 * e] = anbk[k - 6 * e] + d + fz;
-    f.anb[k] = anbk[k - 6 * e];
+    f.anb[k] = static_cast<double>(anbk[k - 6 * e]);
   }
-  if (alpha_u != 1.0) {
-    const double apr = ap / alpha_u, da = apr - ap;
-    bu = bu + da * f.u[e];
-    bv = bv + da * f.v[e];
-    bw = bw + da * f.w[e];
+  if (alpha_u != one) {
+    const T apr = ap / alpha_u, da = apr - ap;
+    bu = bu + da * ue;
+    bv = bv + da * ve;
+    bw = bw + da * we;
     ap = apr;
   }
-  f.ap[e] = ap;
-  f.bu[e] = bu;
-  f.bv[e] = bv;
-  f.bw[e] = bw;
-  f.d[e] = vol / ap;
-  double dc = ap;
+  f.ap[e] = static_cast<double>(ap);
+  f.bu[e] = static_cast<double>(bu);
+  f.bv[e] = static_cast<double>(bv);
+  f.bw[e] = static_cast<double>(bw);
+  f.d[e] = static_cast<double>(vol / ap);
+  T dc = ap;
   for (int q = 0; q < 6; q++) dc = dc - anbk[q];
-  f.dc[e] = vol / dc;
+  f.dc[e] = static_cast<double>(vol / dc);
 }
 
 // ---------------------------------------------------------------------------
@@ -823,9 +857,15 @@ __global__ void k_coef_p(MeshDev m, FieldsDev f) {
 
 // calc_coef_p for meshes whose rows all have six entries, six threads per
 // cell as in k_coef_uvw6: entry terms in parallel, then the cell's sums in
-// CSR order.
+// CSR order.  T = float (the FP32 assembly) computes the coefficients in
+// FP32; the right-hand side, the cell's net inflow (the continuity check),
+// stays FP64 in both.
+template <typename T>
 __global__ void __launch_bounds__(kFaceThreads) k_coef_p6(MeshDev m, FieldsDev f) {
-  __shared__ double sd[kFaceThreads], sfl[kFaceThreads];
+  const auto G = geo_of<T>(m);
+  const T one = static_cast<T>(1), zero = static_cast<T>(0);
+  __shared__ T sd[kFaceThreads];
+  __shared__ double sfl[kFaceThreads];
   __shared__ int sin_[kFaceThreads];
   const int lc = threadIdx.x / 6, j = threadIdx.x - 6 * lc;
   const int e = blockIdx.x * kFaceCells + lc;
@@ -833,24 +873,25 @@ __global__ void __launch_bounds__(kFaceThreads) k_coef_p6(MeshDev m, FieldsDev f
   if (valid) {
     const int k = 6 * e + j;
     const int nb = m.nbr[k];
-    double d = 0.0, fl = 0.0;
+    T d = zero;
+    double fl = 0.0;
     if (nb < m.ne) {
-      double rp[3];
-      centre(m, e, rp);
+      const T rp[3] = {G.xc[e], G.yc[e], G.zc[e]};
       const int fs = m.fsg[k];
-      const double sg = fs >= 0 ? 1.0 : -1.0;
+      const T sg = fs >= 0 ? one : -one;
       const int fg = abs(fs) - 1;
-      const double *a = m.aip + 3 * fg, *rip = m.rip + 3 * fg;
-      const double area = area_of(a);
-      double nrm[3], rpnb[3], dr[3];
+      const T *a = G.aip + 3 * fg, *rip = G.rip + 3 * fg;
+      const T area = area_of(a);
+      T nrm[3], dr[3];
       for (int i = 0; i < 3; i++) nrm[i] = sg * a[i] / area;
-      centre(m, nb, rpnb);
+      const T rpnb[3] = {G.xc[nb], G.yc[nb], G.zc[nb]};
       for (int i = 0; i < 3; i++) dr[i] = rpnb[i] - rp[i];
-      const double wt = vec_weight(rip, rp, rpnb);
-      fl = -(sg * f.mip[fg]);
-      const double rhoip = (1.0 - wt) * m.rho[e] + wt * m.rho[nb];
-      d = ((1.0 - wt) * f.dc[e] + wt * f.dc[nb]) / dot3(dr, nrm) * rhoip * area;
-      f.anb[k] = d;
+      const T wt = vec_weight(rip, rp, rpnb);
+      fl = -((fs >= 0 ? 1.0 : -1.0) * f.mip[fg]);
+      const T rhoip = (one - wt) * G.rho[e] + wt * G.rho[nb];
+      d = ((one - wt) * static_cast<T>(f.dc[e]) + wt * static_cast<T>(f.dc[nb])) / dot3(dr, nrm) *
+          rhoip * area;
+      f.anb[k] = static_cast<double>(d);
     }
     sd[threadIdx.x] = d;
     sfl[threadIdx.x] = fl;
@@ -858,25 +899,26 @@ __global__ void __launch_bounds__(kFaceThreads) k_coef_p6(MeshDev m, FieldsDev f
   }
   __syncthreads();
   if (!valid || j != 0) return;
-  double ap = 0.0, sumf = 0.0;
+  T ap = zero;
+  double sumf = 0.0;
   for (int q = 0; q < 6; q++) {
     const int t = threadIdx.x + q;
     if (sin_[t]) {
       sumf = sumf + sfl[t];
     } else {
-      f.anb[6 * e + q] = sd[t];
+      f.anb[6 * e + q] = static_cast<double>(sd[t]);
     }
     ap = ap + sd[t];
   }
   double b = sumf;
   for (int q = m.bcrow[e]; q < m.bcrow[e + 1]; q++) {
     const int k = m.bcslot[q];
-    const double d = 0.0;  // 'dirichlet' and 'zero_flux' both add nothing
+    const T d = zero;  // 'dirichlet' and 'zero_flux' both add nothing
     ap = ap + d;
-    f.anb[k] = sd[threadIdx.x + (k - 6 * e)] + d;
+    f.anb[k] = static_cast<double>(sd[threadIdx.x + (k - 6 * e)] + d);
     b = b - f.mip[m.fsg[k] - 1];
   }
-  f.ap[e] = ap;
+  f.ap[e] = static_cast<double>(ap);
   f.b[e] = b;
 }
 
@@ -925,6 +967,11 @@ __global__ void k_halo_from_owner(MeshDev m, double *phi) {
 __global__ void k_fill(double *p, int64_t n, double v) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
+}
+__global__ void k_to_f32(const double *src, float *dst, int64_t n) {
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    dst[i] = static_cast<float>(src[i]);
 }
 __global__ void k_copy(double *dst, const double *src, int64_t n) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -1727,6 +1774,9 @@ struct cfdl_ctx {
   // ap, anb, residual and correction (cells and halos)
   float *a32 = nullptr, *anb32 = nullptr, *r32 = nullptr, *d32 = nullptr;
   int pc_nlev = 0;
+  // FP32 assembly of the momentum and pc coefficients (an extension; its
+  // FP32 copies of the geometry live in md.*32)
+  bool asm32 = false;
   // channel extension: inlet profile, the host copy of the halo BC kinds and
   // the inlet / outlet faces (halo order) whose fluxes k_outlet_flux balances
   Inlet inlet{{1.0, 0.0, 1.0, 0.0, 1.0}};
@@ -2999,6 +3049,12 @@ int cfdl_set_properties(cfdl_ctx *c, const double *rho, const double *mu) {
   }
   HIPCHK(hipMemcpyAsync(const_cast<double *>(c->md.rho), rho, 8 * size_t(c->ne), hipMemcpyHostToDevice, c->st));
   HIPCHK(hipMemcpyAsync(const_cast<double *>(c->md.mu), mu, 8 * size_t(c->ne), hipMemcpyHostToDevice, c->st));
+  if (c->md.rho32) {
+    hipLaunchKernelGGL(k_to_f32, dim3(nblocks(c->ne)), dim3(kBlock), 0, c->st, c->md.rho,
+                       const_cast<float *>(c->md.rho32), int64_t{c->ne});
+    hipLaunchKernelGGL(k_to_f32, dim3(nblocks(c->ne)), dim3(kBlock), 0, c->st, c->md.mu,
+                       const_cast<float *>(c->md.mu32), int64_t{c->ne});
+  }
   HIPCHK(hipStreamSynchronize(c->st));
   return CFDL_OK;
 }
@@ -3066,6 +3122,38 @@ static int mg_ensure(cfdl_ctx *c, int fp32) {
   CALL(cfdl::mg_build(ne, row.data(), nbr.data(), x.data(), y.data(), z.data(), kMgAgg,
                       kMgCoarse, &h));
   return cfdl::mgcg_create(h, c->nbf, fp32, &c->mg[fp32]);
+}
+
+int cfdl_set_assembly_precision(cfdl_ctx *c, int bits) {
+  CALL(check_ctx(c));
+  if (bits != 32 && bits != 64) {
+    set_error("cfdl_set_assembly_precision: bits must be 32 or 64");
+    return CFDL_EINVAL;
+  }
+  if (bits == 32 && !c->mhex) {
+    set_error("cfdl_set_assembly_precision: FP32 assembly needs a mesh whose rows all have "
+              "six entries");
+    return CFDL_EINVAL;
+  }
+  if (bits == 32 && !c->md.xc32) {
+    // FP32 copies of the geometry and properties (properties are re-copied
+    // by cfdl_set_properties)
+    const int n = c->ne + c->nbf;
+    float *p[8];
+    const double *src[8] = {c->md.xc, c->md.yc, c->md.zc, c->md.vol, c->md.aip, c->md.rip,
+                            c->md.rho, c->md.mu};
+    const int64_t cnt[8] = {n, n, n, c->ne, 3 * int64_t{c->nf}, 3 * int64_t{c->nf}, c->ne, c->ne};
+    for (int i = 0; i < 8; i++) {
+      CALL(dalloc(c, &p[i], cnt[i]));
+      hipLaunchKernelGGL(k_to_f32, dim3(std::min<int64_t>((cnt[i] + kBlock - 1) / kBlock, 65535)),
+                         dim3(kBlock), 0, c->st, src[i], p[i], cnt[i]);
+    }
+    HIPCHK(hipGetLastError());
+    c->md.xc32 = p[0], c->md.yc32 = p[1], c->md.zc32 = p[2], c->md.vol32 = p[3];
+    c->md.aip32 = p[4], c->md.rip32 = p[5], c->md.rho32 = p[6], c->md.mu32 = p[7];
+  }
+  c->asm32 = bits == 32;
+  return CFDL_OK;
 }
 
 int cfdl_set_pc_order(cfdl_ctx *c, int order) {
@@ -3212,9 +3300,12 @@ int cfdl_calc_coef_uvw(cfdl_ctx *c) {
               "alpha_u < 1 (cfdl_set_relaxation)");
     return CFDL_ESTATE;
   }
-  if (c->mhex)
-    hipLaunchKernelGGL(k_coef_uvw6, dim3((c->ne + kFaceCells - 1) / kFaceCells), dim3(kFaceThreads),
-                       0, c->st, c->md, c->fd, c->dt, c->alpha_u);
+  if (c->mhex && c->asm32)
+    hipLaunchKernelGGL(k_coef_uvw6<float>, dim3((c->ne + kFaceCells - 1) / kFaceCells),
+                       dim3(kFaceThreads), 0, c->st, c->md, c->fd, c->dt, c->alpha_u);
+  else if (c->mhex)
+    hipLaunchKernelGGL(k_coef_uvw6<double>, dim3((c->ne + kFaceCells - 1) / kFaceCells),
+                       dim3(kFaceThreads), 0, c->st, c->md, c->fd, c->dt, c->alpha_u);
   else
     hipLaunchKernelGGL(k_coef_uvw, dim3(nblocks(c->ne)), dim3(kBlock), 0, c->st, c->md, c->fd,
                        c->dt, c->alpha_u);
@@ -3264,9 +3355,12 @@ int cfdl_calc_mip(cfdl_ctx *c, int rhie_chow) {
 
 int cfdl_calc_coef_p(cfdl_ctx *c) {
   CALL(check_ctx(c));
-  if (c->mhex)
-    hipLaunchKernelGGL(k_coef_p6, dim3((c->ne + kFaceCells - 1) / kFaceCells), dim3(kFaceThreads),
-                       0, c->st, c->md, c->fd);
+  if (c->mhex && c->asm32)
+    hipLaunchKernelGGL(k_coef_p6<float>, dim3((c->ne + kFaceCells - 1) / kFaceCells),
+                       dim3(kFaceThreads), 0, c->st, c->md, c->fd);
+  else if (c->mhex)
+    hipLaunchKernelGGL(k_coef_p6<double>, dim3((c->ne + kFaceCells - 1) / kFaceCells),
+                       dim3(kFaceThreads), 0, c->st, c->md, c->fd);
   else
     hipLaunchKernelGGL(k_coef_p, dim3(nblocks(c->ne)), dim3(kBlock), 0, c->st, c->md, c->fd);
   HIPCHK(hipGetLastError());

@@ -228,6 +228,16 @@ enum {
 };
 int cfdl_set_pc_order(cfdl_ctx *c, int order);
 
+/* Arithmetic of the coefficient assembly (calc_coef_uvw and calc_coef_p,
+   mod_uvwp.f90:161-368): 64 (the default) is the reference's FP64 bit for
+   bit; 32 (no counterpart in the reference) computes ap, anb, the momentum
+   sources, d and dc in FP32 from FP32 copies of the geometry and properties
+   and the fields rounded to FP32, stored as FP64.  The pc right-hand side
+   (each cell's net inflow, the continuity check), the fields, the linear
+   solves' residuals and norms and the update stay FP64.  Needs a mesh whose
+   rows all have six entries. */
+int cfdl_set_assembly_precision(cfdl_ctx *c, int bits);
+
 /* Visiting order of solve_gs (mod_solver.f90:255-327: the momentum solves,
    the energy solve, and the pc solve with one subdomain).
    CFDL_GS_REFERENCE (the default) is the reference's global order, bit for

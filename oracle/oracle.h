@@ -110,6 +110,9 @@ typedef struct or_case {
      umax*16*s*(1-s)*t*(1-t) along the inward normal, s = (y-y0)/(y1-y0),
      t = (z-z0)/(z1-z0) at the face centre */
   double inlet[5];
+  /* FP32 coefficient assembly (an extension, oracle_asm32.c): 1 = ap, anb,
+     sources, d, dc in single precision; the pc right-hand side stays double */
+  int asm32;
 } or_case;
 
 /* ---- setup ---- */
@@ -175,6 +178,8 @@ void or_scale_outlet(or_case *c); /* channel extension, called by or_update_boun
 void or_calc_coef_uvw(or_case *c);
 void or_calc_mip(or_case *c, int rhie_chow);
 void or_calc_coef_p(or_case *c);
+void or_calc_coef_uvw32(or_case *c); /* FP32 assembly, oracle_asm32.c */
+void or_calc_coef_p32(or_case *c);
 void or_adjust_pc(or_case *c, double *pc, double pref);
 void or_update_uvwp(or_case *c, const double *pc);
 void or_solve_uvwp(or_case *c);

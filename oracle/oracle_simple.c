@@ -862,7 +862,10 @@ void or_update_uvwp(or_case *c, const double *pc) {
 /* solve_uvwp, src/equations/mod_uvwp.f90:95-134 */
 void or_solve_uvwp(or_case *c) {
   int ne = c->ne, nf = c->nf, nbf = c->nbf;
-  or_calc_coef_uvw(c);
+  if (c->asm32)
+    or_calc_coef_uvw32(c);
+  else
+    or_calc_coef_uvw(c);
   or_solve_gs(c, "u", c->u, c->ap, c->anb, c->bu, c->ef2nb_idx, c->ef2nb, ne, nf, nbf, c->nit);
   or_solve_gs(c, "v", c->v, c->ap, c->anb, c->bv, c->ef2nb_idx, c->ef2nb, ne, nf, nbf, c->nit);
   or_solve_gs(c, "w", c->w, c->ap, c->anb, c->bw, c->ef2nb_idx, c->ef2nb, ne, nf, nbf, c->nit);
@@ -870,7 +873,10 @@ void or_solve_uvwp(or_case *c) {
   or_calc_grad(c->v, c->gv, c->xc, c->yc, c->zc, c->ef2nb_idx, c->ef2nb, ne, nf, nbf);
   or_calc_grad(c->w, c->gw, c->xc, c->yc, c->zc, c->ef2nb_idx, c->ef2nb, ne, nf, nbf);
   or_calc_mip(c, 1);
-  or_calc_coef_p(c);
+  if (c->asm32)
+    or_calc_coef_p32(c);
+  else
+    or_calc_coef_p(c);
   for (int k = 0; k < ne + nbf; k++) c->phic[k] = 0.0;
   or_solve(c, "pc", c->phic);
   double pref = c->phic[0];

@@ -51,7 +51,7 @@ class OrCase(C.Structure):
                 ("cp", PD), ("tc", PD), ("t", PD), ("gt", PD), ("h", PD), ("h0", PD), ("gh", PD),
                 ("pc_order", C.c_int), ("pc_perm", C.POINTER(PI)),
                 ("gs_order", C.c_int), ("gs_perm", PI), ("mg", C.c_void_p),
-                ("inlet", C.c_double * 5)]
+                ("inlet", C.c_double * 5), ("asm32", C.c_int)]
 
 
 _lib = None
@@ -184,6 +184,9 @@ class OracleCase:
     def set_inlet_profile(self, umax=1.0, y0=0.0, y1=1.0, z0=0.0, z1=1.0):
         for k, v in enumerate((umax, y0, y1, z0, z1)):
             self.c.inlet[k] = v
+
+    def set_assembly_precision(self, bits=64):
+        self.c.asm32 = 1 if bits == 32 else 0
 
     def set_relaxation(self, alpha_u=1.0, alpha_p=1.0):
         self.c.alpha_u = alpha_u

@@ -94,3 +94,17 @@ def test_channel_128_distorted_renumbered_properties():
         assert np.all(np.isfinite(eq.get(name))), name
     pc = [l for l in eq.log() if l[0] == "pc"]
     assert len(pc) == 4 and all(l[1] < 100 and l[3] <= l[2] / 10 for l in pc)
+
+
+def test_channel_fp32_assembly_bitwise():
+    """The channel boundaries in the FP32 assembly, against the oracle."""
+    m, eq, oc = make_pair(12, 8, 2.0, 0.2, 7, 4, 1.0 / 200.0, float("inf"))
+    for x in (eq, oc):
+        x.set_relaxation(0.7, 0.3)
+        x.set_pc_order("mgcg-mixed")
+        x.set_gs_order("colored")
+        x.set_assembly_precision(32)
+    eq.run(1, 4)
+    oc.run(1, 4)
+    for name in ["u", "v", "w", "p", "mip", "ap", "anb", "b", "bu", "dc"]:
+        assert np.array_equal(eq.get(name), oc.arr(name)), name

@@ -122,3 +122,24 @@ def test_steady_mode_needs_relaxation():
     eq = cfdl.UVWP.cavity(mesh, rho=1.0, mu=0.01, dt=float("inf"))
     with pytest.raises(cfdl.CfdlError):
         eq.run(1, 1)
+
+
+@pytest.mark.parametrize("n,jit,nsubd,order,gs", [(12, 0.15, 4, "mixed", "colored"),
+                                                  (16, 0.2, 4, "mgcg-mixed", "colored"),
+                                                  (9, 0.1, 2, "reference", "reference")])
+def test_fp32_assembly_bitwise(n, jit, nsubd, order, gs):
+    """FP32 coefficient assembly (an extension): the FP32 operations of
+    k_coef_uvw6<float> / k_coef_p6<float> against the oracle's float
+    restatement, bit for bit, within the steady relaxed loop."""
+    eq, oc = make_pair(n, jit, nsubd, 0.001, float("inf"))
+    for x in (eq, oc):
+        x.set_relaxation(0.7, 0.3)
+        x.set_pc_order(order)
+        x.set_gs_order(gs)
+        x.set_assembly_precision(32)
+    eq.run(1, 4)
+    oc.run(1, 4)
+    for name in ["u", "v", "w", "p", "gp", "mip", "ap", "anb", "b", "bu", "bv", "bw", "d", "dc"]:
+        a, b = eq.get(name), oc.arr(name)
+        assert np.array_equal(a, b), "%s: max |diff| %g" % (name, np.max(np.abs(a - b)))
+    assert [l[:2] for l in eq.log()] == [l[:2] for l in oc.log()]

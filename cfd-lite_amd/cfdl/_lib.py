@@ -34,6 +34,7 @@ _SIGS = {
     "cfdl_mesh_get_f64": (C.c_int, [VP, C.c_char_p, PD, C.c_int64]),
     "cfdl_mesh_free": (None, [VP]),
     "cfdl_calc_residual": (C.c_int, [C.c_int, VP, VP, VP, VP, VP, VP, PD, PD, VP]),
+    "cfdl_calc_residual_noalloc": (C.c_int, [C.c_int, VP, VP, VP, VP, VP, VP, PD, PD, VP]),
     "cfdl_mesh_mg_get": (C.c_int, [VP, C.c_int, C.c_char_p, PI, C.c_int64, C.POINTER(C.c_int64)]),
     "cfdl_mesh_set_section_names": (C.c_int, [VP, C.c_int, C.POINTER(C.c_char_p)]),
     "cfdl_mesh_write_vtu": (C.c_int, [VP, C.c_char_p, C.c_int, C.POINTER(C.c_char_p), PI,

@@ -32,6 +32,7 @@
 #include <cstddef>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -2486,6 +2487,30 @@ static int check_ctx(cfdl_ctx *c) {
   return CFDL_OK;
 }
 
+// Device scratch of cfdl_calc_residual_noalloc: the block partials (2 per
+// block, at most 2048 blocks) and the two reduced values.  Each GPU holds
+// its own copy until the process ends; host code uses the address that
+// hipGetSymbolAddress gives for the current device, never the array's name.
+// Calls that overlap on one GPU share it and race.
+constexpr int kResidualMaxBlocks = 2048;
+static __device__ __align__(256) double scratch[2 * kResidualMaxBlocks + 2];
+
+static int resid_scratch(double **out) {
+  static std::mutex mtx;
+  static std::vector<double *> addr;  // per device id, resolved once
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mtx);
+  if (static_cast<int>(addr.size()) <= dev) addr.resize(dev + 1, nullptr);
+  if (!addr[dev]) {
+    void *p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    addr[dev] = static_cast<double *>(p);
+  }
+  *out = addr[dev];
+  return CFDL_OK;
+}
+
 extern "C" {
 
 const char *cfdl_last_error(void) { return g_err.c_str(); }
@@ -3594,6 +3619,16 @@ int cfdl_calc_residual(int ne, const int *row, const int *nbr, const double *ap,
                            res, res_max);
   HIPCHK(hipFree(buf));
   return rc;
+}
+
+int cfdl_calc_residual_noalloc(int ne, const int *row, const int *nbr, const double *ap,
+                               const double *anb, const double *b, const double *phi, double *res,
+                               double *res_max, void *stream) {
+  CALL(resid_args(ne, row, nbr, ap, anb, b, phi, res, res_max, "cfdl_calc_residual_noalloc"));
+  double *buf = nullptr;
+  CALL(resid_scratch(&buf));
+  return resid_run(ne, row, nbr, ap, anb, b, phi, buf, static_cast<hipStream_t>(stream), res,
+                   res_max);
 }
 
 int cfdl_set_profile(cfdl_ctx *c, int sample_every) {

@@ -335,6 +335,16 @@ int cfdl_log_clear(cfdl_ctx *c);
 int cfdl_calc_residual(int ne, const int *row, const int *nbr, const double *ap,
                        const double *anb, const double *b, const double *phi, double *res,
                        double *res_max, void *stream);
+/* The same, allocating nothing (no counterpart in the reference): for
+   callers that cannot allocate device memory -- code under HIP graph
+   capture, tight serving loops, harnesses that forbid it.  Its block
+   partials live in a fixed device array of 4098 doubles (2048 blocks) that
+   each GPU keeps until the process ends, so ne is limited to 2^30 as above;
+   calls that overlap on one GPU share that array and race.  Results equal
+   cfdl_calc_residual's bit for bit. */
+int cfdl_calc_residual_noalloc(int ne, const int *row, const int *nbr, const double *ap,
+                               const double *anb, const double *b, const double *phi,
+                               double *res, double *res_max, void *stream);
 /* Launch counters and sampled kernel timing (no counterpart in the
    reference, which has no timers).  cfdl_set_profile(c, k) brackets every
    k-th pressure-correction sweep (all of its launches, back to back) with

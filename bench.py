@@ -475,10 +475,13 @@ def main():
     ap.add_argument("--gs-order", choices=("reference", "colored"), default="reference",
                     help="order of solve_gs of --converge runs")
     ap.add_argument("--serving", action="store_true",
-                    help="also run a serving stream and report solves/s with p50/p99 latency")
+                    help="only run the serving stream and report solves/s with p50/p99 latency")
     ap.add_argument("--serving-requests", type=int, default=512)
     ap.add_argument("--serving-rate", type=float, default=10.0, help="arrivals per second")
-    ap.add_argument("--serving-slots", type=int, default=8, help="requests in flight")
+    ap.add_argument("--serving-cases", type=int, default=32, help="cases in one batch")
+    ap.add_argument("--serving-max-outer", type=int, default=500)
+    ap.add_argument("--serving-chunk", type=int, default=10,
+                    help="outer iterations per batch run (convergence checked between runs)")
     ap.add_argument("--pcie", action="store_true",
                     help="also time outer iterations with the host transfers of the drop-in "
                          "Fortran solve_uvwp (state up, state and work arrays down)")
@@ -492,6 +495,9 @@ def main():
         sys.exit(2)
 
     import cfdl
+    if args.serving:
+        serving_line(cfdl, args)
+        return
     mu = 1.0 / args.re
     t0 = time.perf_counter()
     mesh = cfdl.Mesh.box(args.n).partition(4)
@@ -601,30 +607,42 @@ def main():
         out["extras"] = extras(cfdl, args, mesh, {"value": out["value"],
                                                   "hbm_frac_whole_iteration": round(hbm_whole, 4)})
     print(json.dumps(out), flush=True)
-    if args.serving:
-        serving_line(args)
 
 
-def serving_line(args):
+def serving_run(cfdl, nreq, rate, tol, max_outer, chunk, max_cases, seed=20261202):
     """The serving stream (an extension; the reference solves one case per
-    run): seeded Poisson arrivals, each request a cavity solved from rest on
-    a context kept per edge, in the fastest mode (mixed pc solve, colour
-    order of solve_gs).  Printed as its own JSON line."""
+    run): seeded Poisson arrivals of steady cavity and channel solves from
+    rest (N^3, N in {16, ..., 96}, Re log-uniform in [100, 1000], kind a fair
+    draw), continuously batched: admitted into one batch as they arrive and
+    memory frees up, all cases in the same grouped launches, each run to its
+    scaled residual tol or max_outer outer iterations."""
     from cfdl import serving
-    reqs = serving.make_requests(args.serving_requests, args.serving_rate, 20261202)
-    print("bench: serving %d requests" % len(reqs), file=sys.stderr, flush=True)
-    out = serving.serve(reqs, slots=args.serving_slots, pc_order="mixed", gs_order="colored")
-    cells = sum(n ** 3 for _, n, _ in reqs)
-    print(json.dumps({"mode": "serving", "metric": "solves/s with p50/p99 latency",
-                      "solves_per_s": round(out["solves_per_s"], 3),
-                      "p50_ms": round(1e3 * out["p50_s"], 1), "p99_ms": round(1e3 * out["p99_s"], 1),
-                      "wall_s": round(out["wall_s"], 2), "requests": len(reqs),
-                      "arrival_rate_per_s": args.serving_rate, "slots": args.serving_slots,
-                      "cell_iterations_per_s": round(30 * cells / out["wall_s"], 1),
-                      "case": "lid-driven cavity N^3, N in {16,...,96}, Re log-uniform in "
-                              "[100, 1000], 10 x 3 outer iterations from rest, 4 subdomains",
-                      "pc_order": "mixed", "gs_order": "colored",
-                      "dtype": "f32 pc sweeps, f64 otherwise", "data": "synthetic"}), flush=True)
+    reqs = serving.make_requests(nreq, rate, seed)
+    progress("serving %d requests, %g per s, tol %g or %d outer iterations"
+             % (nreq, rate, tol, max_outer))
+    out = serving.serve(reqs, max_cases=max_cases, tol=tol, max_outer=max_outer, chunk=chunk)
+    cell_its = sum(r.n ** 3 * out["outer"][r.idx] for r in reqs)
+    return {"metric": "batched solves/s with p50/p99 latency",
+            "solves_per_s": round(out["solves_per_s"], 3),
+            "p50_ms": round(1e3 * out["p50_s"], 1), "p99_ms": round(1e3 * out["p99_s"], 1),
+            "wall_s": round(out["wall_s"], 2), "requests": nreq, "arrival_rate_per_s": rate,
+            "max_cases_in_batch": max_cases,
+            "mean_cases_per_batch_run": round(out["case_runs"] / max(out["runs"], 1), 2),
+            "outer_iterations_per_batch_run": chunk,
+            "converged": int(sum(out["converged"].values())), "failed": len(out["failed"]),
+            "tol": tol, "max_outer": max_outer,
+            "cell_iterations_per_s": round(cell_its / out["wall_s"], 1),
+            "contexts_created": out["contexts_created"],
+            "case": "steady lid-driven cavity or channel (50/50) on N^3 hex, N in {16,...,96}, "
+                    "Re log-uniform in [100, 1000], relaxation 0.7/0.3, from rest",
+            "pc_order": "mgcg-mixed", "gs_order": "colored", "assembly": "f32",
+            "dtype": "f32 assembly and pc preconditioner, f64 otherwise", "data": "synthetic"}
+
+
+def serving_line(cfdl, args):
+    out = serving_run(cfdl, args.serving_requests, args.serving_rate, args.tol,
+                      args.serving_max_outer, args.serving_chunk, args.serving_cases)
+    print(json.dumps(dict(out, mode="serving")), flush=True)
 
 
 if __name__ == "__main__":

@@ -16,7 +16,7 @@ or no GPU is visible, the calls raise.
 """
 from ._lib import CfdlError, load_library, library_path
 from .mesh import Mesh
-from .solver import UVWP, FIELDS, BC_WALL, BC_LID, BC_SYMMETRY, BC_INLET, BC_OUTLET
+from .solver import UVWP, Batch, FIELDS, BC_WALL, BC_LID, BC_SYMMETRY, BC_INLET, BC_OUTLET
 
-__all__ = ["CfdlError", "load_library", "library_path", "Mesh", "UVWP", "FIELDS",
+__all__ = ["CfdlError", "load_library", "library_path", "Mesh", "UVWP", "Batch", "FIELDS",
            "BC_WALL", "BC_LID", "BC_SYMMETRY", "BC_INLET", "BC_OUTLET"]

@@ -49,6 +49,7 @@ _SIGS = {
     "cfdl_set_inlet_profile": (C.c_int, [VP, C.c_double, C.c_double, C.c_double, C.c_double,
                                          C.c_double]),
     "cfdl_ctx_destroy": (None, [VP]),
+    "cfdl_ctx_reset": (C.c_int, [VP]),
     "cfdl_set_relaxation": (C.c_int, [VP, C.c_double, C.c_double]),
     "cfdl_set_pc_order": (C.c_int, [VP, C.c_int]),
     "cfdl_set_assembly_precision": (C.c_int, [VP, C.c_int]),
@@ -80,6 +81,15 @@ _SIGS = {
     "cfdl_solve_energy": (C.c_int, [VP]),
     "cfdl_run_energy": (C.c_int, [VP, C.c_int, C.c_int]),
     "cfdl_run": (C.c_int, [VP, C.c_int, C.c_int]),
+    "cfdl_batch_create": (C.c_int, [C.POINTER(VP)]),
+    "cfdl_batch_attach": (C.c_int, [VP, VP]),
+    "cfdl_batch_detach": (C.c_int, [VP, VP]),
+    "cfdl_batch_size": (C.c_int, [VP, PI]),
+    "cfdl_batch_set_block": (C.c_int, [VP, C.c_int]),
+    "cfdl_batch_block": (C.c_int, [VP, PI]),
+    "cfdl_batch_run": (C.c_int, [VP, C.c_int, C.c_int]),
+    "cfdl_batch_synchronize": (C.c_int, [VP]),
+    "cfdl_batch_destroy": (None, [VP]),
     "cfdl_log_count": (C.c_int, [VP, PI]),
     "cfdl_log_get": (C.c_int, [VP, C.c_int, C.c_char_p, PI, PD, PD, PD]),
     "cfdl_log_clear": (C.c_int, [VP]),

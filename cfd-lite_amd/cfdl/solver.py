@@ -274,6 +274,11 @@ class UVWP:
     def synchronize(self):
         check(self._lib.cfdl_synchronize(self._h))
 
+    def reset(self):
+        """Back to the state at creation (fields cleared, log cleared); the
+        mesh, properties, BCs and settings stay."""
+        check(self._lib.cfdl_ctx_reset(self._h))
+
     def log(self):
         """[(name, it, res_i, res_f, res_max)] one per linear solve."""
         n = C.c_int()
@@ -309,6 +314,65 @@ class UVWP:
         if self._h is not None:
             self._lib.cfdl_ctx_destroy(self._h)
             self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Continuous batching of independent cases (include/cfdl.h,
+    cfdl_batch_*): run(ntstep, ncoef) does for every attached UVWP what
+    UVWP.run does for one, with one grouped launch per step for all of them;
+    each case's results equal those of the case run alone, bit for bit.
+    Cases attach and detach between runs (up to 32 at a time)."""
+
+    MAX_CASES = 32
+
+    def __init__(self, block=0):
+        self._lib = load_library()
+        h = VP()
+        check(self._lib.cfdl_batch_create(C.byref(h)))
+        self._h = h
+        self.cases = []
+        if block:
+            self.set_block(block)
+
+    def attach(self, eq):
+        check(self._lib.cfdl_batch_attach(self._h, eq._h))
+        self.cases.append(eq)
+
+    def detach(self, eq):
+        check(self._lib.cfdl_batch_detach(self._h, eq._h))
+        self.cases.remove(eq)
+
+    def __len__(self):
+        n = C.c_int(0)
+        check(self._lib.cfdl_batch_size(self._h, C.byref(n)))
+        return n.value
+
+    def set_block(self, block):
+        """Lane count of the element-wise launches (0: tuned)."""
+        check(self._lib.cfdl_batch_set_block(self._h, int(block)))
+
+    def block(self):
+        b = C.c_int(0)
+        check(self._lib.cfdl_batch_block(self._h, C.byref(b)))
+        return b.value
+
+    def run(self, ntstep, ncoef):
+        check(self._lib.cfdl_batch_run(self._h, int(ntstep), int(ncoef)))
+
+    def synchronize(self):
+        check(self._lib.cfdl_batch_synchronize(self._h))
+
+    def close(self):
+        if self._h is not None:
+            self._lib.cfdl_batch_destroy(self._h)
+            self._h = None
+            self.cases = []
 
     def __del__(self):
         try:

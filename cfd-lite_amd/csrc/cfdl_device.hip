@@ -37,6 +37,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "cfdl_group.h"
 #include "cfdl_internal.h"
 
 namespace cfdl {
@@ -242,8 +243,8 @@ struct Inlet {
   double q[5];
 };
 
-__global__ void k_bc_update(MeshDev m, FieldsDev f, Inlet in) {
-  const int h = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void d_bc_update(int bx_, MeshDev m, FieldsDev f, Inlet in) {
+  const int h = bx_ * blockDim.x + threadIdx.x;
   if (h >= m.nbf) return;
   const int e = m.hcell[h], H = m.ne + h, kind = m.hkind[h];
   const int fg = abs(m.fsg[m.hslot[h]]) - 1;
@@ -295,6 +296,9 @@ __global__ void k_bc_update(MeshDev m, FieldsDev f, Inlet in) {
   f.p[H] = f.p[e];
   f.mip[fg] = 0.0;
 }
+__global__ void k_bc_update(MeshDev m, FieldsDev f, Inlet in) {
+  d_bc_update(blockIdx.x, m, f, in);
+}
 
 // Outlet mass flux of the channel extension: Q_in (inlet faces) and the
 // extrapolated Q_out (outlet faces), each summed in the blocked tree order
@@ -335,8 +339,7 @@ __device__ double wg_blocked_sum(int n, F v, double *s, double *part) {
   for (int i = threadIdx.x; i < nblk; i += kOutThreads) acc = acc + part[i];
   return wg_tree(acc, s);
 }
-__global__ void __launch_bounds__(kOutThreads)
-k_outlet_flux(MeshDev m, double *mip, const int *fin, int nin, const int *fout, int nout) {
+__device__ inline void d_outlet_flux(int bx_, MeshDev m, double *mip, const int *fin, int nin, const int *fout, int nout) {
   __shared__ double s[kOutThreads], part[1024];
   if (nout == 0) return;
   const double qin = wg_blocked_sum(nin, [&](int k) { return mip[fin[k]]; }, s, part);
@@ -349,6 +352,10 @@ k_outlet_flux(MeshDev m, double *mip, const int *fin, int nin, const int *fout, 
     for (int k = threadIdx.x; k < nout; k += kOutThreads)
       mip[fout[k]] = -qin * area_of(m.aip + 3 * fout[k]) / aout;
   }
+}
+__global__ void __launch_bounds__(kOutThreads)
+k_outlet_flux(MeshDev m, double *mip, const int *fin, int nin, const int *fout, int nout) {
+  d_outlet_flux(blockIdx.x, m, mip, fin, nin, fout, nout);
 }
 
 // ---------------------------------------------------------------------------
@@ -485,8 +492,7 @@ __global__ void k_coef_uvw(MeshDev m, FieldsDev f, double dt, double alpha_u) {
 // Interior anb are written by their entry's thread, boundary anb by the cell.
 constexpr int kFaceThreads = 256, kFaceCells = kFaceThreads / 6;
 template <typename T>
-__global__ void __launch_bounds__(kFaceThreads)
-k_coef_uvw6(MeshDev m, FieldsDev f, double dt_, double alpha_u_) {
+__device__ inline void d_coef_uvw6(int bx_, MeshDev m, FieldsDev f, double dt_, double alpha_u_) {
   // T = double: the reference's FP64 assembly bit for bit.  T = float: the
   // same operations in FP32 on the FP32 geometry copies and on the fields
   // rounded to FP32 (the FP32 assembly; results stored as FP64).
@@ -497,7 +503,7 @@ k_coef_uvw6(MeshDev m, FieldsDev f, double dt_, double alpha_u_) {
   __shared__ T st[9][kFaceThreads];  // d, fnb, fl, secondary stress, deferred correction
   __shared__ int sin_[kFaceThreads];  // the entry is interior
   const int lc = threadIdx.x / 6, j = threadIdx.x - 6 * lc;
-  const int e = blockIdx.x * kFaceCells + lc;
+  const int e = bx_ * kFaceCells + lc;
   const bool valid = lc < kFaceCells && e < m.ne;
   if (valid) {
     const int k = 6 * e + j;
@@ -635,6 +641,11 @@ k_coef_uvw6(MeshDev m, FieldsDev f, double dt_, double alpha_u_) {
   for (int q = 0; q < 6; q++) dc = dc - anbk[q];
   f.dc[e] = static_cast<double>(vol / dc);
 }
+template <typename T>
+__global__ void __launch_bounds__(kFaceThreads)
+k_coef_uvw6(MeshDev m, FieldsDev f, double dt_, double alpha_u_) {
+  d_coef_uvw6<T>(blockIdx.x, m, f, dt_, alpha_u_);
+}
 
 // ---------------------------------------------------------------------------
 // Energy equation (src/equations/mod_energy.f90; never solved by the
@@ -658,12 +669,15 @@ __global__ void k_energy_init(MeshDev m, EnergyDev en) {
 // BC setters lid (t = 373) and dirichlet0 (t = 273), :182-212, per halo;
 // construct_energy binds 'top' to lid and every other section to
 // dirichlet0 (:41-47), i.e. the lid-driven cavity's uvwp binding
-__global__ void k_bc_energy(MeshDev m, EnergyDev en) {
-  const int h = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void d_bc_energy(int bx_, MeshDev m, EnergyDev en) {
+  const int h = bx_ * blockDim.x + threadIdx.x;
   if (h >= m.nbf) return;
   const int H = m.ne + h, e = m.hcell[h];
   en.t[H] = m.hkind[h] == CFDL_BC_LID ? 373.0 : 273.0;
   en.h[H] = m.cp[e] * en.t[H];
+}
+__global__ void k_bc_energy(MeshDev m, EnergyDev en) {
+  d_bc_energy(blockIdx.x, m, en);
 }
 
 // calc_coef_energy, :82-169: upwind convection with the face mass fluxes of
@@ -733,8 +747,8 @@ __global__ void k_temperature(MeshDev m, EnergyDev en) {
 // calc_grad + matinv3, mod_solver.f90:8-81: weighted least squares over all
 // CSR neighbours (boundary halos included), one thread per cell.
 // ---------------------------------------------------------------------------
-__global__ void k_grad(MeshDev m, const double *__restrict__ phi, double *__restrict__ grad) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void d_grad(int bx_, MeshDev m, const double *__restrict__ phi, double *__restrict__ grad) {
+  const int e = bx_ * blockDim.x + threadIdx.x;
   if (e >= m.ne) return;
   double a11 = 0.0, a12 = 0.0, a13 = 0.0, a22 = 0.0, a23 = 0.0, a33 = 0.0;
   double g[3] = {0.0, 0.0, 0.0};
@@ -777,13 +791,16 @@ __global__ void k_grad(MeshDev m, const double *__restrict__ phi, double *__rest
   grad[3 * e + 1] = b21 * g[0] + b22 * g[1] + b23 * g[2];
   grad[3 * e + 2] = b31 * g[0] + b32 * g[1] + b33 * g[2];
 }
+__global__ void k_grad(MeshDev m, const double *__restrict__ phi, double *__restrict__ grad) {
+  d_grad(blockIdx.x, m, phi, grad);
+}
 
 // ---------------------------------------------------------------------------
 // calc_mip, mod_uvwp.f90:438-490: one thread per face, owner side, interior
 // faces only (boundary mip is set by the BC setters).
 // ---------------------------------------------------------------------------
-__global__ void k_mip(MeshDev m, FieldsDev f, int rc, double dt) {
-  const int fg = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void d_mip(int bx_, MeshDev m, FieldsDev f, int rc, double dt) {
+  const int fg = bx_ * blockDim.x + threadIdx.x;
   if (fg >= m.nf) return;
   const int k = m.own[fg], e = m.fcell[fg], nb = m.nbr[k];
   if (nb >= m.ne) return;
@@ -810,6 +827,9 @@ __global__ void k_mip(MeshDev m, FieldsDev f, int rc, double dt) {
           rhoip / dt * dip * (f.mip0[fg] - dot3(vip0, nrm) * rhoip * area);
   }
   f.mip[fg] = mip;
+}
+__global__ void k_mip(MeshDev m, FieldsDev f, int rc, double dt) {
+  d_mip(blockIdx.x, m, f, rc, dt);
 }
 
 // ---------------------------------------------------------------------------
@@ -862,14 +882,14 @@ __global__ void k_coef_p(MeshDev m, FieldsDev f) {
 // FP32; the right-hand side, the cell's net inflow (the continuity check),
 // stays FP64 in both.
 template <typename T>
-__global__ void __launch_bounds__(kFaceThreads) k_coef_p6(MeshDev m, FieldsDev f) {
+__device__ inline void d_coef_p6(int bx_, MeshDev m, FieldsDev f) {
   const auto G = geo_of<T>(m);
   const T one = static_cast<T>(1), zero = static_cast<T>(0);
   __shared__ T sd[kFaceThreads];
   __shared__ double sfl[kFaceThreads];
   __shared__ int sin_[kFaceThreads];
   const int lc = threadIdx.x / 6, j = threadIdx.x - 6 * lc;
-  const int e = blockIdx.x * kFaceCells + lc;
+  const int e = bx_ * kFaceCells + lc;
   const bool valid = lc < kFaceCells && e < m.ne;
   if (valid) {
     const int k = 6 * e + j;
@@ -922,19 +942,26 @@ __global__ void __launch_bounds__(kFaceThreads) k_coef_p6(MeshDev m, FieldsDev f
   f.ap[e] = static_cast<double>(ap);
   f.b[e] = b;
 }
+template <typename T>
+__global__ void __launch_bounds__(kFaceThreads) k_coef_p6(MeshDev m, FieldsDev f) {
+  d_coef_p6<T>(blockIdx.x, m, f);
+}
 
 // update_uvwp cell part, mod_uvwp.f90:383-392 (no velocity correction)
 // (config mode scales the correction by alpha_p; alpha_p = 1 is exact)
-__global__ void k_update_cells(int ne, FieldsDev f, double alpha_p) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void d_update_cells(int bx_, int ne, FieldsDev f, double alpha_p) {
+  const int e = bx_ * blockDim.x + threadIdx.x;
   if (e >= ne) return;
   f.p[e] = f.p[e] + alpha_p * f.phic[e];
   for (int i = 0; i < 3; i++) f.gp[3 * e + i] = f.gp[3 * e + i] + alpha_p * f.gpc[3 * e + i];
 }
+__global__ void k_update_cells(int ne, FieldsDev f, double alpha_p) {
+  d_update_cells(blockIdx.x, ne, f, alpha_p);
+}
 
 // update_uvwp face part, mod_uvwp.f90:394-415 (arithmetic-mean rho)
-__global__ void k_update_faces(MeshDev m, FieldsDev f) {
-  const int fg = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void d_update_faces(int bx_, MeshDev m, FieldsDev f) {
+  const int fg = bx_ * blockDim.x + threadIdx.x;
   if (fg >= m.nf) return;
   const int k = m.own[fg], e = m.fcell[fg], nb = m.nbr[k];
   if (nb >= m.ne) return;
@@ -951,32 +978,50 @@ __global__ void k_update_faces(MeshDev m, FieldsDev f) {
   const double dmip = rhoip * area * dip * (f.phic[nb] - f.phic[e]) / dot3(dr, nrm);
   f.mip[fg] = f.mip[fg] - dmip;
 }
+__global__ void k_update_faces(MeshDev m, FieldsDev f) {
+  d_update_faces(blockIdx.x, m, f);
+}
 
 // adjust_pc, mod_uvwp.f90:136-158 with pref = phic(1) (:129)
-__global__ void k_pref(const double *phic, double *scal) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) scal[0] = phic[0];
+__device__ inline void d_pref(int bx_, const double *phic, double *scal) {
+  if (bx_ == 0 && threadIdx.x == 0) scal[0] = phic[0];
 }
-__global__ void k_sub_pref(int ne, double *phic, const double *scal) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_pref(const double *phic, double *scal) {
+  d_pref(blockIdx.x, phic, scal);
+}
+__device__ inline void d_sub_pref(int bx_, int ne, double *phic, const double *scal) {
+  const int e = bx_ * blockDim.x + threadIdx.x;
   if (e < ne) phic[e] = phic[e] - scal[0];
 }
-__global__ void k_halo_from_owner(MeshDev m, double *phi) {
-  const int h = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_sub_pref(int ne, double *phic, const double *scal) {
+  d_sub_pref(blockIdx.x, ne, phic, scal);
+}
+__device__ inline void d_halo_from_owner(int bx_, MeshDev m, double *phi) {
+  const int h = bx_ * blockDim.x + threadIdx.x;
   if (h < m.nbf) phi[m.ne + h] = phi[m.hcell[h]];
 }
+__global__ void k_halo_from_owner(MeshDev m, double *phi) {
+  d_halo_from_owner(blockIdx.x, m, phi);
+}
 
-__global__ void k_fill(double *p, int64_t n, double v) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+__device__ inline void d_fill(int bx_, double *p, int64_t n, double v) {
+  const int64_t i = static_cast<int64_t>(bx_) * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
+}
+__global__ void k_fill(double *p, int64_t n, double v) {
+  d_fill(blockIdx.x, p, n, v);
 }
 __global__ void k_to_f32(const double *src, float *dst, int64_t n) {
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += static_cast<int64_t>(gridDim.x) * blockDim.x)
     dst[i] = static_cast<float>(src[i]);
 }
-__global__ void k_copy(double *dst, const double *src, int64_t n) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+__device__ inline void d_copy(int bx_, double *dst, const double *src, int64_t n) {
+  const int64_t i = static_cast<int64_t>(bx_) * blockDim.x + threadIdx.x;
   if (i < n) dst[i] = src[i];
+}
+__global__ void k_copy(double *dst, const double *src, int64_t n) {
+  d_copy(blockIdx.x, dst, src, n);
 }
 
 // Net outward mass flux of each cell, sum over its CSR entries of
@@ -1000,9 +1045,9 @@ __global__ void k_mass_imbalance(MeshDev m, const double *__restrict__ mip, doub
 // ---------------------------------------------------------------------------
 // cells == nullptr: the cells are base .. base + n - 1 (a colour range).
 template <int NC>
-__global__ void k_gs_level(SysDesc s, const int *__restrict__ cells, int base, int n, double sor,
+__device__ inline void d_gs_level(int bx_, SysDesc s, const int *__restrict__ cells, int base, int n, double sor,
                            const int *__restrict__ active) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = bx_ * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int e = cells ? cells[t] : base + t;
   const int r0 = s.row[e], r1 = s.row[e + 1];
@@ -1039,6 +1084,11 @@ __global__ void k_gs_level(SysDesc s, const int *__restrict__ cells, int base, i
     for (int k = r0; k < r1; k++) sum = sum + s.anb[k] * phi[s.nbr[k]];
     phi[e] = (sum + (sor - 1.0) * a * phi[e]) / a / sor;
   }
+}
+template <int NC>
+__global__ void k_gs_level(SysDesc s, const int *__restrict__ cells, int base, int n, double sor,
+                           const int *__restrict__ active) {
+  d_gs_level<NC>(blockIdx.x, s, cells, base, n, sor, active);
 }
 
 // The same update in FP32 on a correction system: cells of one colour (all
@@ -1472,10 +1522,10 @@ k_gs_tiles_lds(SysDesc s, const int4 *__restrict__ bdesc, const int *__restrict_
 // (calc_residual, :239-251).  Fixed-order tree, so reproducible.
 // rout, when given, receives each cell's r rounded to FP32 (the right-hand
 // side of the mixed-precision correction system).
-__global__ void k_resid_partial(SysDesc s, int chunk, int nblk, int absmax, double *part,
+__device__ inline void d_resid_partial(int bx_, SysDesc s, int chunk, int nblk, int absmax, double *part,
                                 float *rout) {
   __shared__ double ssum[kBlock], smax[kBlock];
-  const int y = blockIdx.y, x = blockIdx.x, tid = threadIdx.x;
+  const int y = blockIdx.y, x = bx_, tid = threadIdx.x;
   const int lo = s.beg[y] + x * chunk;
   const int hi = min(lo + chunk, s.end[y]);
   const double *phi = s.phi[y], *b = s.b[y];
@@ -1519,10 +1569,14 @@ __global__ void k_resid_partial(SysDesc s, int chunk, int nblk, int absmax, doub
     part[2 * (y * nblk + x) + 1] = smax[0];
   }
 }
+__global__ void k_resid_partial(SysDesc s, int chunk, int nblk, int absmax, double *part,
+                                float *rout) {
+  d_resid_partial(blockIdx.x, s, chunk, nblk, absmax, part, rout);
+}
 
-__global__ void k_resid_final(const double *part, int nblk, double *red) {
+__device__ inline void d_resid_final(int bx_, const double *part, int nblk, double *red) {
   __shared__ double ssum[kBlock], smax[kBlock];
-  const int y = blockIdx.x, tid = threadIdx.x;
+  const int y = bx_, tid = threadIdx.x;
   double sum = 0.0, mx = 0.0;
   for (int i = tid; i < nblk; i += blockDim.x) {
     sum = sum + part[2 * (y * nblk + i)];
@@ -1543,9 +1597,12 @@ __global__ void k_resid_final(const double *part, int nblk, double *red) {
     red[2 * y + 1] = smax[0];
   }
 }
+__global__ void k_resid_final(const double *part, int nblk, double *red) {
+  d_resid_final(blockIdx.x, part, nblk, red);
+}
 
 // solve_gs bookkeeping, mod_solver.f90:271-287 and :309-322
-__global__ void k_gs_init(GsCtl *c, const double *red, int ny, double ne, int nit) {
+__device__ inline void d_gs_init(int bx_, GsCtl *c, const double *red, int ny, double ne, int nit) {
   if (threadIdx.x != 0) return;
   for (int y = 0; y < ny; y++) {
     const double ri = sqrt(red[2 * y] / ne);
@@ -1557,7 +1614,10 @@ __global__ void k_gs_init(GsCtl *c, const double *red, int ny, double ne, int ni
     c->active[y] = (0 < nit && ri > c->target[y]) ? 1 : 0;
   }
 }
-__global__ void k_gs_iter_end(GsCtl *c, const double *red, int ny, double ne, int nit) {
+__global__ void k_gs_init(GsCtl *c, const double *red, int ny, double ne, int nit) {
+  d_gs_init(blockIdx.x, c, red, ny, ne, nit);
+}
+__device__ inline void d_gs_iter_end(int bx_, GsCtl *c, const double *red, int ny, double ne, int nit) {
   if (threadIdx.x != 0) return;
   for (int y = 0; y < ny; y++) {
     if (!c->active[y]) continue;
@@ -1567,10 +1627,13 @@ __global__ void k_gs_iter_end(GsCtl *c, const double *red, int ny, double ne, in
     c->active[y] = (c->it[y] < nit && c->res_f[y] > c->target[y]) ? 1 : 0;
   }
 }
+__global__ void k_gs_iter_end(GsCtl *c, const double *red, int ny, double ne, int nit) {
+  d_gs_iter_end(blockIdx.x, c, red, ny, ne, nit);
+}
 struct Names {
   char n[3][8];
 };
-__global__ void k_log_gs(const GsCtl *c, int ny, Names nm, LogLine *log, int *nlog) {
+__device__ inline void d_log_gs(int bx_, const GsCtl *c, int ny, Names nm, LogLine *log, int *nlog) {
   if (threadIdx.x != 0) return;
   for (int y = 0; y < ny; y++) {
     const int k = *nlog;
@@ -1583,6 +1646,9 @@ __global__ void k_log_gs(const GsCtl *c, int ny, Names nm, LogLine *log, int *nl
     l.res_max = c->res_max[y];
     *nlog = k + 1;
   }
+}
+__global__ void k_log_gs(const GsCtl *c, int ny, Names nm, LogLine *log, int *nlog) {
+  d_log_gs(blockIdx.x, c, ny, nm, log, nlog);
 }
 
 // multi_subdomain_solver bookkeeping, mod_solver.f90:140-181 (Q1: res_f_tot
@@ -1695,8 +1761,8 @@ struct TileMap {
   double *phi[3];
   double *phi_m[3];
 };
-__global__ void k_tile_gather(TileMap t) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void d_tile_gather(int bx_, TileMap t) {
+  const int i = bx_ * blockDim.x + threadIdx.x;
   if (i < t.ne) {
     const int g = t.l2g[i];
     t.ap_m[i] = t.ap[g];
@@ -1711,12 +1777,168 @@ __global__ void k_tile_gather(TileMap t) {
     for (int y = 0; y < t.ny; y++) t.phi_m[y][i] = t.phi[y][i];
   }
 }
-__global__ void k_tile_scatter(TileMap t) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_tile_gather(TileMap t) {
+  d_tile_gather(blockIdx.x, t);
+}
+__device__ inline void d_tile_scatter(int bx_, TileMap t) {
+  const int i = bx_ * blockDim.x + threadIdx.x;
   if (i >= t.ne) return;
   const int g = t.l2g[i];
   for (int y = 0; y < t.ny; y++) t.phi[y][g] = t.phi_m[y][i];
 }
+__global__ void k_tile_scatter(TileMap t) {
+  d_tile_scatter(blockIdx.x, t);
+}
+
+
+// ---------------------------------------------------------------------------
+// Batched iterations (continuous batching of independent cases): one launch
+// per step of the SIMPLE iteration covers every case of a batch.  Case k
+// owns the blocks [b0[k], b0[k+1]) of a launch (cfdl_group.h) and runs them
+// as blocks 0 .. of its own grid with its own descriptor, so its arithmetic
+// is that of the same step on the case alone, bit for bit.
+// ---------------------------------------------------------------------------
+constexpr int kMaxColours = 16;  // colours of a case's momentum system
+struct BSlot {
+  MeshDev md;
+  FieldsDev fd;
+  EnergyDev en;
+  Inlet in;
+  double dt, alpha_u, alpha_p;
+  const int *fin, *fout;
+  int nin, nout;
+  int nblk_g, chunk_g, nit, nq;
+  int gq[kMaxColours + 1];  // colour ranges of the colour numbering
+  TileMap tmap;             // u, v, w and their system in the colour numbering
+  SysDesc sys;              // the same system as the sweep and residual kernels see it
+  GsCtl *gctl;
+  double *part, *red, *scal;
+  LogLine *log;
+  int *nlog;
+};
+
+#define BSLOT                                                                       \
+  int bx;                                                                           \
+  const BSlot &x = S[group_slot(g, bx)]
+
+__global__ void gb_bc_update(GroupMap g, const BSlot *__restrict__ S) {
+  BSLOT;
+  d_bc_update(bx, x.md, x.fd, x.in);
+}
+__global__ void __launch_bounds__(kOutThreads)
+gb_outlet_flux(GroupMap g, const BSlot *__restrict__ S) {
+  BSLOT;
+  d_outlet_flux(bx, x.md, x.fd.mip, x.fin, x.nin, x.fout, x.nout);
+}
+__global__ void gb_bc_energy(GroupMap g, const BSlot *__restrict__ S) {
+  BSLOT;
+  d_bc_energy(bx, x.md, x.en);
+}
+template <typename T>
+__global__ void __launch_bounds__(kFaceThreads)
+gb_coef_uvw6(GroupMap g, const BSlot *__restrict__ S) {
+  BSLOT;
+  d_coef_uvw6<T>(bx, x.md, x.fd, x.dt, x.alpha_u);
+}
+__global__ void gb_tile_gather(GroupMap g, const BSlot *__restrict__ S) {
+  BSLOT;
+  d_tile_gather(bx, x.tmap);
+}
+__global__ void gb_tile_scatter(GroupMap g, const BSlot *__restrict__ S) {
+  BSLOT;
+  d_tile_scatter(bx, x.tmap);
+}
+// grid (blocks, 3): the three right-hand sides in y, as k_resid_partial
+__global__ void __launch_bounds__(kBlock) gb_resid_partial(GroupMap g, const BSlot *__restrict__ S) {
+  BSLOT;
+  d_resid_partial(bx, x.sys, x.chunk_g, x.nblk_g, 1, x.part, nullptr);
+}
+// grid (cases, 3): one block per case and right-hand side
+__global__ void __launch_bounds__(kBlock) gb_resid_final(GroupMap g, const BSlot *__restrict__ S) {
+  BSLOT;
+  d_resid_final(blockIdx.y, x.part, x.nblk_g, x.red);
+}
+__global__ void gb_gs_init(GroupMap g, const BSlot *__restrict__ S) {
+  BSLOT;
+  d_gs_init(bx, x.gctl, x.red, 3, static_cast<double>(x.md.ne), x.nit);
+}
+__global__ void gb_gs_iter_end(GroupMap g, const BSlot *__restrict__ S) {
+  BSLOT;
+  d_gs_iter_end(bx, x.gctl, x.red, 3, static_cast<double>(x.md.ne), x.nit);
+}
+// colour i of the case's forward order, or of its reverse (backward)
+__global__ void gb_gs_colour(GroupMap g, const BSlot *__restrict__ S, int i, int backward,
+                             double sor) {
+  BSLOT;
+  const int q = backward ? x.nq - 1 - i : i;
+  const int b0 = x.gq[q];
+  d_gs_level<3>(bx, x.sys, nullptr, b0, x.gq[q + 1] - b0, sor, x.gctl->active);
+}
+// flags[k]: case k's momentum solve is still running
+__global__ void gb_gs_flags(int n, const BSlot *__restrict__ S, int *flags) {
+  const int k = threadIdx.x;
+  if (k < n) {
+    const int *a = S[k].gctl->active;
+    flags[k] = a[0] | a[1] | a[2];
+  }
+}
+__global__ void gb_log_gs(GroupMap g, const BSlot *__restrict__ S, Names nm) {
+  BSLOT;
+  d_log_gs(bx, x.gctl, 3, nm, x.log, x.nlog);
+}
+// which: 0 u -> gu, 1 v -> gv, 2 w -> gw, 3 phic -> gpc
+__global__ void gb_grad(GroupMap g, const BSlot *__restrict__ S, int which) {
+  BSLOT;
+  const FieldsDev &f = x.fd;
+  const double *phi = which == 0 ? f.u : which == 1 ? f.v : which == 2 ? f.w : f.phic;
+  double *gr = which == 0 ? f.gu : which == 1 ? f.gv : which == 2 ? f.gw : f.gpc;
+  d_grad(bx, x.md, phi, gr);
+}
+__global__ void gb_mip(GroupMap g, const BSlot *__restrict__ S) {
+  BSLOT;
+  d_mip(bx, x.md, x.fd, 1, x.dt);
+}
+template <typename T>
+__global__ void __launch_bounds__(kFaceThreads) gb_coef_p6(GroupMap g, const BSlot *__restrict__ S) {
+  BSLOT;
+  d_coef_p6<T>(bx, x.md, x.fd);
+}
+__global__ void gb_zero_phic(GroupMap g, const BSlot *__restrict__ S) {
+  BSLOT;
+  d_fill(bx, x.fd.phic, int64_t{x.md.ne} + x.md.nbf, 0.0);
+}
+__global__ void gb_pref(GroupMap g, const BSlot *__restrict__ S) {
+  BSLOT;
+  d_pref(bx, x.fd.phic, x.scal);
+}
+__global__ void gb_sub_pref(GroupMap g, const BSlot *__restrict__ S) {
+  BSLOT;
+  d_sub_pref(bx, x.md.ne, x.fd.phic, x.scal);
+}
+__global__ void gb_halo_from_owner(GroupMap g, const BSlot *__restrict__ S) {
+  BSLOT;
+  d_halo_from_owner(bx, x.md, x.fd.phic);
+}
+__global__ void gb_update_cells(GroupMap g, const BSlot *__restrict__ S) {
+  BSLOT;
+  d_update_cells(bx, x.md.ne, x.fd, x.alpha_p);
+}
+__global__ void gb_update_faces(GroupMap g, const BSlot *__restrict__ S) {
+  BSLOT;
+  d_update_faces(bx, x.md, x.fd);
+}
+// update_time: which 0 u0 <- u, 1 v0 <- v, 2 w0 <- w, 3 mip0 <- mip, 4 h0 <- h
+__global__ void gb_update_time(GroupMap g, const BSlot *__restrict__ S, int which) {
+  BSLOT;
+  const FieldsDev &f = x.fd;
+  const int64_t n = which == 3 ? int64_t{x.md.nf} : int64_t{x.md.ne} + x.md.nbf;
+  double *dst = which == 0 ? f.u0 : which == 1 ? f.v0 : which == 2 ? f.w0 : which == 3 ? f.mip0
+                                                                                         : x.en.h0;
+  const double *src = which == 0 ? f.u : which == 1 ? f.v : which == 2 ? f.w : which == 3 ? f.mip
+                                                                                         : x.en.h;
+  d_copy(bx, dst, src, n);
+}
+#undef BSLOT
 
 }  // namespace cfdl
 
@@ -1745,6 +1967,10 @@ struct DevBuf {
 struct cfdl_ctx {
   int dev = 0;
   hipStream_t st = nullptr;
+  // while the case is attached to a batch (cfdl_batch_attach) its work runs
+  // on the batch's stream; own_st is the context's own stream meanwhile
+  cfdl_batch *batch = nullptr;
+  hipStream_t own_st = nullptr;
   int ne = 0, nf = 0, nbf = 0, L = 0, nsubd = 1, nit = 100;
   double dt = 0.01;
   double alpha_u = 1.0, alpha_p = 1.0;
@@ -2993,6 +3219,7 @@ int cfdl_ctx_create(int ne, int nf, int nbf, const int *ef2nb_idx, const int *ef
 
 void cfdl_ctx_destroy(cfdl_ctx *c) {
   if (!c) return;
+  if (c->batch) (void)cfdl_batch_detach(c->batch, c);
   (void)hipStreamSynchronize(c->st);
   for (auto &m : c->mg) cfdl::mgcg_destroy(m);
   if (c->host_flags) (void)hipHostFree(c->host_flags);
@@ -3246,6 +3473,22 @@ int cfdl_get_field(cfdl_ctx *c, int field, double *dst, int64_t n) {
 int cfdl_synchronize(cfdl_ctx *c) {
   CALL(check_ctx(c));
   HIPCHK(hipStreamSynchronize(c->st));
+  return CFDL_OK;
+}
+
+int cfdl_ctx_reset(cfdl_ctx *c) {
+  CALL(check_ctx(c));
+  for (int f = 0; f < CFDL_NFIELDS; f++)
+    hipLaunchKernelGGL(k_fill, dim3(nblocks(c->fsize[f])), dim3(kBlock), 0, c->st, c->fields[f],
+                       c->fsize[f], 0.0);
+  hipLaunchKernelGGL(k_energy_init, dim3(nblocks(int64_t{c->ne} + c->nbf)), dim3(kBlock), 0, c->st,
+                     c->md, c->en);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->st));
+  const int zero = 0;
+  HIPCHK(hipMemcpy(c->dlog_n, &zero, sizeof(int), hipMemcpyHostToDevice));
+  c->hlog.clear();
+  c->outer = 0;
   return CFDL_OK;
 }
 
@@ -3666,6 +3909,530 @@ int cfdl_stats_reset(cfdl_ctx *c) {
   c->gs_launches = c->pc_launches = c->pc_cells = c->outer = c->pc_sweeps = 0;
   c->nsamples = 0;
   return CFDL_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Batches (continuous batching of independent cases, an extension with no
+// counterpart in the reference, whose program solves one case per run)
+// ===========================================================================
+// A batch runs the SIMPLE loop of cfdl_run for every attached context at
+// once: each step is one grouped launch over all cases (BSlot descriptors in
+// device memory, rebuilt when a case's settings or the composition change),
+// the momentum and pressure-correction solves stop per case, and the host
+// waits once per solver iteration for all cases together.  Cases attach and
+// detach between calls, so a serving loop admits new cases while others are
+// part-way through their solves.  Requirements of a case: hexahedral rows,
+// the colour order of solve_gs, a multigrid-preconditioned pc solve, and the
+// batch's assembly and preconditioner precision (set by its first case).
+struct cfdl_batch {
+  int dev = 0;
+  hipStream_t st = nullptr;
+  std::vector<cfdl_ctx *> ctx;
+  int asm32 = -1, fp32 = -1;
+  cfdl::MgGroup *mg = nullptr;
+  BSlot *d_slots = nullptr;
+  int *d_flags = nullptr;
+  int *h_flags = nullptr;  // pinned, two sets of kMaxGroup
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  std::vector<char> key;   // the descriptors on the device
+  int block = 0;           // lane count of the element-wise launches; 0 = tuned
+  int tuned = 0;           // the tuner's choice for the current composition
+  bool tuning = false;     // tuner events recorded, not yet read
+  hipEvent_t tev[6] = {};
+  int64_t runs = 0;
+};
+
+namespace {
+constexpr int kTuneBlocks[5] = {64, 128, 256, 512, 1024};
+inline int nbk(int64_t n, int blk) { return static_cast<int>((n + blk - 1) / blk); }
+
+// the descriptor of a case (its device pointers and settings)
+int batch_slot(cfdl_ctx *c, BSlot *s) {
+  std::memset(static_cast<void *>(s), 0, sizeof(BSlot));
+  s->md = c->md;
+  s->fd = c->fd;
+  s->en = c->en;
+  s->in = c->inlet;
+  s->dt = c->dt;
+  s->alpha_u = c->alpha_u;
+  s->alpha_p = c->alpha_p;
+  s->fin = c->fin;
+  s->fout = c->fout;
+  s->nin = c->nin;
+  s->nout = c->nout;
+  s->nblk_g = c->nblk_g;
+  s->chunk_g = c->chunk_g;
+  s->nit = c->nit;
+  s->nq = static_cast<int>(c->gcol_ptr.size()) - 1;
+  if (s->nq < 1 || s->nq > kMaxColours) {
+    set_error("cfdl_batch: a case's momentum system has " + std::to_string(s->nq) +
+              " colours (1 .. " + std::to_string(kMaxColours) + " allowed)");
+    return CFDL_ESTATE;
+  }
+  for (int q = 0; q <= s->nq; q++) s->gq[q] = c->gcol_ptr[q];
+  TileMap &tm = s->tmap;
+  tm.ne = c->ne;
+  tm.nbf = c->nbf;
+  tm.ny = 3;
+  tm.l2g = c->l2g_gc;
+  tm.rowm = c->rowgc;
+  tm.rowg = c->md.row;
+  tm.ap = c->fd.ap;
+  tm.anb = c->fd.anb;
+  tm.ap_m = c->ap_m;
+  tm.anb_m = c->anb_m;
+  tm.rap_m = c->rap_m;
+  double *phi[3] = {c->fd.u, c->fd.v, c->fd.w};
+  const double *b[3] = {c->fd.bu, c->fd.bv, c->fd.bw};
+  SysDesc &sy = s->sys;
+  sy.ny = 3;
+  for (int y = 0; y < 3; y++) {
+    tm.b[y] = b[y];
+    tm.b_m[y] = c->b_m[y];
+    tm.phi[y] = phi[y];
+    tm.phi_m[y] = c->phi_m[y];
+    sy.beg[y] = 0;
+    sy.end[y] = c->ne;
+    sy.phi[y] = c->phi_m[y];
+    sy.b[y] = c->b_m[y];
+  }
+  sy.row = c->rowgc;
+  sy.nbr = c->nbrgc;
+  sy.ap = c->ap_m;
+  sy.anb = c->anb_m;
+  sy.rap = c->rap_m;
+  s->gctl = c->gctl;
+  s->part = c->part;
+  s->red = c->red;
+  s->scal = c->scal;
+  s->log = c->dlog;
+  s->nlog = c->dlog_n;
+  return CFDL_OK;
+}
+
+int batch_check(cfdl_batch *b, cfdl_ctx *c) {
+  if (c->dev != b->dev) {
+    set_error("cfdl_batch: the case is on another device");
+    return CFDL_EINVAL;
+  }
+  if (!c->mhex) {
+    set_error("cfdl_batch: cases need hexahedral rows (six entries each)");
+    return CFDL_ESTATE;
+  }
+  if (c->gs_order != CFDL_GS_COLORED) {
+    set_error("cfdl_batch: cases need the colour order of solve_gs (cfdl_set_gs_order)");
+    return CFDL_ESTATE;
+  }
+  if (c->pc_order != CFDL_PC_MGCG && c->pc_order != CFDL_PC_MGCG_MIXED) {
+    set_error("cfdl_batch: cases need a multigrid-preconditioned pc solve (cfdl_set_pc_order)");
+    return CFDL_ESTATE;
+  }
+  const int fp32 = c->pc_order == CFDL_PC_MGCG_MIXED ? 1 : 0, asm32 = c->asm32 ? 1 : 0;
+  if (!c->mg[fp32]) {
+    set_error("cfdl_batch: the case has no multigrid hierarchy");
+    return CFDL_ESTATE;
+  }
+  // the precision of the batch is its first case's
+  const cfdl_ctx *c0 = b->ctx.empty() ? c : b->ctx[0];
+  if (fp32 != (c0->pc_order == CFDL_PC_MGCG_MIXED ? 1 : 0) || asm32 != (c0->asm32 ? 1 : 0)) {
+    set_error("cfdl_batch: the cases of a batch share the assembly and pc precision");
+    return CFDL_ESTATE;
+  }
+  if (std::isinf(c->dt) && c->alpha_u == 1.0) {
+    set_error("cfdl_batch: dt = infinity (steady) needs momentum under-relaxation alpha_u < 1");
+    return CFDL_ESTATE;
+  }
+  return CFDL_OK;
+}
+
+// descriptors of every case, uploaded when they differ from the device copy
+int batch_sync_slots(cfdl_batch *b) {
+  const int n = static_cast<int>(b->ctx.size());
+  std::vector<BSlot> h(n);
+  for (int k = 0; k < n; k++) {
+    CALL(batch_check(b, b->ctx[k]));
+    CALL(batch_slot(b->ctx[k], &h[k]));
+  }
+  if (n > 0) {
+    b->fp32 = b->ctx[0]->pc_order == CFDL_PC_MGCG_MIXED ? 1 : 0;
+    b->asm32 = b->ctx[0]->asm32 ? 1 : 0;
+  }
+  const char *kb = reinterpret_cast<const char *>(h.data());
+  std::vector<char> key(kb, kb + sizeof(BSlot) * n);
+  if (key != b->key) {
+    HIPCHK(hipStreamSynchronize(b->st));
+    if (n > 0) HIPCHK(hipMemcpy(b->d_slots, h.data(), sizeof(BSlot) * n, hipMemcpyHostToDevice));
+    b->key.swap(key);
+  }
+  return CFDL_OK;
+}
+
+// The batch's grouped launches: one GroupMap per launch from the per-case
+// block counts.
+struct BatchLaunch {
+  cfdl_batch *b;
+  int n;
+  GroupMap g;
+  std::vector<int> ids, cnt;
+  explicit BatchLaunch(cfdl_batch *b_) : b(b_), n(static_cast<int>(b_->ctx.size())), g{} {
+    ids.resize(n);
+    cnt.resize(n);
+    for (int k = 0; k < n; k++) ids[k] = k;
+  }
+  template <typename F>
+  int map(F blocks_of) {
+    for (int k = 0; k < n; k++) cnt[k] = blocks_of(b->ctx[k]);
+    return group_map(g, ids.data(), cnt.data(), n);
+  }
+};
+
+// solve_gs_uvw of every case in the colour order (solve_gs_n with three
+// right-hand sides): iteration it+1 is queued before the host reads the
+// flags after iteration it; an iteration queued after a case stopped
+// changes nothing of it (its sweeps skip, its residual is recomputed from
+// the same values and not used)
+int batch_solve_gs_uvw(cfdl_batch *b, int blk) {
+  BatchLaunch L(b);
+  const BSlot *S = b->d_slots;
+  hipStream_t st = b->st;
+  const int n = L.n;
+  int nb = L.map([&](cfdl_ctx *c) { return nbk(int64_t{c->ne} + c->nbf, blk); });
+  hipLaunchKernelGGL(gb_tile_gather, dim3(nb), dim3(blk), 0, st, L.g, S);
+  auto residual = [&]() {
+    const int np = L.map([&](cfdl_ctx *c) { return c->nblk_g; });
+    hipLaunchKernelGGL(gb_resid_partial, dim3(np, 3), dim3(kBlock), 0, st, L.g, S);
+    const int nf = L.map([&](cfdl_ctx *) { return 1; });
+    hipLaunchKernelGGL(gb_resid_final, dim3(nf, 3), dim3(kBlock), 0, st, L.g, S);
+  };
+  residual();
+  nb = L.map([&](cfdl_ctx *) { return 1; });
+  hipLaunchKernelGGL(gb_gs_init, dim3(nb), dim3(64), 0, st, L.g, S);
+  HIPCHK(hipGetLastError());
+  int nit = 0, nq = 0;
+  for (cfdl_ctx *c : b->ctx) {
+    nit = std::max(nit, c->nit);
+    nq = std::max(nq, static_cast<int>(c->gcol_ptr.size()) - 1);
+  }
+  for (int it = 0; it < nit; it++) {
+    for (int d = 0; d < 2; d++)
+      for (int i = 0; i < nq; i++) {
+        nb = L.map([&](cfdl_ctx *c) {
+          const int q_n = static_cast<int>(c->gcol_ptr.size()) - 1;
+          if (i >= q_n) return 0;
+          const int q = d == 0 ? i : q_n - 1 - i;
+          return nbk(c->gcol_ptr[q + 1] - c->gcol_ptr[q], blk);
+        });
+        if (nb > 0) hipLaunchKernelGGL(gb_gs_colour, dim3(nb), dim3(blk), 0, st, L.g, S, i, d, 1.0);
+      }
+    residual();
+    nb = L.map([&](cfdl_ctx *) { return 1; });
+    hipLaunchKernelGGL(gb_gs_iter_end, dim3(nb), dim3(64), 0, st, L.g, S);
+    hipLaunchKernelGGL(gb_gs_flags, dim3(1), dim3(kMaxGroup), 0, st, n, S, b->d_flags);
+    HIPCHK(hipGetLastError());
+    int *hf = b->h_flags + (it & 1) * kMaxGroup;
+    HIPCHK(hipMemcpyAsync(hf, b->d_flags, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipEventRecord(b->ev[it & 1], st));
+    if (it > 0) {
+      HIPCHK(hipEventSynchronize(b->ev[(it - 1) & 1]));
+      const int *pf = b->h_flags + ((it - 1) & 1) * kMaxGroup;
+      bool any = false;
+      for (int k = 0; k < n; k++) any = any || pf[k] != 0;
+      if (!any) break;
+    }
+  }
+  nb = L.map([&](cfdl_ctx *c) { return nbk(c->ne, blk); });
+  hipLaunchKernelGGL(gb_tile_scatter, dim3(nb), dim3(blk), 0, st, L.g, S);
+  Names nm{};
+  std::strncpy(nm.n[0], "u", 7);
+  std::strncpy(nm.n[1], "v", 7);
+  std::strncpy(nm.n[2], "w", 7);
+  nb = L.map([&](cfdl_ctx *) { return 1; });
+  hipLaunchKernelGGL(gb_log_gs, dim3(nb), dim3(64), 0, st, L.g, S, nm);
+  HIPCHK(hipGetLastError());
+  return CFDL_OK;
+}
+
+// The tuner: at the first gradient step after the composition changed,
+// the gradient of u is launched at each candidate lane count (three times
+// each; the launches recompute the same values) between events; the
+// fastest is read at the end of the run and used for every element-wise
+// launch of the batch from then on.
+int batch_grad_u(cfdl_batch *b, int blk) {
+  BatchLaunch L(b);
+  const BSlot *S = b->d_slots;
+  if (b->block == 0 && b->tuned == 0 && !b->tuning) {
+    for (int j = 0; j < 5; j++) {
+      const int nb = L.map([&](cfdl_ctx *c) { return nbk(c->ne, kTuneBlocks[j]); });
+      HIPCHK(hipEventRecord(b->tev[j], b->st));
+      for (int r = 0; r < 3; r++)
+        hipLaunchKernelGGL(gb_grad, dim3(nb), dim3(kTuneBlocks[j]), 0, b->st, L.g, S, 0);
+    }
+    HIPCHK(hipEventRecord(b->tev[5], b->st));
+    b->tuning = true;
+  }
+  const int nb = L.map([&](cfdl_ctx *c) { return nbk(c->ne, blk); });
+  hipLaunchKernelGGL(gb_grad, dim3(nb), dim3(blk), 0, b->st, L.g, S, 0);
+  HIPCHK(hipGetLastError());
+  return CFDL_OK;
+}
+int batch_tune_read(cfdl_batch *b) {
+  if (!b->tuning) return CFDL_OK;
+  HIPCHK(hipEventSynchronize(b->tev[5]));
+  float best = 0.f;
+  int bj = 2;
+  for (int j = 0; j < 5; j++) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, b->tev[j], b->tev[j + 1]));
+    if (j == 0 || ms < best) {
+      best = ms;
+      bj = j;
+    }
+  }
+  b->tuned = kTuneBlocks[bj];
+  b->tuning = false;
+  return CFDL_OK;
+}
+
+int batch_iteration(cfdl_batch *b) {
+  BatchLaunch L(b);
+  const BSlot *S = b->d_slots;
+  hipStream_t st = b->st;
+  const int blk = b->block > 0 ? b->block : b->tuned > 0 ? b->tuned : kBlock;
+  int nb = 0;
+  // update_boundaries
+  nb = L.map([&](cfdl_ctx *c) { return nbk(c->nbf, blk); });
+  if (nb > 0) hipLaunchKernelGGL(gb_bc_update, dim3(nb), dim3(blk), 0, st, L.g, S);
+  nb = L.map([&](cfdl_ctx *c) { return c->nbf > 0 && c->nout > 0 ? 1 : 0; });
+  if (nb > 0) hipLaunchKernelGGL(gb_outlet_flux, dim3(nb), dim3(kOutThreads), 0, st, L.g, S);
+  nb = L.map([&](cfdl_ctx *c) { return nbk(c->nbf, blk); });
+  if (nb > 0) hipLaunchKernelGGL(gb_bc_energy, dim3(nb), dim3(blk), 0, st, L.g, S);
+  // solve_uvwp
+  nb = L.map([&](cfdl_ctx *c) { return (c->ne + kFaceCells - 1) / kFaceCells; });
+  if (b->asm32)
+    hipLaunchKernelGGL(gb_coef_uvw6<float>, dim3(nb), dim3(kFaceThreads), 0, st, L.g, S);
+  else
+    hipLaunchKernelGGL(gb_coef_uvw6<double>, dim3(nb), dim3(kFaceThreads), 0, st, L.g, S);
+  HIPCHK(hipGetLastError());
+  CALL(batch_solve_gs_uvw(b, blk));
+  CALL(batch_grad_u(b, blk));
+  nb = L.map([&](cfdl_ctx *c) { return nbk(c->ne, blk); });
+  hipLaunchKernelGGL(gb_grad, dim3(nb), dim3(blk), 0, st, L.g, S, 1);
+  hipLaunchKernelGGL(gb_grad, dim3(nb), dim3(blk), 0, st, L.g, S, 2);
+  nb = L.map([&](cfdl_ctx *c) { return nbk(c->nf, blk); });
+  hipLaunchKernelGGL(gb_mip, dim3(nb), dim3(blk), 0, st, L.g, S);
+  nb = L.map([&](cfdl_ctx *c) { return (c->ne + kFaceCells - 1) / kFaceCells; });
+  if (b->asm32)
+    hipLaunchKernelGGL(gb_coef_p6<float>, dim3(nb), dim3(kFaceThreads), 0, st, L.g, S);
+  else
+    hipLaunchKernelGGL(gb_coef_p6<double>, dim3(nb), dim3(kFaceThreads), 0, st, L.g, S);
+  nb = L.map([&](cfdl_ctx *c) { return nbk(int64_t{c->ne} + c->nbf, blk); });
+  hipLaunchKernelGGL(gb_zero_phic, dim3(nb), dim3(blk), 0, st, L.g, S);
+  HIPCHK(hipGetLastError());
+  {
+    const int n = L.n;
+    std::vector<cfdl::MgCgSolver *> sv(n);
+    std::vector<cfdl::MgCgSystem> sys(n);
+    for (int k = 0; k < n; k++) {
+      cfdl_ctx *c = b->ctx[k];
+      sv[k] = c->mg[b->fp32];
+      cfdl::MgCgSystem &y = sys[k];
+      y = cfdl::MgCgSystem{};
+      y.ne = c->ne;
+      y.nbf = c->nbf;
+      y.nit = c->nit;
+      y.row = c->md.row;
+      y.nbr = c->md.nbr;
+      y.ap = c->fd.ap;
+      y.anb = c->fd.anb;
+      y.b = c->fd.b;
+      y.phi = c->fd.phic;
+      std::strncpy(y.name, "pc", 7);
+      y.log = c->dlog;
+      y.nlog = c->dlog_n;
+    }
+    CALL(cfdl::mgcg_solve_group(b->mg, sv.data(), sys.data(), n, st, blk));
+  }
+  // adjust_pc, calc_grad(phic), update_uvwp
+  nb = L.map([&](cfdl_ctx *) { return 1; });
+  hipLaunchKernelGGL(gb_pref, dim3(nb), dim3(64), 0, st, L.g, S);
+  nb = L.map([&](cfdl_ctx *c) { return nbk(c->ne, blk); });
+  hipLaunchKernelGGL(gb_sub_pref, dim3(nb), dim3(blk), 0, st, L.g, S);
+  nb = L.map([&](cfdl_ctx *c) { return nbk(c->nbf, blk); });
+  if (nb > 0) hipLaunchKernelGGL(gb_halo_from_owner, dim3(nb), dim3(blk), 0, st, L.g, S);
+  nb = L.map([&](cfdl_ctx *c) { return nbk(c->ne, blk); });
+  hipLaunchKernelGGL(gb_grad, dim3(nb), dim3(blk), 0, st, L.g, S, 3);
+  hipLaunchKernelGGL(gb_update_cells, dim3(nb), dim3(blk), 0, st, L.g, S);
+  nb = L.map([&](cfdl_ctx *c) { return nbk(c->nf, blk); });
+  hipLaunchKernelGGL(gb_update_faces, dim3(nb), dim3(blk), 0, st, L.g, S);
+  HIPCHK(hipGetLastError());
+  for (cfdl_ctx *c : b->ctx) c->outer++;
+  return CFDL_OK;
+}
+
+int batch_update_time(cfdl_batch *b) {
+  BatchLaunch L(b);
+  const int blk = b->block > 0 ? b->block : b->tuned > 0 ? b->tuned : kBlock;
+  for (int w = 0; w < 5; w++) {
+    const int nb = L.map([&](cfdl_ctx *c) {
+      return nbk(w == 3 ? int64_t{c->nf} : int64_t{c->ne} + c->nbf, blk);
+    });
+    hipLaunchKernelGGL(gb_update_time, dim3(nb), dim3(blk), 0, b->st, L.g, b->d_slots, w);
+  }
+  HIPCHK(hipGetLastError());
+  return CFDL_OK;
+}
+
+void batch_free(cfdl_batch *b) {
+  if (b->mg) cfdl::mgcg_group_destroy(b->mg);
+  if (b->d_slots) (void)hipFree(b->d_slots);
+  if (b->d_flags) (void)hipFree(b->d_flags);
+  if (b->h_flags) (void)hipHostFree(b->h_flags);
+  for (auto &e : b->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto &e : b->tev)
+    if (e) (void)hipEventDestroy(e);
+  if (b->st) (void)hipStreamDestroy(b->st);
+  delete b;
+}
+}  // namespace
+
+extern "C" {
+
+int cfdl_batch_create(cfdl_batch **out) {
+  if (!out) {
+    set_error("cfdl_batch_create: null output");
+    return CFDL_EINVAL;
+  }
+  *out = nullptr;
+  cfdl_batch *b = new cfdl_batch();
+  auto fail = [&](hipError_t e, const char *what) {
+    set_error(std::string("cfdl_batch_create: ") + what + ": " + hipGetErrorString(e));
+    batch_free(b);
+    return CFDL_EHIP;
+  };
+  hipError_t e = hipGetDevice(&b->dev);
+  if (e != hipSuccess) return fail(e, "hipGetDevice");
+  if ((e = hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking)) != hipSuccess)
+    return fail(e, "hipStreamCreateWithFlags");
+  if ((e = hipMalloc(reinterpret_cast<void **>(&b->d_slots), sizeof(BSlot) * kMaxGroup)) !=
+      hipSuccess)
+    return fail(e, "hipMalloc");
+  if ((e = hipMalloc(reinterpret_cast<void **>(&b->d_flags), sizeof(int) * kMaxGroup)) != hipSuccess)
+    return fail(e, "hipMalloc");
+  if ((e = hipHostMalloc(reinterpret_cast<void **>(&b->h_flags), 2 * sizeof(int) * kMaxGroup,
+                         hipHostMallocDefault)) != hipSuccess)
+    return fail(e, "hipHostMalloc");
+  for (auto &v : b->ev)
+    if ((e = hipEventCreateWithFlags(&v, hipEventDisableTiming)) != hipSuccess)
+      return fail(e, "hipEventCreate");
+  for (auto &v : b->tev)
+    if ((e = hipEventCreate(&v)) != hipSuccess) return fail(e, "hipEventCreate");
+  if (cfdl::mgcg_group_create(&b->mg) != CFDL_OK) {
+    batch_free(b);
+    return CFDL_EHIP;
+  }
+  *out = b;
+  return CFDL_OK;
+}
+
+int cfdl_batch_attach(cfdl_batch *b, cfdl_ctx *c) {
+  if (!b || !c) {
+    set_error("cfdl_batch_attach: null batch or context");
+    return CFDL_EINVAL;
+  }
+  if (c->batch) {
+    set_error("cfdl_batch_attach: the context is already in a batch");
+    return CFDL_ESTATE;
+  }
+  if (static_cast<int>(b->ctx.size()) >= kMaxGroup) {
+    set_error("cfdl_batch_attach: the batch is full (" + std::to_string(kMaxGroup) + " cases)");
+    return CFDL_ESTATE;
+  }
+  CALL(batch_check(b, c));
+  // the case's queued work completes on its own stream; from here on its
+  // work runs on the batch's
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->own_st = c->st;
+  c->st = b->st;
+  c->batch = b;
+  b->ctx.push_back(c);
+  b->tuned = 0;
+  return CFDL_OK;
+}
+
+int cfdl_batch_detach(cfdl_batch *b, cfdl_ctx *c) {
+  if (!b || !c || c->batch != b) {
+    set_error("cfdl_batch_detach: the context is not in this batch");
+    return CFDL_EINVAL;
+  }
+  HIPCHK(hipStreamSynchronize(b->st));
+  c->st = c->own_st;
+  c->own_st = nullptr;
+  c->batch = nullptr;
+  b->ctx.erase(std::find(b->ctx.begin(), b->ctx.end(), c));
+  b->tuned = 0;
+  return CFDL_OK;
+}
+
+int cfdl_batch_size(cfdl_batch *b, int *n) {
+  if (!b || !n) {
+    set_error("cfdl_batch_size: null argument");
+    return CFDL_EINVAL;
+  }
+  *n = static_cast<int>(b->ctx.size());
+  return CFDL_OK;
+}
+
+int cfdl_batch_set_block(cfdl_batch *b, int block) {
+  if (!b || (block != 0 && (block < 64 || block > 1024 || block % 64 != 0))) {
+    set_error("cfdl_batch_set_block: 0 (tuned) or a multiple of 64 in 64 .. 1024");
+    return CFDL_EINVAL;
+  }
+  b->block = block;
+  return CFDL_OK;
+}
+
+int cfdl_batch_block(cfdl_batch *b, int *block) {
+  if (!b || !block) {
+    set_error("cfdl_batch_block: null argument");
+    return CFDL_EINVAL;
+  }
+  *block = b->block > 0 ? b->block : b->tuned > 0 ? b->tuned : kBlock;
+  return CFDL_OK;
+}
+
+int cfdl_batch_run(cfdl_batch *b, int ntstep, int ncoef) {
+  if (!b || ntstep < 0 || ncoef < 0) {
+    set_error("cfdl_batch_run: null batch or negative counts");
+    return CFDL_EINVAL;
+  }
+  if (b->ctx.empty()) return CFDL_OK;
+  CALL(batch_sync_slots(b));
+  for (int t = 0; t < ntstep; t++) {
+    for (int k = 0; k < ncoef; k++) CALL(batch_iteration(b));
+    CALL(batch_update_time(b));
+  }
+  CALL(batch_tune_read(b));
+  b->runs++;
+  return CFDL_OK;
+}
+
+int cfdl_batch_synchronize(cfdl_batch *b) {
+  if (!b) {
+    set_error("cfdl_batch_synchronize: null batch");
+    return CFDL_EINVAL;
+  }
+  HIPCHK(hipStreamSynchronize(b->st));
+  return CFDL_OK;
+}
+
+void cfdl_batch_destroy(cfdl_batch *b) {
+  if (!b) return;
+  (void)hipStreamSynchronize(b->st);
+  while (!b->ctx.empty()) (void)cfdl_batch_detach(b, b->ctx.back());
+  batch_free(b);
 }
 
 }  // extern "C"

@@ -85,4 +85,25 @@ void mgcg_destroy(MgCgSolver *s);
 int mgcg_solve(MgCgSolver *s, const MgCgSystem &sys, void *stream);
 // iterations of the last solve and launches it made (diagnostics)
 void mgcg_stats(const MgCgSolver *s, int *launches, int *levels);
+
+// Grouped launches over independent systems (cfdl_group.h): slot ids and
+// the first block of each in the launch.
+constexpr int kMaxGroup = 32;
+struct GroupMap {
+  int n;
+  int slot[kMaxGroup];
+  int b0[kMaxGroup + 1];
+};
+// The same solve for n <= kMaxGroup systems at once, each with its own
+// solver (all FP64 or all FP32 preconditioners), in grouped launches: every
+// system's arithmetic and stop test are those of mgcg_solve alone.  The
+// group object holds the systems' device descriptors (uploaded again when
+// the composition changes) and the flag buffers; block is the lane count of
+// the element-wise kernels (a multiple of 64 in 64 .. 1024; the results do
+// not depend on it).
+struct MgGroup;
+int mgcg_group_create(MgGroup **out);
+void mgcg_group_destroy(MgGroup *g);
+int mgcg_solve_group(MgGroup *g, MgCgSolver *const *s, const MgCgSystem *sys, int n, void *stream,
+                     int block);
 }  // namespace cfdl

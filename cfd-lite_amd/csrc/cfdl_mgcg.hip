@@ -28,6 +28,11 @@
 //  * All kernels read the solve's active flag and return at once when it is
 //    clear, so the host can queue an iteration before knowing whether the
 //    previous one stopped the solve.
+//  * Grouped solves (mgcg_solve_group) run the same steps for up to
+//    kMaxGroup independent systems, one launch per step for all of them:
+//    the launch's blocks are split into per-system ranges, each system keeps
+//    its own vectors, reductions and stop test, so its results are those of
+//    a solve alone, bit for bit.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -35,8 +40,10 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
+#include "cfdl_group.h"
 #include "cfdl_internal.h"
 
 namespace cfdl {
@@ -131,16 +138,23 @@ __device__ inline double final_max(const double *part, int nblk, int stride, dou
   return tree_max(acc, s);
 }
 
+// ---------------------------------------------------------------------------
+// CG kernels.  Each step is a device body run by block bx of the system's own
+// grid (bx = blockIdx.x for a solve alone; the block's index inside the
+// system's range in a grouped launch) and two thin kernels around it.
+// ---------------------------------------------------------------------------
+
 // r = b + sum anb phi_nb - ap phi; z = 0 (and r in FP32 for the mixed
 // preconditioner); res_i, target and the active flag from rms(r)
 template <typename T>
-__global__ void __launch_bounds__(kRed)
-k_cg_init(Red R, const int *__restrict__ row, const int *__restrict__ nbr,
-          const double *__restrict__ ap, const double *__restrict__ anb,
-          const double *__restrict__ b, const double *__restrict__ phi, double *__restrict__ r,
-          T *__restrict__ rt, T *__restrict__ z, CgCtl *c, double *part, int nit) {
+__device__ inline void d_cg_init(int bx, Red R, const int *__restrict__ row,
+                                 const int *__restrict__ nbr, const double *__restrict__ ap,
+                                 const double *__restrict__ anb, const double *__restrict__ b,
+                                 const double *__restrict__ phi, double *__restrict__ r,
+                                 T *__restrict__ rt, T *__restrict__ z, CgCtl *c, double *part,
+                                 int nit) {
   __shared__ double s[kRed];
-  const int lo = blockIdx.x * R.chunk, hi = min(lo + R.chunk, R.n);
+  const int lo = bx * R.chunk, hi = min(lo + R.chunk, R.n);
   double acc = 0.0;
   for (int i = lo + threadIdx.x; i < hi; i += kRed) {
     double v = b[i];
@@ -152,7 +166,7 @@ k_cg_init(Red R, const int *__restrict__ row, const int *__restrict__ nbr,
     acc = acc + v * v;
   }
   const double bs = tree_sum(acc, s);
-  if (threadIdx.x == 0) part[blockIdx.x] = bs;
+  if (threadIdx.x == 0) part[bx] = bs;
   if (!arrive_last(c, R.nblk)) return;
   const double rr = final_sum(part, R.nblk, 1, s);
   if (threadIdx.x == 0) {
@@ -171,15 +185,15 @@ k_cg_init(Red R, const int *__restrict__ row, const int *__restrict__ nbr,
 
 // rho = r.z, beta = rho / rho_old (0 on the first iteration)
 template <typename T>
-__global__ void __launch_bounds__(kRed)
-k_cg_rz(Red R, const double *__restrict__ r, const T *__restrict__ z, CgCtl *c, double *part) {
+__device__ inline void d_cg_rz(int bx, Red R, const double *__restrict__ r,
+                               const T *__restrict__ z, CgCtl *c, double *part) {
   __shared__ double s[kRed];
   if (!c->active) return;
-  const int lo = blockIdx.x * R.chunk, hi = min(lo + R.chunk, R.n);
+  const int lo = bx * R.chunk, hi = min(lo + R.chunk, R.n);
   double acc = 0.0;
   for (int i = lo + threadIdx.x; i < hi; i += kRed) acc = acc + r[i] * static_cast<double>(z[i]);
   const double bs = tree_sum(acc, s);
-  if (threadIdx.x == 0) part[blockIdx.x] = bs;
+  if (threadIdx.x == 0) part[bx] = bs;
   if (!arrive_last(c, R.nblk)) return;
   const double rho = final_sum(part, R.nblk, 1, s);
   if (threadIdx.x == 0) {
@@ -189,10 +203,10 @@ k_cg_rz(Red R, const double *__restrict__ r, const T *__restrict__ z, CgCtl *c, 
   }
 }
 
-// p = z, or z + beta p
+// p = z, or z + beta p (element i)
 template <typename T>
-__global__ void k_cg_p(int n, const T *__restrict__ z, double *__restrict__ p, const CgCtl *c) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void d_cg_p(int i, int n, const T *__restrict__ z, double *__restrict__ p,
+                              const CgCtl *c) {
   if (i >= n || !c->active) return;
   if (c->it == 0)
     p[i] = static_cast<double>(z[i]);
@@ -202,13 +216,13 @@ __global__ void k_cg_p(int n, const T *__restrict__ z, double *__restrict__ p, c
 
 // q = ap p - sum anb p_nb, sigma = p.q, alpha = rho / sigma (a breakdown,
 // sigma or rho not positive, ends the solve)
-__global__ void __launch_bounds__(kRed)
-k_cg_spmv(Red R, const int *__restrict__ row, const int *__restrict__ nbr,
-          const double *__restrict__ ap, const double *__restrict__ anb,
-          const double *__restrict__ p, double *__restrict__ q, CgCtl *c, double *part) {
+__device__ inline void d_cg_spmv(int bx, Red R, const int *__restrict__ row,
+                                 const int *__restrict__ nbr, const double *__restrict__ ap,
+                                 const double *__restrict__ anb, const double *__restrict__ p,
+                                 double *__restrict__ q, CgCtl *c, double *part) {
   __shared__ double s[kRed];
   if (!c->active) return;
-  const int lo = blockIdx.x * R.chunk, hi = min(lo + R.chunk, R.n);
+  const int lo = bx * R.chunk, hi = min(lo + R.chunk, R.n);
   double acc = 0.0;
   for (int i = lo + threadIdx.x; i < hi; i += kRed) {
     const int r0 = row[i], r1 = row[i + 1];
@@ -233,7 +247,7 @@ k_cg_spmv(Red R, const int *__restrict__ row, const int *__restrict__ nbr,
     acc = acc + pi * qi;
   }
   const double bs = tree_sum(acc, s);
-  if (threadIdx.x == 0) part[blockIdx.x] = bs;
+  if (threadIdx.x == 0) part[bx] = bs;
   if (!arrive_last(c, R.nblk)) return;
   const double sigma = final_sum(part, R.nblk, 1, s);
   if (threadIdx.x == 0) {
@@ -248,14 +262,14 @@ k_cg_spmv(Red R, const int *__restrict__ row, const int *__restrict__ nbr,
 // phi += alpha p, r -= alpha q, z = 0 for the next V-cycle; rms and max |r|;
 // the iteration count and the stop test
 template <typename T>
-__global__ void __launch_bounds__(kRed)
-k_cg_update(Red R, double *__restrict__ phi, const double *__restrict__ p,
-            const double *__restrict__ q, double *__restrict__ r, T *__restrict__ rt,
-            T *__restrict__ z, CgCtl *c, double *part) {
+__device__ inline void d_cg_update(int bx, Red R, double *__restrict__ phi,
+                                   const double *__restrict__ p, const double *__restrict__ q,
+                                   double *__restrict__ r, T *__restrict__ rt, T *__restrict__ z,
+                                   CgCtl *c, double *part) {
   __shared__ double s[kRed];
   if (!c->active) return;
   const double alpha = c->alpha;
-  const int lo = blockIdx.x * R.chunk, hi = min(lo + R.chunk, R.n);
+  const int lo = bx * R.chunk, hi = min(lo + R.chunk, R.n);
   double acc = 0.0, mx = 0.0;
   for (int i = lo + threadIdx.x; i < hi; i += kRed) {
     phi[i] = phi[i] + alpha * p[i];
@@ -268,8 +282,8 @@ k_cg_update(Red R, double *__restrict__ phi, const double *__restrict__ p,
   }
   const double bs = tree_sum(acc, s), bm = tree_max(mx, s);
   if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = bs;
-    part[2 * blockIdx.x + 1] = bm;
+    part[2 * bx] = bs;
+    part[2 * bx + 1] = bm;
   }
   if (!arrive_last(c, R.nblk)) return;
   const double rr = final_sum(part, R.nblk, 2, s);
@@ -284,8 +298,7 @@ k_cg_update(Red R, double *__restrict__ phi, const double *__restrict__ p,
   }
 }
 
-__global__ void k_cg_log(const CgCtl *c, LogLine *log, int *nlog, LogLine tmpl) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__device__ inline void d_cg_log(const CgCtl *c, LogLine *log, int *nlog, const LogLine &tmpl) {
   const int k = *nlog;
   if (k >= kLogCap) return;
   LogLine l = tmpl;
@@ -297,6 +310,40 @@ __global__ void k_cg_log(const CgCtl *c, LogLine *log, int *nlog, LogLine tmpl) 
   *nlog = k + 1;
 }
 
+template <typename T>
+__global__ void __launch_bounds__(kRed)
+k_cg_init(Red R, const int *__restrict__ row, const int *__restrict__ nbr,
+          const double *__restrict__ ap, const double *__restrict__ anb,
+          const double *__restrict__ b, const double *__restrict__ phi, double *__restrict__ r,
+          T *__restrict__ rt, T *__restrict__ z, CgCtl *c, double *part, int nit) {
+  d_cg_init<T>(blockIdx.x, R, row, nbr, ap, anb, b, phi, r, rt, z, c, part, nit);
+}
+template <typename T>
+__global__ void __launch_bounds__(kRed)
+k_cg_rz(Red R, const double *__restrict__ r, const T *__restrict__ z, CgCtl *c, double *part) {
+  d_cg_rz<T>(blockIdx.x, R, r, z, c, part);
+}
+template <typename T>
+__global__ void k_cg_p(int n, const T *__restrict__ z, double *__restrict__ p, const CgCtl *c) {
+  d_cg_p<T>(blockIdx.x * blockDim.x + threadIdx.x, n, z, p, c);
+}
+__global__ void __launch_bounds__(kRed)
+k_cg_spmv(Red R, const int *__restrict__ row, const int *__restrict__ nbr,
+          const double *__restrict__ ap, const double *__restrict__ anb,
+          const double *__restrict__ p, double *__restrict__ q, CgCtl *c, double *part) {
+  d_cg_spmv(blockIdx.x, R, row, nbr, ap, anb, p, q, c, part);
+}
+template <typename T>
+__global__ void __launch_bounds__(kRed)
+k_cg_update(Red R, double *__restrict__ phi, const double *__restrict__ p,
+            const double *__restrict__ q, double *__restrict__ r, T *__restrict__ rt,
+            T *__restrict__ z, CgCtl *c, double *part) {
+  d_cg_update<T>(blockIdx.x, R, phi, p, q, r, rt, z, c, part);
+}
+__global__ void k_cg_log(const CgCtl *c, LogLine *log, int *nlog, LogLine tmpl) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) d_cg_log(c, log, nlog, tmpl);
+}
+
 // ---------------------------------------------------------------------------
 // V-cycle kernels on one level (T = double or float)
 // ---------------------------------------------------------------------------
@@ -304,7 +351,7 @@ __global__ void k_cg_log(const CgCtl *c, LogLine *log, int *nlog, LogLine tmpl) 
 template <typename T>
 struct Lev {
   int n, nnz, ncol;
-  const int *row, *col, *cperm, *cptr;  // cptr on the device (coarse kernel)
+  const int *row, *col, *cperm, *cptr;  // cptr on the device (coarse and grouped kernels)
   const int *agg, *mptr, *mem, *iptr, *ilist, *gptr, *glist;
   const T *ap, *anb;
   T *b, *x;
@@ -317,23 +364,19 @@ __device__ inline T row_sum(const Lev<T> &L, int i, T acc) {
   return acc;
 }
 
-// FP32 copies of the level-0 coefficients (mixed preconditioner)
-__global__ void k_mg_copy0(int ne, int L, const double *__restrict__ ap,
-                           const double *__restrict__ anb, float *__restrict__ ap32,
-                           float *__restrict__ anb32) {
-  const int stride = gridDim.x * blockDim.x;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < L || i < ne; i += stride) {
-    if (i < L) anb32[i] = static_cast<float>(anb[i]);
-    if (i < ne) ap32[i] = static_cast<float>(ap[i]);
-  }
+// FP32 copies of the level-0 coefficients (mixed preconditioner), element i
+__device__ inline void d_mg_copy0(int i, int ne, int L, const double *__restrict__ ap,
+                                  const double *__restrict__ anb, float *__restrict__ ap32,
+                                  float *__restrict__ anb32) {
+  if (i < L) anb32[i] = static_cast<float>(anb[i]);
+  if (i < ne) ap32[i] = static_cast<float>(ap[i]);
 }
 
 // Galerkin operator of the next level: thread g < nC sums aggregate g's
 // diagonal (members' ap in order, minus the inside entries in order), thread
 // nC + s sums coarse slot s (its list of fine entries in order)
 template <typename T>
-__global__ void k_mg_galerkin(Lev<T> F, Lev<T> C) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void d_mg_galerkin(int g, const Lev<T> &F, const Lev<T> &C) {
   if (g < C.n) {
     T acc = static_cast<T>(0);
     for (int q = F.mptr[g]; q < F.mptr[g + 1]; q++) acc = acc + F.ap[F.mem[q]];
@@ -349,8 +392,7 @@ __global__ void k_mg_galerkin(Lev<T> F, Lev<T> C) {
 
 // one colour of a Gauss-Seidel sweep: x_i = (b_i + sum anb x_nb) / ap_i
 template <typename T>
-__global__ void k_mg_sweep(Lev<T> L, int c0, int cnt, const CgCtl *ctl) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void d_mg_sweep(int t, const Lev<T> &L, int c0, int cnt, const CgCtl *ctl) {
   if (t >= cnt || !ctl->active) return;
   const int i = L.cperm[c0 + t];
   L.x[i] = row_sum(L, i, L.b[i]) / L.ap[i];
@@ -359,8 +401,7 @@ __global__ void k_mg_sweep(Lev<T> L, int c0, int cnt, const CgCtl *ctl) {
 // restriction of the residual: b_C(I) = sum over members of
 // (b_i + sum anb x_nb) - ap_i x_i, from zero; x_C(I) = 0
 template <typename T>
-__global__ void k_mg_restrict(Lev<T> F, Lev<T> C, const CgCtl *ctl) {
-  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void d_mg_restrict(int I, const Lev<T> &F, const Lev<T> &C, const CgCtl *ctl) {
   if (I >= C.n || !ctl->active) return;
   T acc = static_cast<T>(0);
   for (int q = F.mptr[I]; q < F.mptr[I + 1]; q++) {
@@ -374,8 +415,8 @@ __global__ void k_mg_restrict(Lev<T> F, Lev<T> C, const CgCtl *ctl) {
 
 // prolongation: x_i += x_C(agg_i)
 template <typename T>
-__global__ void k_mg_prolong(Lev<T> F, const T *__restrict__ xc, const CgCtl *ctl) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void d_mg_prolong(int i, const Lev<T> &F, const T *__restrict__ xc,
+                                    const CgCtl *ctl) {
   if (i < F.n && ctl->active) F.x[i] = F.x[i] + xc[F.agg[i]];
 }
 
@@ -395,8 +436,8 @@ __device__ void wg_sweep(const Lev<T> &L, bool backward) {
   }
 }
 template <typename T>
-__global__ void __launch_bounds__(kCoarseThreads)
-k_mg_coarse(const Lev<T> *__restrict__ lv, int lc, int nlev, const CgCtl *ctl) {
+__device__ inline void d_mg_coarse(const Lev<T> *__restrict__ lv, int lc, int nlev,
+                                   const CgCtl *ctl) {
   if (!ctl->active) return;
   for (int l = lc; l < nlev - 1; l++) {
     const Lev<T> &F = lv[l], &C = lv[l + 1];
@@ -426,10 +467,148 @@ k_mg_coarse(const Lev<T> *__restrict__ lv, int lc, int nlev, const CgCtl *ctl) {
   }
 }
 
+__global__ void k_mg_copy0(int ne, int L, const double *__restrict__ ap,
+                           const double *__restrict__ anb, float *__restrict__ ap32,
+                           float *__restrict__ anb32) {
+  const int stride = gridDim.x * blockDim.x;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < L || i < ne; i += stride)
+    d_mg_copy0(i, ne, L, ap, anb, ap32, anb32);
+}
+template <typename T>
+__global__ void k_mg_galerkin(Lev<T> F, Lev<T> C) {
+  d_mg_galerkin<T>(blockIdx.x * blockDim.x + threadIdx.x, F, C);
+}
+template <typename T>
+__global__ void k_mg_sweep(Lev<T> L, int c0, int cnt, const CgCtl *ctl) {
+  d_mg_sweep<T>(blockIdx.x * blockDim.x + threadIdx.x, L, c0, cnt, ctl);
+}
+template <typename T>
+__global__ void k_mg_restrict(Lev<T> F, Lev<T> C, const CgCtl *ctl) {
+  d_mg_restrict<T>(blockIdx.x * blockDim.x + threadIdx.x, F, C, ctl);
+}
+template <typename T>
+__global__ void k_mg_prolong(Lev<T> F, const T *__restrict__ xc, const CgCtl *ctl) {
+  d_mg_prolong<T>(blockIdx.x * blockDim.x + threadIdx.x, F, xc, ctl);
+}
+template <typename T>
+__global__ void __launch_bounds__(kCoarseThreads)
+k_mg_coarse(const Lev<T> *__restrict__ lv, int lc, int nlev, const CgCtl *ctl) {
+  d_mg_coarse<T>(lv, lc, nlev, ctl);
+}
+
+// ---------------------------------------------------------------------------
+// Grouped kernels: one launch runs a step for every system of a group.  Each
+// system's descriptor (device memory) holds its vectors, its control block
+// and its levels with level 0 bound to the system.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct MgSlot {
+  Red R;
+  int ne, nit, nlev, lc;
+  const int *row, *nbr;
+  const double *ap, *anb, *b;
+  double *phi, *r, *p, *q, *part;
+  T *rt, *z;              // FP32 residual copy (nullptr for FP64), preconditioned residual
+  float *ap32, *anb32;    // FP32 level-0 coefficients (mixed)
+  CgCtl *ctl;
+  const Lev<T> *lv;       // device levels
+  LogLine *log;
+  int *nlog;
+  LogLine tmpl;
+};
+
+template <typename T>
+__global__ void g_mg_copy0(GroupMap g, const MgSlot<T> *__restrict__ S) {
+  int bx;
+  const MgSlot<T> &x = S[group_slot(g, bx)];
+  d_mg_copy0(bx * blockDim.x + threadIdx.x, x.ne, x.lv[0].nnz, x.ap, x.anb, x.ap32, x.anb32);
+}
+template <typename T>
+__global__ void g_mg_galerkin(GroupMap g, const MgSlot<T> *__restrict__ S, int l) {
+  int bx;
+  const MgSlot<T> &x = S[group_slot(g, bx)];
+  const Lev<T> F = x.lv[l], C = x.lv[l + 1];
+  d_mg_galerkin<T>(bx * blockDim.x + threadIdx.x, F, C);
+}
+template <typename T>
+__global__ void __launch_bounds__(kRed) g_cg_init(GroupMap g, const MgSlot<T> *__restrict__ S) {
+  int bx;
+  const MgSlot<T> &x = S[group_slot(g, bx)];
+  d_cg_init<T>(bx, x.R, x.row, x.nbr, x.ap, x.anb, x.b, x.phi, x.r, x.rt, x.z, x.ctl, x.part,
+               x.nit);
+}
+// colour cc (counted from the last colour when backward) of level l, or of
+// the system's coarsest level for l < 0
+template <typename T>
+__global__ void g_mg_sweep(GroupMap g, const MgSlot<T> *__restrict__ S, int l, int cc,
+                           int backward) {
+  int bx;
+  const MgSlot<T> &x = S[group_slot(g, bx)];
+  const Lev<T> L = x.lv[l < 0 ? x.nlev - 1 : l];
+  const int c = backward ? L.ncol - 1 - cc : cc;
+  const int c0 = L.cptr[c];
+  d_mg_sweep<T>(bx * blockDim.x + threadIdx.x, L, c0, L.cptr[c + 1] - c0, x.ctl);
+}
+template <typename T>
+__global__ void g_mg_restrict(GroupMap g, const MgSlot<T> *__restrict__ S, int l) {
+  int bx;
+  const MgSlot<T> &x = S[group_slot(g, bx)];
+  const Lev<T> F = x.lv[l], C = x.lv[l + 1];
+  d_mg_restrict<T>(bx * blockDim.x + threadIdx.x, F, C, x.ctl);
+}
+template <typename T>
+__global__ void g_mg_prolong(GroupMap g, const MgSlot<T> *__restrict__ S, int l) {
+  int bx;
+  const MgSlot<T> &x = S[group_slot(g, bx)];
+  const Lev<T> F = x.lv[l];
+  d_mg_prolong<T>(bx * blockDim.x + threadIdx.x, F, x.lv[l + 1].x, x.ctl);
+}
+template <typename T>
+__global__ void __launch_bounds__(kCoarseThreads)
+g_mg_coarse(GroupMap g, const MgSlot<T> *__restrict__ S) {
+  int bx;
+  const MgSlot<T> &x = S[group_slot(g, bx)];
+  d_mg_coarse<T>(x.lv, x.lc, x.nlev, x.ctl);
+}
+template <typename T>
+__global__ void __launch_bounds__(kRed) g_cg_rz(GroupMap g, const MgSlot<T> *__restrict__ S) {
+  int bx;
+  const MgSlot<T> &x = S[group_slot(g, bx)];
+  d_cg_rz<T>(bx, x.R, x.r, x.z, x.ctl, x.part);
+}
+template <typename T>
+__global__ void g_cg_p(GroupMap g, const MgSlot<T> *__restrict__ S) {
+  int bx;
+  const MgSlot<T> &x = S[group_slot(g, bx)];
+  d_cg_p<T>(bx * blockDim.x + threadIdx.x, x.ne, x.z, x.p, x.ctl);
+}
+template <typename T>
+__global__ void __launch_bounds__(kRed) g_cg_spmv(GroupMap g, const MgSlot<T> *__restrict__ S) {
+  int bx;
+  const MgSlot<T> &x = S[group_slot(g, bx)];
+  d_cg_spmv(bx, x.R, x.row, x.nbr, x.ap, x.anb, x.p, x.q, x.ctl, x.part);
+}
+template <typename T>
+__global__ void __launch_bounds__(kRed) g_cg_update(GroupMap g, const MgSlot<T> *__restrict__ S) {
+  int bx;
+  const MgSlot<T> &x = S[group_slot(g, bx)];
+  d_cg_update<T>(bx, x.R, x.phi, x.p, x.q, x.r, x.rt, x.z, x.ctl, x.part);
+}
+// the active flags of systems 0 .. n-1, and their log lines at the end
+template <typename T>
+__global__ void g_cg_flags(int n, const MgSlot<T> *__restrict__ S, int *flags) {
+  const int k = threadIdx.x;
+  if (k < n) flags[k] = S[k].ctl->active;
+}
+template <typename T>
+__global__ void g_cg_log(int n, const MgSlot<T> *__restrict__ S) {
+  const int k = threadIdx.x;
+  if (k < n) d_cg_log(S[k].ctl, S[k].log, S[k].nlog, S[k].tmpl);
+}
+
 inline int nb(int n, int b) { return (n + b - 1) / b; }
 
 }  // namespace
-
 // device copy of the hierarchy and the solve's work vectors
 struct MgCgSolver {
   int fp32 = 0, ne = 0, nbf = 0, nlev = 0, lc = 0;
@@ -446,6 +625,10 @@ struct MgCgSolver {
   int *hflag = nullptr;  // pinned
   hipEvent_t ev[2] = {nullptr, nullptr};
   int launches = 0;
+  // grouped solves: the arrays level 0 was last bound to on the device
+  // (d_lvd[0] / d_lvf[0]); nullptr until the first grouped solve
+  const int *bound_row = nullptr;
+  const double *bound_ap = nullptr;
 
   ~MgCgSolver() {
     for (void *m : mem) (void)hipFree(m);
@@ -707,6 +890,294 @@ int mgcg_solve(MgCgSolver *s, const MgCgSystem &sys, void *stream) {
                      sys.log, sys.nlog, tmpl);
   MGCHK(hipGetLastError());
   return CFDL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Grouped solves
+// ---------------------------------------------------------------------------
+struct MgGroup {
+  void *slots = nullptr;  // device MgSlot<T>[kMaxGroup]
+  int *flags = nullptr;   // device active flags
+  int *hflags = nullptr;  // pinned, two sets of kMaxGroup
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  std::vector<char> key;  // the composition whose descriptors are on the device
+  int launches = 0;
+  ~MgGroup() {
+    if (slots) (void)hipFree(slots);
+    if (flags) (void)hipFree(flags);
+    if (hflags) (void)hipHostFree(hflags);
+    for (auto &e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+};
+
+int mgcg_group_create(MgGroup **out) {
+  std::unique_ptr<MgGroup> g(new MgGroup());
+  const size_t bytes = sizeof(MgSlot<double>) > sizeof(MgSlot<float>) ? sizeof(MgSlot<double>)
+                                                                       : sizeof(MgSlot<float>);
+  MGCHK(hipMalloc(&g->slots, bytes * kMaxGroup));
+  MGCHK(hipMalloc(reinterpret_cast<void **>(&g->flags), sizeof(int) * kMaxGroup));
+  MGCHK(hipHostMalloc(reinterpret_cast<void **>(&g->hflags), 2 * sizeof(int) * kMaxGroup,
+                      hipHostMallocDefault));
+  MGCHK(hipEventCreateWithFlags(&g->ev[0], hipEventDisableTiming));
+  MGCHK(hipEventCreateWithFlags(&g->ev[1], hipEventDisableTiming));
+  *out = g.release();
+  return CFDL_OK;
+}
+
+void mgcg_group_destroy(MgGroup *g) { delete g; }
+
+namespace {
+// Level 0 of a grouped solve lives on the device (the grouped kernels read
+// every level from there); it is bound to the system's arrays once, with the
+// stream drained, and again only if they change.
+template <typename T>
+int bind_level0(MgCgSolver *s, const MgCgSystem &sys, std::vector<Lev<T>> &lv, Lev<T> *d_lv,
+                hipStream_t st, Lev<T> *out) {
+  Lev<T> z = lv[0];
+  z.row = sys.row;
+  z.col = sys.nbr;
+  if (s->fp32) {
+    z.ap = z.apw = reinterpret_cast<T *>(s->ap32);
+    z.anb = z.anbw = reinterpret_cast<T *>(s->anb32);
+    z.b = reinterpret_cast<T *>(s->r32);
+    z.x = reinterpret_cast<T *>(s->z32);
+  } else {
+    z.ap = reinterpret_cast<const T *>(sys.ap);
+    z.anb = reinterpret_cast<const T *>(sys.anb);
+    z.apw = z.anbw = nullptr;
+    z.b = reinterpret_cast<T *>(s->r);
+    z.x = reinterpret_cast<T *>(s->zd);
+  }
+  if (s->bound_row != sys.row || s->bound_ap != sys.ap) {
+    MGCHK(hipStreamSynchronize(st));
+    MGCHK(hipMemcpy(d_lv, &z, sizeof(Lev<T>), hipMemcpyHostToDevice));
+    s->bound_row = sys.row;
+    s->bound_ap = sys.ap;
+  }
+  *out = z;
+  return CFDL_OK;
+}
+
+template <typename T>
+Lev<T> *dev_levels(MgCgSolver *s);
+template <>
+Lev<double> *dev_levels<double>(MgCgSolver *s) { return s->d_lvd; }
+template <>
+Lev<float> *dev_levels<float>(MgCgSolver *s) { return s->d_lvf; }
+template <typename T>
+std::vector<Lev<T>> &host_levels(MgCgSolver *s);
+template <>
+std::vector<Lev<double>> &host_levels<double>(MgCgSolver *s) { return s->lvd; }
+template <>
+std::vector<Lev<float>> &host_levels<float>(MgCgSolver *s) { return s->lvf; }
+
+// The solve of mgcg_solve for n systems at once.  Every launch covers the
+// systems that take part in its step; each system runs exactly its own
+// sequence of steps (a system with fewer levels or colours sits out the
+// launches it has no part in), so its arithmetic is that of a solve alone.
+template <typename T>
+int solve_group(MgGroup *G, MgCgSolver *const *ss, const MgCgSystem *sys, int n, hipStream_t st,
+                int blk) {
+  std::vector<MgSlot<T>> h(n);
+  std::memset(static_cast<void *>(h.data()), 0, sizeof(MgSlot<T>) * n);  // the key compares bytes
+  std::vector<Lev<T>> l0(n);
+  int maxtop = 0, maxlev = 0, nitmax = 0;
+  for (int k = 0; k < n; k++) {
+    MgCgSolver *s = ss[k];
+    MGCALL(bind_level0<T>(s, sys[k], host_levels<T>(s), dev_levels<T>(s), st, &l0[k]));
+    MgSlot<T> &x = h[k];
+    x.R = red_of(s->ne);
+    x.ne = s->ne;
+    x.nit = sys[k].nit;
+    x.nlev = s->nlev;
+    x.lc = s->lc;
+    x.row = sys[k].row;
+    x.nbr = sys[k].nbr;
+    x.ap = sys[k].ap;
+    x.anb = sys[k].anb;
+    x.b = sys[k].b;
+    x.phi = sys[k].phi;
+    x.r = s->r;
+    x.p = s->p;
+    x.q = s->q;
+    x.part = s->part;
+    x.rt = s->fp32 ? reinterpret_cast<T *>(s->r32) : nullptr;
+    x.z = s->fp32 ? reinterpret_cast<T *>(s->z32) : reinterpret_cast<T *>(s->zd);
+    x.ap32 = s->ap32;
+    x.anb32 = s->anb32;
+    x.ctl = s->ctl;
+    x.lv = dev_levels<T>(s);
+    x.log = sys[k].log;
+    x.nlog = sys[k].nlog;
+    std::memcpy(x.tmpl.name, sys[k].name, 8);
+    maxtop = std::max(maxtop, s->lc < s->nlev ? s->lc : s->nlev - 1);
+    maxlev = std::max(maxlev, s->nlev);
+    nitmax = std::max(nitmax, sys[k].nit);
+  }
+  // the descriptors go to the device when the composition changes
+  const char *kb = reinterpret_cast<const char *>(h.data());
+  std::vector<char> key(kb, kb + sizeof(MgSlot<T>) * n);
+  key.push_back(static_cast<char>(sizeof(T)));
+  if (key != G->key) {
+    MGCHK(hipStreamSynchronize(st));
+    MGCHK(hipMemcpy(G->slots, h.data(), sizeof(MgSlot<T>) * n, hipMemcpyHostToDevice));
+    G->key.swap(key);
+  }
+  const MgSlot<T> *S = static_cast<const MgSlot<T> *>(G->slots);
+
+  GroupMap g{};
+  std::vector<int> ids(n), cnt(n);
+  for (int k = 0; k < n; k++) ids[k] = k;
+  int launches = 0;
+  // one grouped launch: blocks_of(k) blocks for system k (0 = sits out)
+  auto map = [&](auto blocks_of) {
+    for (int k = 0; k < n; k++) cnt[k] = blocks_of(k);
+    return group_map(g, ids.data(), cnt.data(), n);
+  };
+  auto top_of = [&](int k) { return ss[k]->lc < ss[k]->nlev ? ss[k]->lc : ss[k]->nlev - 1; };
+  auto colour_count = [&](int k, int l, int c) {
+    return ss[k]->cptr[l][c + 1] - ss[k]->cptr[l][c];
+  };
+  // colour cc of level l (l < 0: the coarsest level of systems without small
+  // levels), forward or backward
+  auto sweep = [&](int l, bool backward) {
+    int ncmax = 0;
+    for (int k = 0; k < n; k++) {
+      const int lk = l < 0 ? ss[k]->nlev - 1 : l;
+      const bool part = l < 0 ? ss[k]->lc == ss[k]->nlev : l < top_of(k);
+      if (part) ncmax = std::max(ncmax, host_levels<T>(ss[k])[lk].ncol);
+    }
+    for (int cc = 0; cc < ncmax; cc++) {
+      const int nbk = map([&](int k) {
+        const int lk = l < 0 ? ss[k]->nlev - 1 : l;
+        const bool part = l < 0 ? ss[k]->lc == ss[k]->nlev : l < top_of(k);
+        const int nc = host_levels<T>(ss[k])[lk].ncol;
+        if (!part || cc >= nc) return 0;
+        const int c = backward ? nc - 1 - cc : cc;
+        return nb(colour_count(k, lk, c), blk);
+      });
+      if (nbk > 0) {
+        hipLaunchKernelGGL(g_mg_sweep<T>, dim3(nbk), dim3(blk), 0, st, g, S, l, cc,
+                           backward ? 1 : 0);
+        launches++;
+      }
+    }
+  };
+
+  int nbk = 0;
+  if (std::is_same<T, float>::value) {
+    nbk = map([&](int k) { return nb(std::max(l0[k].nnz, ss[k]->ne), blk); });
+    if (nbk > 0) {
+      hipLaunchKernelGGL(g_mg_copy0<T>, dim3(nbk), dim3(blk), 0, st, g, S);
+      launches++;
+    }
+  }
+  for (int l = 0; l + 1 < maxlev; l++) {
+    nbk = map([&](int k) {
+      if (l + 1 >= ss[k]->nlev) return 0;
+      const Lev<T> &C = host_levels<T>(ss[k])[l + 1];
+      return nb(C.n + C.nnz, blk);
+    });
+    if (nbk > 0) {
+      hipLaunchKernelGGL(g_mg_galerkin<T>, dim3(nbk), dim3(blk), 0, st, g, S, l);
+      launches++;
+    }
+  }
+  nbk = map([&](int k) { return h[k].R.nblk; });
+  hipLaunchKernelGGL(g_cg_init<T>, dim3(nbk), dim3(kRed), 0, st, g, S);
+  launches++;
+  MGCHK(hipGetLastError());
+
+  for (int it = 0; it < nitmax; it++) {
+    // V-cycle
+    for (int l = 0; l < maxtop; l++) {
+      sweep(l, false);
+      nbk = map([&](int k) { return l < top_of(k) ? nb(host_levels<T>(ss[k])[l + 1].n, blk) : 0; });
+      if (nbk > 0) {
+        hipLaunchKernelGGL(g_mg_restrict<T>, dim3(nbk), dim3(blk), 0, st, g, S, l);
+        launches++;
+      }
+    }
+    nbk = map([&](int k) { return ss[k]->lc < ss[k]->nlev ? 1 : 0; });
+    if (nbk > 0) {
+      hipLaunchKernelGGL(g_mg_coarse<T>, dim3(nbk), dim3(kCoarseThreads), 0, st, g, S);
+      launches++;
+    }
+    bool any_flat = false;
+    for (int k = 0; k < n; k++) any_flat = any_flat || ss[k]->lc == ss[k]->nlev;
+    if (any_flat)
+      for (int r = 0; r < kCoarseSweeps; r++) {
+        sweep(-1, false);
+        sweep(-1, true);
+      }
+    for (int l = maxtop - 1; l >= 0; l--) {
+      nbk = map([&](int k) { return l < top_of(k) ? nb(host_levels<T>(ss[k])[l].n, blk) : 0; });
+      if (nbk > 0) {
+        hipLaunchKernelGGL(g_mg_prolong<T>, dim3(nbk), dim3(blk), 0, st, g, S, l);
+        launches++;
+      }
+      sweep(l, true);
+    }
+    // CG
+    nbk = map([&](int k) { return h[k].R.nblk; });
+    hipLaunchKernelGGL(g_cg_rz<T>, dim3(nbk), dim3(kRed), 0, st, g, S);
+    const GroupMap gR = g;
+    nbk = map([&](int k) { return nb(ss[k]->ne, blk); });
+    hipLaunchKernelGGL(g_cg_p<T>, dim3(nbk), dim3(blk), 0, st, g, S);
+    hipLaunchKernelGGL(g_cg_spmv<T>, dim3(gR.b0[gR.n]), dim3(kRed), 0, st, gR, S);
+    hipLaunchKernelGGL(g_cg_update<T>, dim3(gR.b0[gR.n]), dim3(kRed), 0, st, gR, S);
+    hipLaunchKernelGGL(g_cg_flags<T>, dim3(1), dim3(kMaxGroup), 0, st, n, S, G->flags);
+    launches += 5;
+    MGCHK(hipGetLastError());
+    int *hf = G->hflags + (it & 1) * kMaxGroup;
+    MGCHK(hipMemcpyAsync(hf, G->flags, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+    MGCHK(hipEventRecord(G->ev[it & 1], st));
+    if (it > 0) {
+      MGCHK(hipEventSynchronize(G->ev[(it - 1) & 1]));
+      const int *pf = G->hflags + ((it - 1) & 1) * kMaxGroup;
+      bool any = false;
+      for (int k = 0; k < n; k++) any = any || pf[k] != 0;
+      if (!any) break;
+    }
+  }
+  hipLaunchKernelGGL(g_cg_log<T>, dim3(1), dim3(kMaxGroup), 0, st, n, S);
+  launches++;
+  MGCHK(hipGetLastError());
+  G->launches = launches;
+  for (int k = 0; k < n; k++) ss[k]->launches = launches;
+  return CFDL_OK;
+}
+}  // namespace
+
+int mgcg_solve_group(MgGroup *G, MgCgSolver *const *s, const MgCgSystem *sys, int n,
+                     void *stream, int block) {
+  if (!G || !s || !sys || n < 1 || n > kMaxGroup) {
+    set_error("mgcg_solve_group: 1 .. " + std::to_string(kMaxGroup) + " systems");
+    return CFDL_EINVAL;
+  }
+  if (block < 64 || block > 1024 || block % 64 != 0) {
+    set_error("mgcg_solve_group: block must be a multiple of 64 in 64 .. 1024");
+    return CFDL_EINVAL;
+  }
+  for (int k = 0; k < n; k++) {
+    if (!s[k] || sys[k].ne != s[k]->ne || sys[k].nbf != s[k]->nbf) {
+      set_error("mgcg_solve_group: system " + std::to_string(k) + " does not match its hierarchy");
+      return CFDL_EINVAL;
+    }
+    if (s[k]->fp32 != s[0]->fp32) {
+      set_error("mgcg_solve_group: the systems of a group share one preconditioner precision");
+      return CFDL_EINVAL;
+    }
+    for (int j = 0; j < k; j++)
+      if (s[j] == s[k]) {
+        set_error("mgcg_solve_group: a solver appears twice in the group");
+        return CFDL_EINVAL;
+      }
+  }
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  return s[0]->fp32 ? solve_group<float>(G, s, sys, n, st, block)
+                    : solve_group<double>(G, s, sys, n, st, block);
 }
 
 }  // namespace cfdl

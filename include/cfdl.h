@@ -273,6 +273,11 @@ int cfdl_set_inlet_profile(cfdl_ctx *c, double umax, double y0, double y1, doubl
 int cfdl_set_field(cfdl_ctx *c, int field, const double *src, int64_t n);
 int cfdl_get_field(cfdl_ctx *c, int field, double *dst, int64_t n);
 int cfdl_synchronize(cfdl_ctx *c);
+/* The context back to its state at creation (every field cleared and the
+   energy fields initialised, the log and the outer-iteration count
+   cleared), keeping its mesh, properties, BCs and settings.  No counterpart
+   in the reference; serving reuses a context for the next case this way. */
+int cfdl_ctx_reset(cfdl_ctx *c);
 
 /* One entry point per reference routine, all on device-resident state.    */
 int cfdl_update_boundaries(cfdl_ctx *c);   /* mod_physics.f90:38-50 (uvwp, energy BCs) */
@@ -345,6 +350,36 @@ int cfdl_calc_residual(int ne, const int *row, const int *nbr, const double *ap,
 int cfdl_calc_residual_noalloc(int ne, const int *row, const int *nbr, const double *ap,
                                const double *anb, const double *b, const double *phi,
                                double *res, double *res_max, void *stream);
+
+/* Batches: continuous batching of independent cases (no counterpart in the
+   reference, whose program solves one case per run, src/main.f90:25-38).
+   cfdl_batch_run(b, ntstep, ncoef) does for every attached context what
+   cfdl_run(c, ntstep, ncoef) does for one (the loop of src/main.f90:50-63),
+   with one grouped launch per step for all cases: each case keeps its own
+   fields, solver state, stop tests and log, and its results equal those of
+   cfdl_run on the case alone, bit for bit.  Up to 32 cases per batch, all on
+   the device current at cfdl_batch_create.  A case needs six-entry
+   (hexahedral) rows, the colour order of solve_gs (cfdl_set_gs_order),
+   pc order mgcg or mgcg-mixed (cfdl_set_pc_order), and the assembly and pc
+   precision of the batch's first case.  While attached, a context's work
+   (including cfdl_get_field, cfdl_log_*, and single steps) runs on the
+   batch's stream, so cases attach and detach between cfdl_batch_run calls
+   while others are part-way through their solves; cfdl_ctx_destroy detaches
+   first.  cfdl_batch_set_block sets the lane count of the element-wise
+   launches (a multiple of 64 in 64 .. 1024; 0, the default, lets the batch
+   time the candidates at the first run after each change of composition and
+   keep the fastest); results do not depend on it.  Not thread-safe. */
+typedef struct cfdl_batch cfdl_batch;
+int cfdl_batch_create(cfdl_batch **out);
+int cfdl_batch_attach(cfdl_batch *b, cfdl_ctx *c);
+int cfdl_batch_detach(cfdl_batch *b, cfdl_ctx *c);
+int cfdl_batch_size(cfdl_batch *b, int *n);
+int cfdl_batch_set_block(cfdl_batch *b, int block);
+int cfdl_batch_block(cfdl_batch *b, int *block);
+int cfdl_batch_run(cfdl_batch *b, int ntstep, int ncoef);
+int cfdl_batch_synchronize(cfdl_batch *b);
+void cfdl_batch_destroy(cfdl_batch *b);
+
 /* Launch counters and sampled kernel timing (no counterpart in the
    reference, which has no timers).  cfdl_set_profile(c, k) brackets every
    k-th pressure-correction sweep (all of its launches, back to back) with

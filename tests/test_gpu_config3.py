@@ -7,11 +7,14 @@ iterations from rest:
   - "mixed" (FP32 colour sweeps under FP64 defect correction at every
     residual check) against the FP64 colour order: per field relative L2
     <= 1e-6 (measured about 1e-9);
-  - "mgcg-mixed" (FP32 V-cycle inside FP64 CG) against "mgcg": <= 1e-5
-    (measured 1.8e-6 in u, 5.7e-6 in p).  Each CG solve stops at a tenfold
-    residual reduction, so the two preconditioners leave different
-    unconverged remainders in pc, which the outer iterations carry; the
-    bound is set between that measurement and an order of magnitude above;
+  - "mgcg-mixed" (FP32 V-cycle inside FP64 CG) against "mgcg": <= 1e-5.
+    Each CG solve stops at a tenfold residual reduction, so the two
+    preconditioners leave different unconverged remainders in pc, which the
+    outer iterations carry (a difference of solver path, not of rounding:
+    at 128^3 Re=400 it measures 1.8e-6 in u and 5.7e-6 in p);
+velocity errors are relative to the norm of the whole velocity field (u, v,
+w together: v alone is small in this cavity, and its own norm would inflate
+its relative error), p to its own norm;
 the pc right-hand side sums to zero (a closed cavity: inflow = outflow); and a
 second run on a fresh context repeats the first bit for bit.
 """
@@ -47,8 +50,11 @@ def test_re1000_mixed_bitwise_against_the_oracle(n, jit, order, gs):
     assert [l[:2] for l in eq.log()] == [l[:2] for l in oc.log()]
 
 
-def _relerr(a, b):
-    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+def _errors(got, ref):
+    vel = np.linalg.norm(np.concatenate([ref[k] for k in ("u", "v", "w")]))
+    return {k: float(np.linalg.norm(got[k] - ref[k]) /
+                     max(vel if k != "p" else np.linalg.norm(ref[k]), 1e-300))
+            for k in ("u", "v", "w", "p")}
 
 
 @pytest.fixture(scope="module")
@@ -81,9 +87,8 @@ def test_256_mixed_precision_against_fp64(mesh256):
     mixed_mg = _run(mesh256, "mgcg-mixed", "colored", K)
     for ref, got, bound in ((fp64_col, mixed, 1e-6), (fp64_mg, mixed_mg, 1e-5)):
         assert [l[:2] for l in ref["log"]] == [l[:2] for l in got["log"]]
-        for k in ("u", "v", "w", "p"):
-            e = _relerr(got[k], ref[k])
-            assert e <= bound, (k, e)
+        err = _errors(got, ref)
+        assert max(err.values()) <= bound, err
     for r in (fp64_col, mixed, fp64_mg, mixed_mg):
         assert abs(r["bsum"][0]) <= 1e-12 * r["bsum"][1]
     again = _run(mesh256, "mgcg-mixed", "colored", K)

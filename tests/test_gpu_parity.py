@@ -142,7 +142,7 @@ def test_set_section_bc_rejects_bad_input():
     with pytest.raises(cfdl.CfdlError):
         eq.set_section_bc(mesh, "cells", "symmetry")
     with pytest.raises(cfdl.CfdlError):
-        eq.set_section_bc(mesh, "north", "inlet")
+        eq.set_section_bc(mesh, "north", "outflow")  # not a BC kind
     with pytest.raises(cfdl.CfdlError):
         eq.set_bc_kind(1, 5, "symmetry")  # cells, not halos
 
@@ -469,29 +469,34 @@ def test_colored_gs_order_bitwise(n, jit, nsubd, pc):
 
 
 def test_serving_alone_vs_mix():
-    """Requests served three at a time from reused contexts (every field
-    cleared, viscosity from Re) give the bits of the same requests solved
-    alone on fresh contexts, in the fastest mode (mixed pc solve, colour
-    order of solve_gs)."""
+    """Continuous batching: cavity and channel requests of different sizes
+    arrive while others run, join the batch, and leave when their scaled
+    residual reaches tol or after max_outer iterations.  Each result (and
+    its iteration count) equals the same request solved alone on a fresh
+    context in the same chunks, with room to spare and with a budget of two
+    contexts (admissions wait, idle contexts are closed)."""
     from cfdl import serving
-    reqs = [(0.0, 8, 150.0), (0.0, 10, 400.0), (0.0, 8, 900.0), (0.01, 12, 250.0),
-            (0.02, 10, 120.0), (0.03, 8, 600.0)]
-    out = serving.serve(reqs, slots=3, pc_order="mixed", gs_order="colored", ntstep=2, ncoef=3,
-                        keep_fields=True)
-    assert out["solves_per_s"] > 0 and out["p99_s"] >= out["p50_s"] > 0
-    # a budget of two contexts: admissions wait and idle contexts are evicted
-    tight = serving.footprint(12) + serving.footprint(10)
-    out2 = serving.serve(reqs, slots=3, hbm_budget=tight, pc_order="mixed", gs_order="colored",
-                         ntstep=2, ncoef=3, keep_fields=True)
-    for i, (_, n, re) in enumerate(reqs):
-        pool = serving.ContextPool("mixed", "colored")
+    R = serving.Request
+    # four slots for six requests: the last two join when others leave
+    reqs = [R(0, 0.0, "cavity", 16, 150.0), R(1, 0.0, "channel", 24, 400.0),
+            R(2, 0.0, "cavity", 16, 900.0), R(3, 0.01, "channel", 16, 250.0),
+            R(4, 0.02, "cavity", 24, 120.0), R(5, 0.03, "channel", 16, 600.0)]
+    kw = dict(tol=3e-2, max_outer=60, chunk=5, keep_fields=True)
+    out = serving.serve(reqs, max_cases=4, **kw)
+    assert out["solves_per_s"] > 0 and out["p99_s"] >= out["p50_s"] > 0 and not out["failed"]
+    tight = serving.footprint(24) + serving.footprint(16)
+    out2 = serving.serve(reqs, max_cases=4, hbm_budget=tight, **kw)
+    assert out2["contexts_created"] >= out["contexts_created"]
+    for r in reqs:
+        pool = serving.ContextPool()
         try:
-            alone = serving.solve_case(pool, n, re, 2, 3)
+            alone, outer, worst = serving.solve_alone(pool, r, tol=3e-2, max_outer=60, chunk=5)
         finally:
             pool.close()
-        for k in ("u", "v", "w", "p"):
-            assert np.array_equal(out["fields"][i][k], alone[k]), (i, k)
-            assert np.array_equal(out2["fields"][i][k], alone[k]), (i, k)
+        for o in (out, out2):
+            assert o["outer"][r.idx] == outer and o["scaled_residual"][r.idx] == worst, r.idx
+            for k in ("u", "v", "w", "p"):
+                assert np.array_equal(o["fields"][r.idx][k], alone[k]), (r.idx, k)
 
 
 def test_energy_equation_colored_gs_bitwise():

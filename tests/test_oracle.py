@@ -616,15 +616,18 @@ def test_gs_colour_order_is_a_proper_greedy_colouring():
 
 def test_serving_request_stream_is_seeded_and_bounded():
     """The serving benchmark's stream: Poisson arrivals (mean gap 1/rate),
-    edges from {16, 24, ..., 96}, Re log-uniform in [100, 1000], the same
-    stream for the same seed."""
+    edges from {16, 24, ..., 96}, Re log-uniform in [100, 1000], cavity and
+    channel a fair draw, the same stream for the same seed."""
     from cfdl import serving
     a = serving.make_requests(512, 20.0, 7)
     assert a == serving.make_requests(512, 20.0, 7)
-    t = np.array([r[0] for r in a])
+    assert [r.idx for r in a] == list(range(512))
+    t = np.array([r.arrival for r in a])
     assert np.all(np.diff(t) > 0) and abs(np.mean(np.diff(t)) - 0.05) < 0.01
-    assert {r[1] for r in a} <= set(range(16, 97, 8)) and len({r[1] for r in a}) == 11
-    re = np.array([r[2] for r in a])
+    assert {r.n for r in a} <= set(range(16, 97, 8)) and len({r.n for r in a}) == 11
+    kinds = [r.kind for r in a]
+    assert set(kinds) == {"cavity", "channel"} and abs(kinds.count("cavity") - 256) < 40
+    re = np.array([r.re for r in a])
     assert re.min() >= 100.0 and re.max() <= 1000.0
     assert abs(np.median(np.log(re)) - 0.5 * (np.log(100.0) + np.log(1000.0))) < 0.2
 

@@ -474,6 +474,8 @@ def main():
                     help="pc solver of --converge runs")
     ap.add_argument("--gs-order", choices=("reference", "colored"), default="reference",
                     help="order of solve_gs of --converge runs")
+    ap.add_argument("--assembly-bits", type=int, choices=(32, 64), default=64,
+                    help="coefficient assembly precision of --converge runs")
     ap.add_argument("--serving", action="store_true",
                     help="only run the serving stream and report solves/s with p50/p99 latency")
     ap.add_argument("--serving-requests", type=int, default=512)
@@ -513,9 +515,11 @@ def main():
             eq.set_dt(float("inf"))
         eq.set_pc_order(args.pc_order)
         eq.set_gs_order(args.gs_order)
+        eq.set_assembly_precision(args.assembly_bits)
         k, secs, worst = run_to_convergence(eq, ne, args.tstep_every, args.max_outer, args.tol)
         print(json.dumps({
             "metric": METRIC, "mode": "converge", "pc_order": args.pc_order,
+            "assembly_bits": args.assembly_bits,
             "gs_order": args.gs_order, "steady": bool(args.steady or steady),
             "value": round(ne * k / secs, 1),
             "unit": "cell-iterations/s", "outer_iterations": k, "seconds": round(secs, 3),

@@ -87,12 +87,42 @@ void bisect_node(Bisect &B, int *ids, int n, long parent) {
     }
 }
 
-// greedy colouring in index order; nodes >= nint are halos (level 0)
+// Colours of a level: two, by the parity of the breadth-first distance from
+// the least node of each connected part (neighbours taken in CSR order),
+// when that is a proper colouring -- the cell graph of a hex box and the
+// coarse graphs of its box-shaped aggregates are bipartite; otherwise the
+// greedy colouring in index order.  Nodes >= nint are halos (level 0).
 void colour(MgLevel &L, int nint) {
   const int n = L.n;
-  std::vector<int> col(n, 0);
+  std::vector<int> col(n, -1), queue;
+  queue.reserve(n);
+  bool two = true;
+  for (int s = 0; s < n && two; s++) {
+    if (col[s] >= 0) continue;
+    col[s] = 0;
+    queue.assign(1, s);
+    for (size_t h = 0; h < queue.size() && two; h++) {
+      const int e = queue[h];
+      for (int k = L.row[e]; k < L.row[e + 1]; k++) {
+        const int j = L.col[k];
+        if (j >= nint || j == e) continue;
+        if (col[j] < 0) {
+          col[j] = 1 - col[e];
+          queue.push_back(j);
+        } else if (col[j] == col[e]) {
+          two = false;
+          break;
+        }
+      }
+    }
+  }
   int ncol = 0;
-  for (int e = 0; e < n; e++) {
+  if (two) {
+    for (int e = 0; e < n; e++) ncol = std::max(ncol, col[e] + 1);
+  } else {
+    std::fill(col.begin(), col.end(), 0);
+  }
+  for (int e = 0; e < n && !two; e++) {
     unsigned long long used = 0;
     for (int k = L.row[e]; k < L.row[e + 1]; k++) {
       const int j = L.col[k];

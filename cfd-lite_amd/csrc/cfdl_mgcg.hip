@@ -65,6 +65,8 @@ constexpr int kBlk = 256;          // lanes per block of the grid-wide V-cycle k
 constexpr int kCoarseMax = 4096;   // levels at most this large run in one workgroup
 constexpr int kCoarseThreads = 1024;
 constexpr int kCoarseSweeps = 10;  // forward + backward sweep pairs on the coarsest level
+// a coarsest level at most this large is swept from LDS by one wavefront
+constexpr int kLdsNodes = 64, kLdsEntries = 4096;
 
 // control block of one solve
 struct CgCtl {
@@ -413,6 +415,38 @@ __device__ inline void d_mg_restrict(int I, const Lev<T> &F, const Lev<T> &C, co
   C.x[I] = static_cast<T>(0);
 }
 
+// The same restriction with NL lanes per aggregate (NL a power of two, at
+// least the aggregate's member count): lane j computes member j's term,
+// then every lane of the group adds the terms in member order from
+// shuffles, so the sum is the loop's above.  All lanes reach the shuffles
+// (the active flag is the same for the whole launch of a system).
+template <typename T, int NL>
+__device__ inline void d_mg_restrict_lanes(int gt, const Lev<T> &F, const Lev<T> &C,
+                                           const CgCtl *ctl) {
+  if (!ctl->active) return;
+  const int I = gt / NL, j = gt % NL;
+  T t = static_cast<T>(0);
+  int cnt = 0;
+  if (I < C.n) {
+    const int q0 = F.mptr[I];
+    cnt = F.mptr[I + 1] - q0;
+    if (j < cnt) {
+      const int i = F.mem[q0 + j];
+      t = row_sum(F, i, F.b[i]) - F.ap[i] * F.x[i];
+    }
+  }
+  T acc = static_cast<T>(0);
+#pragma unroll
+  for (int jj = 0; jj < NL; jj++) {
+    const T v = __shfl(t, jj, NL);
+    if (jj < cnt) acc = acc + v;
+  }
+  if (j == 0 && I < C.n) {
+    C.b[I] = acc;
+    C.x[I] = static_cast<T>(0);
+  }
+}
+
 // prolongation: x_i += x_C(agg_i)
 template <typename T>
 __device__ inline void d_mg_prolong(int i, const Lev<T> &F, const T *__restrict__ xc,
@@ -454,9 +488,53 @@ __device__ inline void d_mg_coarse(const Lev<T> *__restrict__ lv, int lc, int nl
     }
     __syncthreads();
   }
-  for (int k = 0; k < kCoarseSweeps; k++) {
-    wg_sweep(lv[nlev - 1], false);
-    wg_sweep(lv[nlev - 1], true);
+  const Lev<T> &Z = lv[nlev - 1];
+  if (Z.n <= kLdsNodes && Z.nnz <= kLdsEntries && Z.ncol <= kLdsNodes) {
+    // The coarsest level in LDS, swept by wavefront 0 alone: one wavefront
+    // holds every node of a colour, and its LDS accesses complete in program
+    // order, so no barrier is needed between colours.  The same operations
+    // in the same order as wg_sweep.
+    __shared__ T xs[kLdsNodes], bs[kLdsNodes], as[kLdsNodes], anbs[kLdsEntries];
+    __shared__ int rows[kLdsNodes + 1], cols[kLdsEntries], perms[kLdsNodes], cps[kLdsNodes + 1];
+    for (int i = threadIdx.x; i < Z.n; i += blockDim.x) {
+      xs[i] = Z.x[i];
+      bs[i] = Z.b[i];
+      as[i] = Z.ap[i];
+      perms[i] = Z.cperm[i];
+    }
+    for (int i = threadIdx.x; i <= Z.n; i += blockDim.x) rows[i] = Z.row[i];
+    for (int i = threadIdx.x; i <= Z.ncol; i += blockDim.x) cps[i] = Z.cptr[i];
+    for (int k = threadIdx.x; k < Z.nnz; k += blockDim.x) {
+      cols[k] = Z.col[k];
+      anbs[k] = Z.anb[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+      const int t = threadIdx.x;
+      auto colour = [&](int c) {
+        const int q = cps[c] + t;
+        if (q < cps[c + 1]) {
+          const int i = perms[q];
+          T acc = bs[i];
+          for (int k = rows[i]; k < rows[i + 1]; k++) acc = acc + anbs[k] * xs[cols[k]];
+          xs[i] = acc / as[i];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      };
+      for (int k = 0; k < kCoarseSweeps; k++) {
+        for (int c = 0; c < Z.ncol; c++) colour(c);
+        for (int c = Z.ncol - 1; c >= 0; c--) colour(c);
+      }
+      for (int i = t; i < Z.n; i += 64) Z.x[i] = xs[i];
+    }
+    __syncthreads();
+  } else {
+    for (int k = 0; k < kCoarseSweeps; k++) {
+      wg_sweep(Z, false);
+      wg_sweep(Z, true);
+    }
   }
   for (int l = nlev - 2; l >= lc; l--) {
     const Lev<T> &F = lv[l];
@@ -485,6 +563,10 @@ __global__ void k_mg_sweep(Lev<T> L, int c0, int cnt, const CgCtl *ctl) {
 template <typename T>
 __global__ void k_mg_restrict(Lev<T> F, Lev<T> C, const CgCtl *ctl) {
   d_mg_restrict<T>(blockIdx.x * blockDim.x + threadIdx.x, F, C, ctl);
+}
+template <typename T, int NL>
+__global__ void k_mg_restrict_lanes(Lev<T> F, Lev<T> C, const CgCtl *ctl) {
+  d_mg_restrict_lanes<T, NL>(blockIdx.x * blockDim.x + threadIdx.x, F, C, ctl);
 }
 template <typename T>
 __global__ void k_mg_prolong(Lev<T> F, const T *__restrict__ xc, const CgCtl *ctl) {
@@ -556,6 +638,13 @@ __global__ void g_mg_restrict(GroupMap g, const MgSlot<T> *__restrict__ S, int l
   const Lev<T> F = x.lv[l], C = x.lv[l + 1];
   d_mg_restrict<T>(bx * blockDim.x + threadIdx.x, F, C, x.ctl);
 }
+template <typename T, int NL>
+__global__ void g_mg_restrict_lanes(GroupMap g, const MgSlot<T> *__restrict__ S, int l) {
+  int bx;
+  const MgSlot<T> &x = S[group_slot(g, bx)];
+  const Lev<T> F = x.lv[l], C = x.lv[l + 1];
+  d_mg_restrict_lanes<T, NL>(bx * blockDim.x + threadIdx.x, F, C, x.ctl);
+}
 template <typename T>
 __global__ void g_mg_prolong(GroupMap g, const MgSlot<T> *__restrict__ S, int l) {
   int bx;
@@ -608,12 +697,55 @@ __global__ void g_cg_log(int n, const MgSlot<T> *__restrict__ S) {
 
 inline int nb(int n, int b) { return (n + b - 1) / b; }
 
+// lanes per aggregate of the restriction from a level whose aggregates have
+// at most maxm members: the least power of two >= maxm, 0 (one lane per
+// aggregate, k_mg_restrict) above a wavefront
+inline int restrict_lanes(int maxm) {
+  int nl = 1;
+  while (nl < maxm) nl <<= 1;
+  return nl <= 64 ? nl : 0;
+}
+
+// the restriction from level l of a solve alone
+template <typename T>
+void launch_restrict(int nl, const Lev<T> &F, const Lev<T> &C, const CgCtl *ctl, hipStream_t st) {
+  const int n = C.n * (nl > 0 ? nl : 1);
+  const dim3 g(nb(n, kBlk)), b(kBlk);
+  switch (nl) {
+    case 1: hipLaunchKernelGGL((k_mg_restrict_lanes<T, 1>), g, b, 0, st, F, C, ctl); break;
+    case 2: hipLaunchKernelGGL((k_mg_restrict_lanes<T, 2>), g, b, 0, st, F, C, ctl); break;
+    case 4: hipLaunchKernelGGL((k_mg_restrict_lanes<T, 4>), g, b, 0, st, F, C, ctl); break;
+    case 8: hipLaunchKernelGGL((k_mg_restrict_lanes<T, 8>), g, b, 0, st, F, C, ctl); break;
+    case 16: hipLaunchKernelGGL((k_mg_restrict_lanes<T, 16>), g, b, 0, st, F, C, ctl); break;
+    case 32: hipLaunchKernelGGL((k_mg_restrict_lanes<T, 32>), g, b, 0, st, F, C, ctl); break;
+    case 64: hipLaunchKernelGGL((k_mg_restrict_lanes<T, 64>), g, b, 0, st, F, C, ctl); break;
+    default: hipLaunchKernelGGL(k_mg_restrict<T>, g, b, 0, st, F, C, ctl); break;
+  }
+}
+// the same for a grouped launch (nbk blocks of blk lanes)
+template <typename T>
+void launch_restrict_group(int nl, int nbk, int blk, const GroupMap &gm, const MgSlot<T> *S, int l,
+                           hipStream_t st) {
+  const dim3 g(nbk), b(blk);
+  switch (nl) {
+    case 1: hipLaunchKernelGGL((g_mg_restrict_lanes<T, 1>), g, b, 0, st, gm, S, l); break;
+    case 2: hipLaunchKernelGGL((g_mg_restrict_lanes<T, 2>), g, b, 0, st, gm, S, l); break;
+    case 4: hipLaunchKernelGGL((g_mg_restrict_lanes<T, 4>), g, b, 0, st, gm, S, l); break;
+    case 8: hipLaunchKernelGGL((g_mg_restrict_lanes<T, 8>), g, b, 0, st, gm, S, l); break;
+    case 16: hipLaunchKernelGGL((g_mg_restrict_lanes<T, 16>), g, b, 0, st, gm, S, l); break;
+    case 32: hipLaunchKernelGGL((g_mg_restrict_lanes<T, 32>), g, b, 0, st, gm, S, l); break;
+    case 64: hipLaunchKernelGGL((g_mg_restrict_lanes<T, 64>), g, b, 0, st, gm, S, l); break;
+    default: hipLaunchKernelGGL(g_mg_restrict<T>, g, b, 0, st, gm, S, l); break;
+  }
+}
+
 }  // namespace
 // device copy of the hierarchy and the solve's work vectors
 struct MgCgSolver {
   int fp32 = 0, ne = 0, nbf = 0, nlev = 0, lc = 0;
   std::vector<void *> mem;
   std::vector<std::vector<int>> cptr;  // host colour offsets per level
+  std::vector<int> lanes;              // restriction lanes per aggregate, per level
   // per level, for each real type (index 0 double, 1 float)
   std::vector<Lev<double>> lvd;
   std::vector<Lev<float>> lvf;
@@ -720,8 +852,7 @@ int vcycle(MgCgSolver *s, const std::vector<Lev<T>> &lv, const Lev<T> *d_lv, hip
   const int top = lc < nlev ? lc : nlev - 1;
   for (int l = 0; l < top; l++) {
     sweep(l, false);
-    hipLaunchKernelGGL(k_mg_restrict<T>, dim3(nb(lv[l + 1].n, kBlk)), dim3(kBlk), 0, st, lv[l],
-                       lv[l + 1], ctl);
+    launch_restrict<T>(s->lanes[l], lv[l], lv[l + 1], ctl, st);
     s->launches++;
   }
   if (lc < nlev) {
@@ -758,7 +889,13 @@ int mgcg_create(const MgHier &h, int nbf, int fp32, MgCgSolver **out) {
   // level 0) of at most kCoarseMax nodes
   s->lc = s->nlev;
   while (s->lc - 1 >= 1 && h.lv[s->lc - 1].n <= kCoarseMax) s->lc--;
-  for (int l = 0; l < s->nlev; l++) s->cptr.push_back(h.lv[l].cptr);
+  for (int l = 0; l < s->nlev; l++) {
+    s->cptr.push_back(h.lv[l].cptr);
+    int maxm = 1;
+    const std::vector<int> &mp = h.lv[l].mptr;
+    for (size_t g = 0; g + 1 < mp.size(); g++) maxm = std::max(maxm, mp[g + 1] - mp[g]);
+    s->lanes.push_back(restrict_lanes(maxm));
+  }
   const int ne = s->ne, nall = ne + nbf;
   MGCALL(s->alloc(&s->r, ne));
   MGCALL(s->alloc(&s->q, ne));
@@ -1093,9 +1230,16 @@ int solve_group(MgGroup *G, MgCgSolver *const *ss, const MgCgSystem *sys, int n,
     // V-cycle
     for (int l = 0; l < maxtop; l++) {
       sweep(l, false);
-      nbk = map([&](int k) { return l < top_of(k) ? nb(host_levels<T>(ss[k])[l + 1].n, blk) : 0; });
+      // one lane count for the launch: the largest of the systems taking part
+      // (0, one lane per aggregate, if any needs it); sums do not depend on it
+      int nl = 1;
+      for (int k = 0; k < n; k++)
+        if (l < top_of(k)) nl = (nl == 0 || ss[k]->lanes[l] == 0) ? 0 : std::max(nl, ss[k]->lanes[l]);
+      nbk = map([&](int k) {
+        return l < top_of(k) ? nb(host_levels<T>(ss[k])[l + 1].n * (nl > 0 ? nl : 1), blk) : 0;
+      });
       if (nbk > 0) {
-        hipLaunchKernelGGL(g_mg_restrict<T>, dim3(nbk), dim3(blk), 0, st, g, S, l);
+        launch_restrict_group<T>(nl, nbk, blk, g, S, l, st);
         launches++;
       }
     }

@@ -146,9 +146,10 @@ void or_calc_residual(const double *phi, const double *ap, const double *anb,
                       int ne, int nf, int nbf, double *res, double *res_max);
 void or_solve(or_case *c, const char *name, double *phi);
 /* colour order of the subdomain pressure-correction smoother (extension):
-   order 1 colours every subdomain greedily in its local order (a cell takes
-   the least colour none of its earlier interior neighbours has) and visits
-   the cells by colour, then by local index; order 0 restores the
+   orders 1 and 2 colour every subdomain greedily in global cell order (its
+   cells taken by increasing global index l2g; a cell takes the least colour
+   that none of its interior neighbours with a smaller global index has) and
+   visit the cells by colour, then by local index; order 0 restores the
    reference's order. */
 void or_set_pc_order(or_case *c, int order);
 void or_set_gs_order(or_case *c, int order);

@@ -27,8 +27,12 @@
  *   - each level's graph: level 0 is the mesh CSR (interior neighbours);
  *     level l+1 has an edge I-J when an edge of level l joins them; rows
  *     list neighbours in increasing order.
- *   - colours: greedy in index order (least colour none of the earlier
- *     neighbours has); the visiting order is by colour, then index.
+ *   - colours: two, by the parity of the breadth-first distance from the
+ *     least node of each connected part (neighbours in CSR order), when that
+ *     is a proper colouring (the graphs of hex boxes and of their box-shaped
+ *     aggregates are bipartite); otherwise greedy in index order (least
+ *     colour none of the earlier neighbours has); the visiting order is by
+ *     colour, then index.
  *   - Galerkin operator A_{l+1} = P^T A_l P for the piecewise-constant P:
  *     ap_I = (sum over members i of ap_i, in member order) minus (anb of the
  *     level-l entries inside I, in (member, CSR) order); anb_IJ = sum of the
@@ -247,9 +251,40 @@ static void bisect_node(bisect_t *B, int *ids, int n, long parent_count) {
 
 static void colour_level(or_mglevel *L, int interior_limit) {
   const int n = L->n;
-  int *col = calloc((size_t)n, sizeof(int));
-  int ncol = 0;
-  for (int e = 0; e < n; e++) {
+  int *col = malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+  int *queue = malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+  int ncol = 0, two = 1;
+  for (int e = 0; e < n; e++) col[e] = -1;
+  /* two colours by breadth-first parity from the least node of each part */
+  for (int s = 0; s < n && two; s++) {
+    if (col[s] >= 0) continue;
+    col[s] = 0;
+    int qh = 0, qt = 0;
+    queue[qt++] = s;
+    while (qh < qt && two) {
+      const int e = queue[qh++];
+      for (int k = L->row[e]; k < L->row[e + 1]; k++) {
+        const int j = L->col[k];
+        if (j < 0 || j >= interior_limit || j == e) continue;
+        if (col[j] < 0) {
+          col[j] = 1 - col[e];
+          queue[qt++] = j;
+        } else if (col[j] == col[e]) {
+          two = 0;
+          break;
+        }
+      }
+    }
+  }
+  free(queue);
+  if (two) {
+    for (int e = 0; e < n; e++)
+      if (col[e] + 1 > ncol) ncol = col[e] + 1;
+  } else {
+    for (int e = 0; e < n; e++) col[e] = 0;
+  }
+  /* otherwise greedy in index order */
+  for (int e = 0; e < n && !two; e++) {
     unsigned long long used = 0;
     for (int k = L->row[e]; k < L->row[e + 1]; k++) {
       const int j = L->col[k];

@@ -185,6 +185,40 @@ def _ocall(oc, fn, *args):
     getattr(oracle_lib.lib(), fn)(oc.p, *args)
 
 
+@pytest.mark.parametrize("scale", [0.0, 1e10])
+@pytest.mark.parametrize("order", ["reference", "colored", "mixed"])
+def test_pc_solve_stopping_early_bitwise(order, scale):
+    """The 4-subdomain pc solve stopping before its iteration cap (quirk Q1
+    keeps every ordinary solve at the cap).  With the face fluxes scaled by
+    1e10 the accumulated residual passes the tenfold test early (iteration
+    30 in the reference order, 50 in the colour orders); with zero fluxes
+    the right-hand side is zero and the solve stops on entry.  The blocks
+    queued after the stop (halo exchanges that do not read the active flag)
+    are covered by the comparison of phic."""
+    eq, oc = make_pair(8, 0.1, 4)
+    eq.set_pc_order(order)
+    oc.set_pc_order(order)
+    eq.run(1, 1)
+    oc.run(1, 1)
+    eq.set("mip", eq.get("mip") * scale)
+    om = oc.arr("mip")
+    om *= scale
+    assert_fields_equal(eq, oc, ["mip"])
+    eq.calc_coef_p()
+    _ocall(oc, "or_calc_coef_p")
+    eq.set_a_0("phic")
+    oc.arr("phic")[:] = 0.0
+    eq.clear_log()
+    n0 = len(oc.log())
+    eq.solve("pc")
+    oracle_lib.lib().or_solve(oc.p, b"pc", oc.c.phic)
+    assert_fields_equal(eq, oc, ["phic"])
+    gl, ol = eq.log(), oc.log()[n0:]
+    assert len(gl) == len(ol) == 1
+    assert_logs_match(gl, ol)
+    assert gl[0][1] == (0 if scale == 0.0 else 30 if order == "reference" else 50)
+
+
 def test_routines_one_by_one():
     """Each C-ABI routine against the matching oracle routine, from a state
     reached after two outer iterations (non-trivial fields)."""

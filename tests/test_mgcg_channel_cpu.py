@@ -326,3 +326,38 @@ def test_steady_mode_needs_relaxation_in_the_oracle_contract():
     e = 2 + 6 * (2 + 6 * 2)  # an interior cell
     s = anb[idx[e] - 1:idx[e + 1] - 1].sum()
     assert ap[e] * 0.7 == pytest.approx(s, rel=1e-12)
+
+
+@pytest.mark.parametrize("dims,jit,renum,two_everywhere", [
+    ((16, 16, 16), 0.0, 0, True), ((8, 16, 32), 0.2, 0, True), ((16, 16, 16), 0.2, 77, True),
+    ((12, 12, 12), 0.0, 0, False), ((9, 9, 9), 0.2, 77, False), ((7, 5, 9), 0.1, 0, False)])
+def test_mg_colours_are_proper_and_two_where_bipartite(dims, jit, renum, two_everywhere):
+    """Every level's colouring is proper (no edge inside a colour).  The cell
+    graph of a hex box is bipartite, so level 0 has two colours (the parity
+    of the breadth-first distance from the least cell) even when jittered or
+    renumbered; on power-of-two boxes the aggregates are 2x2x2 blocks whose
+    coarse graphs are grids too, so every level has two; elsewhere the
+    coarse graphs have odd cycles and keep the greedy colouring."""
+    m = cfdl.Mesh.box(*dims, jitter=jit)
+    if renum:
+        m.renumber(renum)
+    ne = m.info()["ne"]
+    lev = 0
+    while m.mg_get(lev, "n") is not None:
+        n = int(m.mg_get(lev, "n")[0])
+        row, col = m.mg_get(lev, "row"), m.mg_get(lev, "col")
+        cptr, cperm = m.mg_get(lev, "cptr"), m.mg_get(lev, "cperm")
+        colour = np.empty(n, np.int64)
+        for c in range(len(cptr) - 1):
+            colour[cperm[cptr[c]:cptr[c + 1]]] = c
+        edges = 0
+        for i in range(n):
+            for k in range(row[i], row[i + 1]):
+                j = col[k]
+                if j < (ne if lev == 0 else n) and j != i:
+                    assert colour[i] != colour[j], (lev, i, j)
+                    edges += 1
+        if edges and (lev == 0 or two_everywhere):
+            assert len(cptr) - 1 == 2 and colour[0] == 0, lev
+        lev += 1
+    assert lev >= 2

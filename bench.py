@@ -155,32 +155,54 @@ TRAFFIC_KIB_128 = 7706.4
 PUT_STATE = ("u", "v", "w", "p", "u0", "v0", "w0", "gu", "gv", "gw", "gp", "mip", "mip0")
 GET_STATE = ("u", "v", "w", "p", "gu", "gv", "gw", "gp", "gpc", "mip", "bu", "bv", "bw", "d", "dc")
 GET_WORK = ("ap", "anb", "b", "phic")
+PULL_EXTRA = ("u0", "v0", "w0", "mip0")
 
 
-def pcie_rate(eq, ne, args, steps=4):
-    """Outer iterations as the drop-in Fortran modules run them: the host
-    arrays (pageable, as the Fortran host's) go to the device before every
-    solve_uvwp and come back after it.  Printed as its own JSON line; the
-    headline value stays the device-resident rate."""
-    host = {k: eq.get(k) for k in set(PUT_STATE + GET_STATE + GET_WORK)}
-    nbytes = 8 * (sum(host[k].size for k in PUT_STATE) +
-                  sum(host[k].size for k in GET_STATE + GET_WORK))
-    eq.synchronize()
-    t = time.perf_counter()
-    for k in range(steps):
-        for name in PUT_STATE:
-            eq.set(name, host[name])
-        eq.update_boundaries()
-        eq.solve_uvwp()
-        for name in GET_STATE + GET_WORK:
-            host[name] = eq.get(name)
-        if (k + 1) % args.tstep_every == 0:
-            eq.update_time()
-    eq.synchronize()
-    secs = time.perf_counter() - t
-    print(json.dumps({"mode": "pcie", "metric": METRIC, "value": round(ne * steps / secs, 1),
-                      "unit": "cell-iterations/s", "ms_per_step": round(1e3 * secs / steps, 3),
-                      "host_device_bytes_per_step": nbytes, "steps": steps}), flush=True)
+def pcie_rate(eq, ne, tstep_every, steps=30, rho=None, mu=None):
+    """Outer iterations as the drop-in Fortran modules run them, with host
+    arrays (pageable, as the Fortran host's):
+      * per_call -- with the reference's mod_physics: the state goes to the
+        device before every solve_uvwp and the state and work arrays come
+        back after it;
+      * resident -- with the drop-in mod_physics: the properties go up and
+        the new solver lines come back every call, update_time runs on the
+        device, and the state and work arrays come back every 10th time step.
+    Returns both rates; the headline value stays the device-resident rate."""
+    host = {k: eq.get(k) for k in set(PUT_STATE + GET_STATE + GET_WORK + PULL_EXTRA)}
+    rho = np.ones(ne) if rho is None else rho
+    mu = np.full(ne, 0.0025) if mu is None else mu
+    out = {}
+    for mode in ("per_call", "resident"):
+        eq.synchronize()
+        moved = 0
+        t = time.perf_counter()
+        for k in range(steps):
+            if mode == "per_call":
+                for name in PUT_STATE:
+                    eq.set(name, host[name])
+                    moved += host[name].nbytes
+            eq.set_properties(rho, mu)
+            moved += rho.nbytes + mu.nbytes
+            eq.update_boundaries()
+            eq.solve_uvwp()
+            eq.log()
+            eq.clear_log()
+            if mode == "per_call":
+                for name in GET_STATE + GET_WORK:
+                    host[name] = eq.get(name)
+                    moved += host[name].nbytes
+            if (k + 1) % tstep_every == 0:
+                eq.update_time()
+                if mode == "resident" and ((k + 1) // tstep_every) % 10 == 0:
+                    for name in GET_STATE + GET_WORK + PULL_EXTRA:
+                        host[name] = eq.get(name)
+                        moved += host[name].nbytes
+        eq.synchronize()
+        secs = time.perf_counter() - t
+        out[mode] = {"value": round(ne * steps / secs, 1), "unit": "cell-iterations/s",
+                     "ms_per_step": round(1e3 * secs / steps, 3),
+                     "host_device_bytes_per_step": int(moved / steps), "steps": steps}
+    return out
 
 
 def write_ref_mesh(mesh, path):
@@ -435,11 +457,32 @@ def extras(cfdl, args, base_mesh, headline):
                          "RCM ordered, parabolic inlet u_max=1, zero-gradient outlet, Re=200, "
                          "steady", **r)
 
+    def transfers128():
+        mu = 1.0 / args.re
+        eq = cfdl.UVWP.cavity(base_mesh, rho=1.0, mu=mu, dt=float("inf"))
+        eq.set_relaxation(args.alpha_u, args.alpha_p)
+        ne = base_mesh.info()["ne"]
+        r = pcie_rate(eq, ne, args.tstep_every, 30, np.ones(ne), np.full(ne, mu))
+        eq.close()
+        return dict(case="the headline configuration (128^3, reference order) driven the way "
+                         "the Fortran drop-in modules drive it, host arrays pageable; "
+                         "per_call: reference mod_physics, resident: drop-in mod_physics",
+                    device_resident=headline["value"], **r)
+
+    def serving():
+        sat = serving_run(cfdl, 64, 1000.0, args.tol, 100, 10, 32, warm=2)
+        stream = serving_run(cfdl, 48, 4.0, args.tol, 100, 10, 32, warm=2)
+        return {"saturated": sat, "stream_4_per_s": stream,
+                "note": "the configuration's 512-request stream to 1e-6 or 500 outer "
+                        "iterations: bench.py --serving"}
+
     section("converge_cavity32", conv32)
     section("modes_128", modes128)
     section("converge_cavity128", conv128)
     section("modes_256", modes256)
     section("channel_128", channel128)
+    section("dropin_transfers_128", transfers128)
+    section("serving", serving)
     return out
 
 
@@ -484,6 +527,8 @@ def main():
     ap.add_argument("--serving-max-outer", type=int, default=500)
     ap.add_argument("--serving-chunk", type=int, default=10,
                     help="outer iterations per batch run (convergence checked between runs)")
+    ap.add_argument("--serving-warm", type=int, default=2,
+                    help="contexts per (kind, N) created before the stream starts")
     ap.add_argument("--pcie", action="store_true",
                     help="also time outer iterations with the host transfers of the drop-in "
                          "Fortran solve_uvwp (state up, state and work arrays down)")
@@ -540,7 +585,8 @@ def main():
             eq.update_time()
 
     if args.pcie:
-        pcie_rate(eq, ne, args)
+        r = pcie_rate(eq, ne, args.tstep_every)
+        print(json.dumps({"mode": "pcie", "metric": METRIC, **r}), flush=True)
 
     for _ in range(args.warmup):
         step()
@@ -613,7 +659,7 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def serving_run(cfdl, nreq, rate, tol, max_outer, chunk, max_cases, seed=20261202):
+def serving_run(cfdl, nreq, rate, tol, max_outer, chunk, max_cases, seed=20261202, warm=0):
     """The serving stream (an extension; the reference solves one case per
     run): seeded Poisson arrivals of steady cavity and channel solves from
     rest (N^3, N in {16, ..., 96}, Re log-uniform in [100, 1000], kind a fair
@@ -624,7 +670,8 @@ def serving_run(cfdl, nreq, rate, tol, max_outer, chunk, max_cases, seed=2026120
     reqs = serving.make_requests(nreq, rate, seed)
     progress("serving %d requests, %g per s, tol %g or %d outer iterations"
              % (nreq, rate, tol, max_outer))
-    out = serving.serve(reqs, max_cases=max_cases, tol=tol, max_outer=max_outer, chunk=chunk)
+    out = serving.serve(reqs, max_cases=max_cases, tol=tol, max_outer=max_outer, chunk=chunk,
+                        warm=warm)
     cell_its = sum(r.n ** 3 * out["outer"][r.idx] for r in reqs)
     return {"metric": "batched solves/s with p50/p99 latency",
             "solves_per_s": round(out["solves_per_s"], 3),
@@ -637,6 +684,11 @@ def serving_run(cfdl, nreq, rate, tol, max_outer, chunk, max_cases, seed=2026120
             "tol": tol, "max_outer": max_outer,
             "cell_iterations_per_s": round(cell_its / out["wall_s"], 1),
             "contexts_created": out["contexts_created"],
+            "contexts_warmed_before_start": out["contexts_warmed"],
+            "warm_up_s": round(out["warm_s"], 2),
+            "host_seconds": {"admission": round(out["admit_s"], 2),
+                             "batch_runs": round(out["run_s"], 2),
+                             "retirement": round(out["retire_s"], 2)},
             "case": "steady lid-driven cavity or channel (50/50) on N^3 hex, N in {16,...,96}, "
                     "Re log-uniform in [100, 1000], relaxation 0.7/0.3, from rest",
             "pc_order": "mgcg-mixed", "gs_order": "colored", "assembly": "f32",
@@ -645,7 +697,8 @@ def serving_run(cfdl, nreq, rate, tol, max_outer, chunk, max_cases, seed=2026120
 
 def serving_line(cfdl, args):
     out = serving_run(cfdl, args.serving_requests, args.serving_rate, args.tol,
-                      args.serving_max_outer, args.serving_chunk, args.serving_cases)
+                      args.serving_max_outer, args.serving_chunk, args.serving_cases,
+                      warm=args.serving_warm)
     print(json.dumps(dict(out, mode="serving")), flush=True)
 
 

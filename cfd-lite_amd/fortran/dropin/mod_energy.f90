@@ -75,6 +75,7 @@ contains
 
   ! the state the energy routines read, host -> device
   subroutine put_energy(ctx, eqn, geom, prop, dt)
+    use mod_solver, only: cfdl_resident
     type(c_ptr) :: ctx
     class(energy_t) :: eqn
     type(geometry_t) :: geom
@@ -90,7 +91,10 @@ contains
     call cfdl_check(cfdl_set_field(ctx, CFDL_ENTH0, eqn%phi0, n), 'upload')
     call cfdl_check(cfdl_set_field(ctx, CFDL_GENTH, eqn%grad, 3*n), 'upload')
     call cfdl_check(cfdl_set_field(ctx, CFDL_GT, eqn%gt, 3*n), 'upload')
-    call cfdl_check(cfdl_set_field(ctx, CFDL_MIP, eqn%mip, int(geom%nf, c_int64_t)), 'upload')
+    ! the face fluxes are the uvwp state's: on the device already when it
+    ! owns that state (the drop-in mod_physics)
+    if (.not. cfdl_resident) &
+      call cfdl_check(cfdl_set_field(ctx, CFDL_MIP, eqn%mip, int(geom%nf, c_int64_t)), 'upload')
   end subroutine
 
   subroutine get_work(ctx, geom, ap, anb, b)

@@ -16,6 +16,9 @@ module mod_solver
   type(c_ptr), save :: cfdl_active = c_null_ptr
   integer, save :: cfdl_ne = 0, cfdl_nf = 0, cfdl_nbf = 0, cfdl_nit = -1
   integer, save :: cfdl_logged = 0
+  ! set by the drop-in mod_physics: the device owns the uvwp state between
+  ! the steps where the program writes output (mod_uvwp, mod_energy)
+  logical, save :: cfdl_resident = .false.
 
 contains
 

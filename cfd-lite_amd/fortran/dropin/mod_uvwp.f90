@@ -3,10 +3,13 @@
 ! adjust_pc, calc_coef_uvw, calc_coef_p, update_uvwp, calc_mip and the BC
 ! setters) with every computation done by libcfdl_hip.so on the GPU.
 !
-! The host arrays of uvwp_t stay authoritative between calls, as the rest
-! of the reference (mod_physics's update_time, output) reads and writes
-! them: each entry point uploads the state it reads and downloads what it
-! writes.  The BC setters keep the reference's host-side role of naming the
+! With the reference's mod_physics the host arrays of uvwp_t stay
+! authoritative between calls, as the rest of the reference (mod_physics's
+! update_time, output) reads and writes them: each entry point uploads the
+! state it reads and downloads what it writes.  With the drop-in
+! mod_physics (mod_solver's cfdl_resident) the device owns the state: it is
+! uploaded once, solve_uvwp downloads nothing, the other entry points
+! download what they write, and pull_state refreshes the host copy.  The BC setters keep the reference's host-side role of naming the
 ! boundary type of their section; the halo values they define are applied
 ! by the device inside solve_uvwp (update_boundaries runs directly before
 ! solve_uvwp in the reference's loop, src/main.f90:52-56).
@@ -28,6 +31,8 @@ module mod_uvwp
     integer :: ctx_nit = -1, ctx_nsubd = -1
     ! per boundary interface: CFDL_BC_* kind named by its setter
     integer, allocatable :: bc_kind(:)
+    ! the device holds the state (resident mode, after the first upload)
+    logical :: on_device = .false.
   contains
     procedure :: calc_coef_uvw
   end type
@@ -68,6 +73,7 @@ contains
          g2gf_idx, prop%rho, prop%mu, dt, nit, eqn%ctx), 'context')
     eqn%ctx_nit = nit
     eqn%ctx_nsubd = nsubd
+    eqn%on_device = .false.
     call cfdl_bind(eqn%ctx, ne, geom%nf, nbf, nit)
     do s = 1, size(eqn%bcs)
       if (eqn%bc_kind(s) >= 0) call cfdl_check(cfdl_set_bc_kind(eqn%ctx, eqn%bcs(s)%esec(1), &
@@ -108,7 +114,10 @@ contains
   ! 131 before 385), so host changes to them (the VTU writer overwrites dc)
   ! cannot reach its results.
   subroutine put_inputs(eqn)
+    use mod_solver, only: cfdl_resident
     class(uvwp_t) :: eqn
+    if (cfdl_resident .and. eqn%on_device) return
+    eqn%on_device = cfdl_resident
     call dev_put(eqn, CFDL_U, eqn%u); call dev_put(eqn, CFDL_V, eqn%v); call dev_put(eqn, CFDL_W, eqn%w)
     call dev_put(eqn, CFDL_P, eqn%p)
     call dev_put(eqn, CFDL_U0, eqn%u0); call dev_put(eqn, CFDL_V0, eqn%v0); call dev_put(eqn, CFDL_W0, eqn%w0)
@@ -120,7 +129,9 @@ contains
   ! the whole state, for the routines called one by one (calc_coef_p reads
   ! dc, calc_mip d, update_uvwp dc and gpc)
   subroutine put_state(eqn)
+    use mod_solver, only: cfdl_resident
     class(uvwp_t) :: eqn
+    if (cfdl_resident .and. eqn%on_device) return
     call put_inputs(eqn)
     call dev_put(eqn, CFDL_GPC, eqn%gpc)
     call dev_put(eqn, CFDL_BU, eqn%bu); call dev_put(eqn, CFDL_BV, eqn%bv); call dev_put(eqn, CFDL_BW, eqn%bw)
@@ -136,6 +147,20 @@ contains
     call dev_get(eqn, CFDL_MIP, eqn%mip)
     call dev_get(eqn, CFDL_BU, eqn%bu); call dev_get(eqn, CFDL_BV, eqn%bv); call dev_get(eqn, CFDL_BW, eqn%bw)
     call dev_get(eqn, CFDL_D, eqn%d); call dev_get(eqn, CFDL_DC, eqn%dc)
+  end subroutine
+
+  ! resident mode: the host copy of the whole state and of the work arrays
+  ! (the drop-in mod_physics's update_time, at the steps the program writes
+  ! output after)
+  subroutine pull_state(eqn, ap, anb, b, phic)
+    class(uvwp_t) :: eqn
+    real, dimension(:) :: ap, anb, b, phic
+    if (.not. c_associated(eqn%ctx)) return
+    call get_state(eqn)
+    call dev_get(eqn, CFDL_U0, eqn%u0); call dev_get(eqn, CFDL_V0, eqn%v0)
+    call dev_get(eqn, CFDL_W0, eqn%w0); call dev_get(eqn, CFDL_MIP0, eqn%mip0)
+    call dev_get(eqn, CFDL_AP, ap); call dev_get(eqn, CFDL_ANB, anb)
+    call dev_get(eqn, CFDL_B, b); call dev_get(eqn, CFDL_PHIC, phic)
   end subroutine
 
   subroutine put_work(eqn, geom, ap, anb, b, phic)
@@ -214,7 +239,7 @@ contains
   subroutine solve_uvwp(eqn, prop, geom, dt, nit, ap, anb, b, phic, subdomain, intf, nsubd)
     use mod_properties
     use mod_subdomains
-    use mod_solver, only: print_new_log_lines
+    use mod_solver, only: print_new_log_lines, cfdl_resident
     type(uvwp_t) :: eqn
     type(properties_t) :: prop
     type(geometry_t) :: geom
@@ -228,8 +253,10 @@ contains
     call put_inputs(eqn)
     call cfdl_check(cfdl_update_boundaries(eqn%ctx), 'update_boundaries')
     call cfdl_check(cfdl_solve_uvwp(eqn%ctx), 'solve_uvwp')
-    call get_state(eqn)
-    call get_work(eqn, geom, ap, anb, b, phic)
+    if (.not. cfdl_resident) then
+      call get_state(eqn)
+      call get_work(eqn, geom, ap, anb, b, phic)
+    end if
     call print_new_log_lines()
   end subroutine
 

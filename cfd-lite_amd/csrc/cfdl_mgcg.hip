@@ -360,9 +360,26 @@ struct Lev {
   T *apw, *anbw;  // writable views of ap / anb (coarse levels, Galerkin)
 };
 
+// acc + sum over row i of anb * x[col], in CSR order; a six-entry
+// (hexahedral) row issues all its loads before the sum
 template <typename T>
 __device__ inline T row_sum(const Lev<T> &L, int i, T acc) {
-  for (int k = L.row[i]; k < L.row[i + 1]; k++) acc = acc + L.anb[k] * L.x[L.col[k]];
+  const int r0 = L.row[i], r1 = L.row[i + 1];
+  if (r1 - r0 == 6) {
+    int c[6];
+    T a[6], v[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      c[k] = L.col[r0 + k];
+      a[k] = L.anb[r0 + k];
+    }
+#pragma unroll
+    for (int k = 0; k < 6; k++) v[k] = L.x[c[k]];
+#pragma unroll
+    for (int k = 0; k < 6; k++) acc = acc + a[k] * v[k];
+    return acc;
+  }
+  for (int k = r0; k < r1; k++) acc = acc + L.anb[k] * L.x[L.col[k]];
   return acc;
 }
 

@@ -88,6 +88,8 @@ _SIGS = {
     "cfdl_batch_set_block": (C.c_int, [VP, C.c_int]),
     "cfdl_batch_block": (C.c_int, [VP, PI]),
     "cfdl_batch_run": (C.c_int, [VP, C.c_int, C.c_int]),
+    "cfdl_batch_set_graphs": (C.c_int, [VP, C.c_int]),
+    "cfdl_batch_graph_stats": (C.c_int, [VP, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "cfdl_batch_synchronize": (C.c_int, [VP]),
     "cfdl_batch_destroy": (None, [VP]),
     "cfdl_log_count": (C.c_int, [VP, PI]),

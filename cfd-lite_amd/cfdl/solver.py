@@ -331,7 +331,7 @@ class Batch:
 
     MAX_CASES = 32
 
-    def __init__(self, block=0):
+    def __init__(self, block=0, graphs=True):
         self._lib = load_library()
         h = VP()
         check(self._lib.cfdl_batch_create(C.byref(h)))
@@ -339,6 +339,18 @@ class Batch:
         self.cases = []
         if block:
             self.set_block(block)
+        if not graphs:
+            self.set_graphs(False)
+
+    def set_graphs(self, on):
+        """Outer iterations as captured HIP graphs (default) or direct launches."""
+        check(self._lib.cfdl_batch_set_graphs(self._h, 1 if on else 0))
+
+    def graph_stats(self):
+        """(graph captures, graph launches) so far."""
+        a, b = C.c_int64(0), C.c_int64(0)
+        check(self._lib.cfdl_batch_graph_stats(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def attach(self, eq):
         check(self._lib.cfdl_batch_attach(self._h, eq._h))

@@ -3942,6 +3942,18 @@ struct cfdl_batch {
   bool tuning = false;     // tuner events recorded, not yet read
   hipEvent_t tev[6] = {};
   int64_t runs = 0;
+  // Captured graphs of the iteration's segments (prefix, one momentum-solver
+  // iteration per flag slot, middle, one CG iteration per flag slot, suffix,
+  // update_time), valid for one composition and lane count
+  struct Seg {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t x = nullptr;
+  };
+  bool use_graphs = true;
+  bool graphs_valid = false;
+  int graph_blk = 0;
+  Seg seg[8];
+  int64_t graph_launches = 0, captures = 0;
 };
 
 namespace {
@@ -4065,6 +4077,7 @@ int batch_sync_slots(cfdl_batch *b) {
     HIPCHK(hipStreamSynchronize(b->st));
     if (n > 0) HIPCHK(hipMemcpy(b->d_slots, h.data(), sizeof(BSlot) * n, hipMemcpyHostToDevice));
     b->key.swap(key);
+    b->graphs_valid = false;
   }
   return CFDL_OK;
 }
@@ -4088,69 +4101,73 @@ struct BatchLaunch {
   }
 };
 
-// solve_gs_uvw of every case in the colour order (solve_gs_n with three
-// right-hand sides): iteration it+1 is queued before the host reads the
-// flags after iteration it; an iteration queued after a case stopped
-// changes nothing of it (its sweeps skip, its residual is recomputed from
-// the same values and not used)
-int batch_solve_gs_uvw(cfdl_batch *b, int blk) {
+// The iteration in segments, each only enqueueing work on the batch's
+// stream (so it can be captured once and replayed as a graph):
+//   prefix: update_boundaries, calc_coef_uvw, the momentum system gathered
+//           into the colour numbering, its initial residuals;
+//   gs:     one iteration of solve_gs_uvw for every case still running, its
+//           stop test, the cases' flags copied to host slot 0 or 1;
+//   middle: the system scattered back, the momentum log lines, calc_grad of
+//           u, v, w, calc_mip, calc_coef_p, set_a_0(phic), the pc solve's
+//           setup;
+//   cg:     one CG iteration with its V-cycle, flags to host slot 0 or 1;
+//   suffix: the pc log lines, adjust_pc, calc_grad(phic), update_uvwp;
+//   time:   update_time.
+// The host runs the gs and cg segments until no case's flag is set (it
+// queues iteration it+1 before reading the flags after iteration it; an
+// iteration queued after a case stopped changes nothing of that case: its
+// kernels skip, and a recomputed residual of unchanged values is not used).
+int seg_prefix(cfdl_batch *b, int blk) {
   BatchLaunch L(b);
   const BSlot *S = b->d_slots;
   hipStream_t st = b->st;
-  const int n = L.n;
-  int nb = L.map([&](cfdl_ctx *c) { return nbk(int64_t{c->ne} + c->nbf, blk); });
+  int nb = L.map([&](cfdl_ctx *c) { return nbk(c->nbf, blk); });
+  if (nb > 0) hipLaunchKernelGGL(gb_bc_update, dim3(nb), dim3(blk), 0, st, L.g, S);
+  nb = L.map([&](cfdl_ctx *c) { return c->nbf > 0 && c->nout > 0 ? 1 : 0; });
+  if (nb > 0) hipLaunchKernelGGL(gb_outlet_flux, dim3(nb), dim3(kOutThreads), 0, st, L.g, S);
+  nb = L.map([&](cfdl_ctx *c) { return nbk(c->nbf, blk); });
+  if (nb > 0) hipLaunchKernelGGL(gb_bc_energy, dim3(nb), dim3(blk), 0, st, L.g, S);
+  nb = L.map([&](cfdl_ctx *c) { return (c->ne + kFaceCells - 1) / kFaceCells; });
+  if (b->asm32)
+    hipLaunchKernelGGL(gb_coef_uvw6<float>, dim3(nb), dim3(kFaceThreads), 0, st, L.g, S);
+  else
+    hipLaunchKernelGGL(gb_coef_uvw6<double>, dim3(nb), dim3(kFaceThreads), 0, st, L.g, S);
+  nb = L.map([&](cfdl_ctx *c) { return nbk(int64_t{c->ne} + c->nbf, blk); });
   hipLaunchKernelGGL(gb_tile_gather, dim3(nb), dim3(blk), 0, st, L.g, S);
-  auto residual = [&]() {
-    const int np = L.map([&](cfdl_ctx *c) { return c->nblk_g; });
-    hipLaunchKernelGGL(gb_resid_partial, dim3(np, 3), dim3(kBlock), 0, st, L.g, S);
-    const int nf = L.map([&](cfdl_ctx *) { return 1; });
-    hipLaunchKernelGGL(gb_resid_final, dim3(nf, 3), dim3(kBlock), 0, st, L.g, S);
-  };
-  residual();
+  const int np = L.map([&](cfdl_ctx *c) { return c->nblk_g; });
+  hipLaunchKernelGGL(gb_resid_partial, dim3(np, 3), dim3(kBlock), 0, st, L.g, S);
   nb = L.map([&](cfdl_ctx *) { return 1; });
+  hipLaunchKernelGGL(gb_resid_final, dim3(nb, 3), dim3(kBlock), 0, st, L.g, S);
   hipLaunchKernelGGL(gb_gs_init, dim3(nb), dim3(64), 0, st, L.g, S);
   HIPCHK(hipGetLastError());
-  int nit = 0, nq = 0;
-  for (cfdl_ctx *c : b->ctx) {
-    nit = std::max(nit, c->nit);
-    nq = std::max(nq, static_cast<int>(c->gcol_ptr.size()) - 1);
-  }
-  for (int it = 0; it < nit; it++) {
-    for (int d = 0; d < 2; d++)
-      for (int i = 0; i < nq; i++) {
-        nb = L.map([&](cfdl_ctx *c) {
-          const int q_n = static_cast<int>(c->gcol_ptr.size()) - 1;
-          if (i >= q_n) return 0;
-          const int q = d == 0 ? i : q_n - 1 - i;
-          return nbk(c->gcol_ptr[q + 1] - c->gcol_ptr[q], blk);
-        });
-        if (nb > 0) hipLaunchKernelGGL(gb_gs_colour, dim3(nb), dim3(blk), 0, st, L.g, S, i, d, 1.0);
-      }
-    residual();
-    nb = L.map([&](cfdl_ctx *) { return 1; });
-    hipLaunchKernelGGL(gb_gs_iter_end, dim3(nb), dim3(64), 0, st, L.g, S);
-    hipLaunchKernelGGL(gb_gs_flags, dim3(1), dim3(kMaxGroup), 0, st, n, S, b->d_flags);
-    HIPCHK(hipGetLastError());
-    int *hf = b->h_flags + (it & 1) * kMaxGroup;
-    HIPCHK(hipMemcpyAsync(hf, b->d_flags, sizeof(int) * n, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipEventRecord(b->ev[it & 1], st));
-    if (it > 0) {
-      HIPCHK(hipEventSynchronize(b->ev[(it - 1) & 1]));
-      const int *pf = b->h_flags + ((it - 1) & 1) * kMaxGroup;
-      bool any = false;
-      for (int k = 0; k < n; k++) any = any || pf[k] != 0;
-      if (!any) break;
+  return CFDL_OK;
+}
+
+int seg_gs(cfdl_batch *b, int blk, int slot) {
+  BatchLaunch L(b);
+  const BSlot *S = b->d_slots;
+  hipStream_t st = b->st;
+  int nq = 0;
+  for (cfdl_ctx *c : b->ctx) nq = std::max(nq, static_cast<int>(c->gcol_ptr.size()) - 1);
+  for (int d = 0; d < 2; d++)
+    for (int i = 0; i < nq; i++) {
+      const int nb = L.map([&](cfdl_ctx *c) {
+        const int q_n = static_cast<int>(c->gcol_ptr.size()) - 1;
+        if (i >= q_n) return 0;
+        const int q = d == 0 ? i : q_n - 1 - i;
+        return nbk(c->gcol_ptr[q + 1] - c->gcol_ptr[q], blk);
+      });
+      if (nb > 0) hipLaunchKernelGGL(gb_gs_colour, dim3(nb), dim3(blk), 0, st, L.g, S, i, d, 1.0);
     }
-  }
-  nb = L.map([&](cfdl_ctx *c) { return nbk(c->ne, blk); });
-  hipLaunchKernelGGL(gb_tile_scatter, dim3(nb), dim3(blk), 0, st, L.g, S);
-  Names nm{};
-  std::strncpy(nm.n[0], "u", 7);
-  std::strncpy(nm.n[1], "v", 7);
-  std::strncpy(nm.n[2], "w", 7);
-  nb = L.map([&](cfdl_ctx *) { return 1; });
-  hipLaunchKernelGGL(gb_log_gs, dim3(nb), dim3(64), 0, st, L.g, S, nm);
+  const int np = L.map([&](cfdl_ctx *c) { return c->nblk_g; });
+  hipLaunchKernelGGL(gb_resid_partial, dim3(np, 3), dim3(kBlock), 0, st, L.g, S);
+  const int nb = L.map([&](cfdl_ctx *) { return 1; });
+  hipLaunchKernelGGL(gb_resid_final, dim3(nb, 3), dim3(kBlock), 0, st, L.g, S);
+  hipLaunchKernelGGL(gb_gs_iter_end, dim3(nb), dim3(64), 0, st, L.g, S);
+  hipLaunchKernelGGL(gb_gs_flags, dim3(1), dim3(kMaxGroup), 0, st, L.n, S, b->d_flags);
   HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(b->h_flags + slot * kMaxGroup, b->d_flags, sizeof(int) * L.n,
+                        hipMemcpyDeviceToHost, st));
   return CFDL_OK;
 }
 
@@ -4195,28 +4212,25 @@ int batch_tune_read(cfdl_batch *b) {
   return CFDL_OK;
 }
 
-int batch_iteration(cfdl_batch *b) {
+int seg_middle(cfdl_batch *b, int blk, bool tune) {
   BatchLaunch L(b);
   const BSlot *S = b->d_slots;
   hipStream_t st = b->st;
-  const int blk = b->block > 0 ? b->block : b->tuned > 0 ? b->tuned : kBlock;
-  int nb = 0;
-  // update_boundaries
-  nb = L.map([&](cfdl_ctx *c) { return nbk(c->nbf, blk); });
-  if (nb > 0) hipLaunchKernelGGL(gb_bc_update, dim3(nb), dim3(blk), 0, st, L.g, S);
-  nb = L.map([&](cfdl_ctx *c) { return c->nbf > 0 && c->nout > 0 ? 1 : 0; });
-  if (nb > 0) hipLaunchKernelGGL(gb_outlet_flux, dim3(nb), dim3(kOutThreads), 0, st, L.g, S);
-  nb = L.map([&](cfdl_ctx *c) { return nbk(c->nbf, blk); });
-  if (nb > 0) hipLaunchKernelGGL(gb_bc_energy, dim3(nb), dim3(blk), 0, st, L.g, S);
-  // solve_uvwp
-  nb = L.map([&](cfdl_ctx *c) { return (c->ne + kFaceCells - 1) / kFaceCells; });
-  if (b->asm32)
-    hipLaunchKernelGGL(gb_coef_uvw6<float>, dim3(nb), dim3(kFaceThreads), 0, st, L.g, S);
-  else
-    hipLaunchKernelGGL(gb_coef_uvw6<double>, dim3(nb), dim3(kFaceThreads), 0, st, L.g, S);
+  int nb = L.map([&](cfdl_ctx *c) { return nbk(c->ne, blk); });
+  hipLaunchKernelGGL(gb_tile_scatter, dim3(nb), dim3(blk), 0, st, L.g, S);
+  Names nm{};
+  std::strncpy(nm.n[0], "u", 7);
+  std::strncpy(nm.n[1], "v", 7);
+  std::strncpy(nm.n[2], "w", 7);
+  nb = L.map([&](cfdl_ctx *) { return 1; });
+  hipLaunchKernelGGL(gb_log_gs, dim3(nb), dim3(64), 0, st, L.g, S, nm);
   HIPCHK(hipGetLastError());
-  CALL(batch_solve_gs_uvw(b, blk));
-  CALL(batch_grad_u(b, blk));
+  if (tune) {
+    CALL(batch_grad_u(b, blk));
+  } else {
+    nb = L.map([&](cfdl_ctx *c) { return nbk(c->ne, blk); });
+    hipLaunchKernelGGL(gb_grad, dim3(nb), dim3(blk), 0, st, L.g, S, 0);
+  }
   nb = L.map([&](cfdl_ctx *c) { return nbk(c->ne, blk); });
   hipLaunchKernelGGL(gb_grad, dim3(nb), dim3(blk), 0, st, L.g, S, 1);
   hipLaunchKernelGGL(gb_grad, dim3(nb), dim3(blk), 0, st, L.g, S, 2);
@@ -4230,32 +4244,17 @@ int batch_iteration(cfdl_batch *b) {
   nb = L.map([&](cfdl_ctx *c) { return nbk(int64_t{c->ne} + c->nbf, blk); });
   hipLaunchKernelGGL(gb_zero_phic, dim3(nb), dim3(blk), 0, st, L.g, S);
   HIPCHK(hipGetLastError());
-  {
-    const int n = L.n;
-    std::vector<cfdl::MgCgSolver *> sv(n);
-    std::vector<cfdl::MgCgSystem> sys(n);
-    for (int k = 0; k < n; k++) {
-      cfdl_ctx *c = b->ctx[k];
-      sv[k] = c->mg[b->fp32];
-      cfdl::MgCgSystem &y = sys[k];
-      y = cfdl::MgCgSystem{};
-      y.ne = c->ne;
-      y.nbf = c->nbf;
-      y.nit = c->nit;
-      y.row = c->md.row;
-      y.nbr = c->md.nbr;
-      y.ap = c->fd.ap;
-      y.anb = c->fd.anb;
-      y.b = c->fd.b;
-      y.phi = c->fd.phic;
-      std::strncpy(y.name, "pc", 7);
-      y.log = c->dlog;
-      y.nlog = c->dlog_n;
-    }
-    CALL(cfdl::mgcg_solve_group(b->mg, sv.data(), sys.data(), n, st, blk));
-  }
-  // adjust_pc, calc_grad(phic), update_uvwp
-  nb = L.map([&](cfdl_ctx *) { return 1; });
+  return cfdl::mgcg_group_setup(b->mg, st);
+}
+
+int seg_cg(cfdl_batch *b, int slot) { return cfdl::mgcg_group_iteration(b->mg, b->st, slot); }
+
+int seg_suffix(cfdl_batch *b, int blk) {
+  BatchLaunch L(b);
+  const BSlot *S = b->d_slots;
+  hipStream_t st = b->st;
+  CALL(cfdl::mgcg_group_finish(b->mg, st));
+  int nb = L.map([&](cfdl_ctx *) { return 1; });
   hipLaunchKernelGGL(gb_pref, dim3(nb), dim3(64), 0, st, L.g, S);
   nb = L.map([&](cfdl_ctx *c) { return nbk(c->ne, blk); });
   hipLaunchKernelGGL(gb_sub_pref, dim3(nb), dim3(blk), 0, st, L.g, S);
@@ -4267,13 +4266,11 @@ int batch_iteration(cfdl_batch *b) {
   nb = L.map([&](cfdl_ctx *c) { return nbk(c->nf, blk); });
   hipLaunchKernelGGL(gb_update_faces, dim3(nb), dim3(blk), 0, st, L.g, S);
   HIPCHK(hipGetLastError());
-  for (cfdl_ctx *c : b->ctx) c->outer++;
   return CFDL_OK;
 }
 
-int batch_update_time(cfdl_batch *b) {
+int seg_time(cfdl_batch *b, int blk) {
   BatchLaunch L(b);
-  const int blk = b->block > 0 ? b->block : b->tuned > 0 ? b->tuned : kBlock;
   for (int w = 0; w < 5; w++) {
     const int nb = L.map([&](cfdl_ctx *c) {
       return nbk(w == 3 ? int64_t{c->nf} : int64_t{c->ne} + c->nbf, blk);
@@ -4284,7 +4281,120 @@ int batch_update_time(cfdl_batch *b) {
   return CFDL_OK;
 }
 
+void graphs_drop(cfdl_batch *b) {
+  for (auto &s : b->seg) {
+    if (s.x) (void)hipGraphExecDestroy(s.x);
+    if (s.g) (void)hipGraphDestroy(s.g);
+    s.x = nullptr;
+    s.g = nullptr;
+  }
+  b->graphs_valid = false;
+}
+
+// Runs segment k: eagerly, or as its graph (captured on first use for the
+// current composition and lane count)
+template <typename F>
+int run_seg(cfdl_batch *b, bool graph, int k, F fn) {
+  if (!graph) return fn();
+  cfdl_batch::Seg &s = b->seg[k];
+  if (!s.x) {
+    HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeRelaxed));
+    const int rc = fn();
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->st, &g);
+    if (rc != CFDL_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    HIPCHK(e);
+    hipGraphExec_t x = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      HIPCHK(ei);
+    }
+    s.g = g;
+    s.x = x;
+    b->captures++;
+  }
+  HIPCHK(hipGraphLaunch(s.x, b->st));
+  b->graph_launches++;
+  return CFDL_OK;
+}
+
+int batch_iteration(cfdl_batch *b) {
+  hipStream_t st = b->st;
+  const bool tune = b->block == 0 && b->tuned == 0 && !b->tuning;
+  const int blk = b->block > 0 ? b->block : b->tuned > 0 ? b->tuned : kBlock;
+  // the pc solve's descriptors (outside any capture: this may synchronise)
+  const int n = static_cast<int>(b->ctx.size());
+  std::vector<cfdl::MgCgSolver *> sv(n);
+  std::vector<cfdl::MgCgSystem> sys(n);
+  for (int k = 0; k < n; k++) {
+    cfdl_ctx *c = b->ctx[k];
+    sv[k] = c->mg[b->fp32];
+    cfdl::MgCgSystem &y = sys[k];
+    y = cfdl::MgCgSystem{};
+    y.ne = c->ne;
+    y.nbf = c->nbf;
+    y.nit = c->nit;
+    y.row = c->md.row;
+    y.nbr = c->md.nbr;
+    y.ap = c->fd.ap;
+    y.anb = c->fd.anb;
+    y.b = c->fd.b;
+    y.phi = c->fd.phic;
+    std::strncpy(y.name, "pc", 7);
+    y.log = c->dlog;
+    y.nlog = c->dlog_n;
+  }
+  int changed = 0;
+  CALL(cfdl::mgcg_group_begin(b->mg, sv.data(), sys.data(), n, st, blk, &changed));
+  if (changed || blk != b->graph_blk || !b->graphs_valid) {
+    graphs_drop(b);
+    b->graph_blk = blk;
+    b->graphs_valid = true;
+  }
+  // the tuner's first iteration runs eagerly (its timed launches are not
+  // captured)
+  const bool graph = b->use_graphs && !tune;
+  CALL(run_seg(b, graph, 0, [&] { return seg_prefix(b, blk); }));
+  int nit = 0;
+  for (cfdl_ctx *c : b->ctx) nit = std::max(nit, c->nit);
+  for (int it = 0; it < nit; it++) {
+    CALL(run_seg(b, graph, 1 + (it & 1), [&] { return seg_gs(b, blk, it & 1); }));
+    HIPCHK(hipEventRecord(b->ev[it & 1], st));
+    if (it > 0) {
+      HIPCHK(hipEventSynchronize(b->ev[(it - 1) & 1]));
+      const int *pf = b->h_flags + ((it - 1) & 1) * kMaxGroup;
+      bool any = false;
+      for (int k = 0; k < n; k++) any = any || pf[k] != 0;
+      if (!any) break;
+    }
+  }
+  CALL(run_seg(b, graph, 3, [&] { return seg_middle(b, blk, tune); }));
+  const int cit = cfdl::mgcg_group_nit(b->mg);
+  for (int it = 0; it < cit; it++) {
+    CALL(run_seg(b, graph, 4 + (it & 1), [&] { return seg_cg(b, it & 1); }));
+    HIPCHK(hipEventRecord(b->ev[it & 1], st));
+    if (it > 0) {
+      HIPCHK(hipEventSynchronize(b->ev[(it - 1) & 1]));
+      if (!cfdl::mgcg_group_any_active(b->mg, (it - 1) & 1)) break;
+    }
+  }
+  CALL(run_seg(b, graph, 6, [&] { return seg_suffix(b, blk); }));
+  for (cfdl_ctx *c : b->ctx) c->outer++;
+  return CFDL_OK;
+}
+
+int batch_update_time(cfdl_batch *b) {
+  const int blk = b->block > 0 ? b->block : b->tuned > 0 ? b->tuned : kBlock;
+  const bool graph = b->use_graphs && blk == b->graph_blk && b->graphs_valid;
+  return run_seg(b, graph, 7, [&] { return seg_time(b, blk); });
+}
+
 void batch_free(cfdl_batch *b) {
+  graphs_drop(b);
   if (b->mg) cfdl::mgcg_group_destroy(b->mg);
   if (b->d_slots) (void)hipFree(b->d_slots);
   if (b->d_flags) (void)hipFree(b->d_flags);
@@ -4400,6 +4510,25 @@ int cfdl_batch_block(cfdl_batch *b, int *block) {
     return CFDL_EINVAL;
   }
   *block = b->block > 0 ? b->block : b->tuned > 0 ? b->tuned : kBlock;
+  return CFDL_OK;
+}
+
+int cfdl_batch_set_graphs(cfdl_batch *b, int on) {
+  if (!b) {
+    set_error("cfdl_batch_set_graphs: null batch");
+    return CFDL_EINVAL;
+  }
+  b->use_graphs = on != 0;
+  return CFDL_OK;
+}
+
+int cfdl_batch_graph_stats(cfdl_batch *b, int64_t *captures, int64_t *launches) {
+  if (!b) {
+    set_error("cfdl_batch_graph_stats: null batch");
+    return CFDL_EINVAL;
+  }
+  if (captures) *captures = b->captures;
+  if (launches) *launches = b->graph_launches;
   return CFDL_OK;
 }
 

@@ -106,4 +106,18 @@ int mgcg_group_create(MgGroup **out);
 void mgcg_group_destroy(MgGroup *g);
 int mgcg_solve_group(MgGroup *g, MgCgSolver *const *s, const MgCgSystem *sys, int n, void *stream,
                      int block);
+// The same solve in phases, for callers that drive the iterations
+// themselves (and may capture the enqueued phases in a graph): begin binds
+// the systems and uploads their descriptors when they changed (*changed;
+// it synchronises the stream then, so it runs outside any capture); setup,
+// iteration and finish only enqueue work on the stream.  iteration copies
+// the systems' active flags to host slot 0 or 1, which any_active reads
+// once the stream has passed that point; nit is the largest iteration cap.
+int mgcg_group_begin(MgGroup *g, MgCgSolver *const *s, const MgCgSystem *sys, int n, void *stream,
+                     int block, int *changed);
+int mgcg_group_setup(MgGroup *g, void *stream);
+int mgcg_group_iteration(MgGroup *g, void *stream, int slot);
+int mgcg_group_finish(MgGroup *g, void *stream);
+int mgcg_group_nit(const MgGroup *g);
+bool mgcg_group_any_active(const MgGroup *g, int slot);
 }  // namespace cfdl

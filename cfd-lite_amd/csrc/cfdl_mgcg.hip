@@ -1049,7 +1049,22 @@ int mgcg_solve(MgCgSolver *s, const MgCgSystem &sys, void *stream) {
 // ---------------------------------------------------------------------------
 // Grouped solves
 // ---------------------------------------------------------------------------
+namespace {
+// A grouped solve between its phases (host side): the systems, their
+// descriptors as uploaded, and the launch shape.
+template <typename T>
+struct GroupRun {
+  std::vector<MgCgSolver *> ss;
+  std::vector<MgSlot<T>> h;
+  std::vector<Lev<T>> l0;
+  int n = 0, blk = 256, maxtop = 0, maxlev = 0, nitmax = 0;
+};
+}  // namespace
+
 struct MgGroup {
+  int fp32 = 0;            // precision of the current run
+  GroupRun<double> rd;
+  GroupRun<float> rf;
   void *slots = nullptr;  // device MgSlot<T>[kMaxGroup]
   int *flags = nullptr;   // device active flags
   int *hflags = nullptr;  // pinned, two sets of kMaxGroup
@@ -1126,21 +1141,37 @@ std::vector<Lev<double>> &host_levels<double>(MgCgSolver *s) { return s->lvd; }
 template <>
 std::vector<Lev<float>> &host_levels<float>(MgCgSolver *s) { return s->lvf; }
 
-// The solve of mgcg_solve for n systems at once.  Every launch covers the
-// systems that take part in its step; each system runs exactly its own
-// sequence of steps (a system with fewer levels or colours sits out the
-// launches it has no part in), so its arithmetic is that of a solve alone.
 template <typename T>
-int solve_group(MgGroup *G, MgCgSolver *const *ss, const MgCgSystem *sys, int n, hipStream_t st,
-                int blk) {
-  std::vector<MgSlot<T>> h(n);
-  std::memset(static_cast<void *>(h.data()), 0, sizeof(MgSlot<T>) * n);  // the key compares bytes
-  std::vector<Lev<T>> l0(n);
-  int maxtop = 0, maxlev = 0, nitmax = 0;
+GroupRun<T> &run_of(MgGroup *G);
+template <>
+GroupRun<double> &run_of<double>(MgGroup *G) { return G->rd; }
+template <>
+GroupRun<float> &run_of<float>(MgGroup *G) { return G->rf; }
+
+// The solve of mgcg_solve for n systems at once, in phases: begin (level 0
+// bound, descriptors uploaded when they changed; synchronises then, so it
+// runs outside any graph capture), setup (FP32 copies, Galerkin operators,
+// initial residual), one iteration at a time (V-cycle and CG step, then the
+// systems' active flags copied to a host slot), finish (the log lines).
+// Every launch covers the systems that take part in its step; each system
+// runs exactly its own sequence of steps (a system with fewer levels or
+// colours sits out the launches it has no part in), so its arithmetic is
+// that of a solve alone.
+template <typename T>
+int group_begin(MgGroup *G, MgCgSolver *const *ss, const MgCgSystem *sys, int n, hipStream_t st,
+                int blk, int *changed) {
+  GroupRun<T> &R = run_of<T>(G);
+  R.ss.assign(ss, ss + n);
+  R.n = n;
+  R.blk = blk;
+  R.h.assign(n, MgSlot<T>{});
+  std::memset(static_cast<void *>(R.h.data()), 0, sizeof(MgSlot<T>) * n);  // the key compares bytes
+  R.l0.assign(n, Lev<T>{});
+  R.maxtop = R.maxlev = R.nitmax = 0;
   for (int k = 0; k < n; k++) {
     MgCgSolver *s = ss[k];
-    MGCALL(bind_level0<T>(s, sys[k], host_levels<T>(s), dev_levels<T>(s), st, &l0[k]));
-    MgSlot<T> &x = h[k];
+    MGCALL(bind_level0<T>(s, sys[k], host_levels<T>(s), dev_levels<T>(s), st, &R.l0[k]));
+    MgSlot<T> &x = R.h[k];
     x.R = red_of(s->ne);
     x.ne = s->ne;
     x.nit = sys[k].nit;
@@ -1165,154 +1196,178 @@ int solve_group(MgGroup *G, MgCgSolver *const *ss, const MgCgSystem *sys, int n,
     x.log = sys[k].log;
     x.nlog = sys[k].nlog;
     std::memcpy(x.tmpl.name, sys[k].name, 8);
-    maxtop = std::max(maxtop, s->lc < s->nlev ? s->lc : s->nlev - 1);
-    maxlev = std::max(maxlev, s->nlev);
-    nitmax = std::max(nitmax, sys[k].nit);
+    R.maxtop = std::max(R.maxtop, s->lc < s->nlev ? s->lc : s->nlev - 1);
+    R.maxlev = std::max(R.maxlev, s->nlev);
+    R.nitmax = std::max(R.nitmax, sys[k].nit);
   }
   // the descriptors go to the device when the composition changes
-  const char *kb = reinterpret_cast<const char *>(h.data());
+  const char *kb = reinterpret_cast<const char *>(R.h.data());
   std::vector<char> key(kb, kb + sizeof(MgSlot<T>) * n);
   key.push_back(static_cast<char>(sizeof(T)));
-  if (key != G->key) {
+  *changed = key != G->key;
+  if (*changed) {
     MGCHK(hipStreamSynchronize(st));
-    MGCHK(hipMemcpy(G->slots, h.data(), sizeof(MgSlot<T>) * n, hipMemcpyHostToDevice));
+    MGCHK(hipMemcpy(G->slots, R.h.data(), sizeof(MgSlot<T>) * n, hipMemcpyHostToDevice));
     G->key.swap(key);
   }
-  const MgSlot<T> *S = static_cast<const MgSlot<T> *>(G->slots);
+  G->launches = 0;
+  return CFDL_OK;
+}
 
+// the launch shapes of a run
+template <typename T>
+struct GroupShape {
+  GroupRun<T> &R;
   GroupMap g{};
-  std::vector<int> ids(n), cnt(n);
-  for (int k = 0; k < n; k++) ids[k] = k;
-  int launches = 0;
+  std::vector<int> ids, cnt;
+  explicit GroupShape(GroupRun<T> &r) : R(r), ids(r.n), cnt(r.n) {
+    for (int k = 0; k < R.n; k++) ids[k] = k;
+  }
   // one grouped launch: blocks_of(k) blocks for system k (0 = sits out)
-  auto map = [&](auto blocks_of) {
-    for (int k = 0; k < n; k++) cnt[k] = blocks_of(k);
-    return group_map(g, ids.data(), cnt.data(), n);
-  };
-  auto top_of = [&](int k) { return ss[k]->lc < ss[k]->nlev ? ss[k]->lc : ss[k]->nlev - 1; };
+  template <typename F>
+  int map(F blocks_of) {
+    for (int k = 0; k < R.n; k++) cnt[k] = blocks_of(k);
+    return group_map(g, ids.data(), cnt.data(), R.n);
+  }
+  int top_of(int k) const {
+    return R.ss[k]->lc < R.ss[k]->nlev ? R.ss[k]->lc : R.ss[k]->nlev - 1;
+  }
+};
+
+template <typename T>
+int group_setup(MgGroup *G, hipStream_t st) {
+  GroupRun<T> &R = run_of<T>(G);
+  GroupShape<T> sh(R);
+  const MgSlot<T> *S = static_cast<const MgSlot<T> *>(G->slots);
+  const int blk = R.blk;
+  int nbk = 0;
+  if (std::is_same<T, float>::value) {
+    nbk = sh.map([&](int k) { return nb(std::max(R.l0[k].nnz, R.ss[k]->ne), blk); });
+    if (nbk > 0) {
+      hipLaunchKernelGGL(g_mg_copy0<T>, dim3(nbk), dim3(blk), 0, st, sh.g, S);
+      G->launches++;
+    }
+  }
+  for (int l = 0; l + 1 < R.maxlev; l++) {
+    nbk = sh.map([&](int k) {
+      if (l + 1 >= R.ss[k]->nlev) return 0;
+      const Lev<T> &C = host_levels<T>(R.ss[k])[l + 1];
+      return nb(C.n + C.nnz, blk);
+    });
+    if (nbk > 0) {
+      hipLaunchKernelGGL(g_mg_galerkin<T>, dim3(nbk), dim3(blk), 0, st, sh.g, S, l);
+      G->launches++;
+    }
+  }
+  nbk = sh.map([&](int k) { return R.h[k].R.nblk; });
+  hipLaunchKernelGGL(g_cg_init<T>, dim3(nbk), dim3(kRed), 0, st, sh.g, S);
+  G->launches++;
+  MGCHK(hipGetLastError());
+  return CFDL_OK;
+}
+
+template <typename T>
+int group_iteration(MgGroup *G, hipStream_t st, int slot) {
+  GroupRun<T> &R = run_of<T>(G);
+  GroupShape<T> sh(R);
+  const MgSlot<T> *S = static_cast<const MgSlot<T> *>(G->slots);
+  const int n = R.n, blk = R.blk;
   auto colour_count = [&](int k, int l, int c) {
-    return ss[k]->cptr[l][c + 1] - ss[k]->cptr[l][c];
+    return R.ss[k]->cptr[l][c + 1] - R.ss[k]->cptr[l][c];
   };
   // colour cc of level l (l < 0: the coarsest level of systems without small
   // levels), forward or backward
   auto sweep = [&](int l, bool backward) {
     int ncmax = 0;
     for (int k = 0; k < n; k++) {
-      const int lk = l < 0 ? ss[k]->nlev - 1 : l;
-      const bool part = l < 0 ? ss[k]->lc == ss[k]->nlev : l < top_of(k);
-      if (part) ncmax = std::max(ncmax, host_levels<T>(ss[k])[lk].ncol);
+      const int lk = l < 0 ? R.ss[k]->nlev - 1 : l;
+      const bool part = l < 0 ? R.ss[k]->lc == R.ss[k]->nlev : l < sh.top_of(k);
+      if (part) ncmax = std::max(ncmax, host_levels<T>(R.ss[k])[lk].ncol);
     }
     for (int cc = 0; cc < ncmax; cc++) {
-      const int nbk = map([&](int k) {
-        const int lk = l < 0 ? ss[k]->nlev - 1 : l;
-        const bool part = l < 0 ? ss[k]->lc == ss[k]->nlev : l < top_of(k);
-        const int nc = host_levels<T>(ss[k])[lk].ncol;
+      const int nbk = sh.map([&](int k) {
+        const int lk = l < 0 ? R.ss[k]->nlev - 1 : l;
+        const bool part = l < 0 ? R.ss[k]->lc == R.ss[k]->nlev : l < sh.top_of(k);
+        const int nc = host_levels<T>(R.ss[k])[lk].ncol;
         if (!part || cc >= nc) return 0;
         const int c = backward ? nc - 1 - cc : cc;
         return nb(colour_count(k, lk, c), blk);
       });
       if (nbk > 0) {
-        hipLaunchKernelGGL(g_mg_sweep<T>, dim3(nbk), dim3(blk), 0, st, g, S, l, cc,
+        hipLaunchKernelGGL(g_mg_sweep<T>, dim3(nbk), dim3(blk), 0, st, sh.g, S, l, cc,
                            backward ? 1 : 0);
-        launches++;
+        G->launches++;
       }
     }
   };
-
   int nbk = 0;
-  if (std::is_same<T, float>::value) {
-    nbk = map([&](int k) { return nb(std::max(l0[k].nnz, ss[k]->ne), blk); });
-    if (nbk > 0) {
-      hipLaunchKernelGGL(g_mg_copy0<T>, dim3(nbk), dim3(blk), 0, st, g, S);
-      launches++;
-    }
-  }
-  for (int l = 0; l + 1 < maxlev; l++) {
-    nbk = map([&](int k) {
-      if (l + 1 >= ss[k]->nlev) return 0;
-      const Lev<T> &C = host_levels<T>(ss[k])[l + 1];
-      return nb(C.n + C.nnz, blk);
+  // V-cycle
+  for (int l = 0; l < R.maxtop; l++) {
+    sweep(l, false);
+    // one lane count for the launch: the largest of the systems taking part
+    // (0, one lane per aggregate, if any needs it); sums do not depend on it
+    int nl = 1;
+    for (int k = 0; k < n; k++)
+      if (l < sh.top_of(k))
+        nl = (nl == 0 || R.ss[k]->lanes[l] == 0) ? 0 : std::max(nl, R.ss[k]->lanes[l]);
+    nbk = sh.map([&](int k) {
+      return l < sh.top_of(k) ? nb(host_levels<T>(R.ss[k])[l + 1].n * (nl > 0 ? nl : 1), blk) : 0;
     });
     if (nbk > 0) {
-      hipLaunchKernelGGL(g_mg_galerkin<T>, dim3(nbk), dim3(blk), 0, st, g, S, l);
-      launches++;
+      launch_restrict_group<T>(nl, nbk, blk, sh.g, S, l, st);
+      G->launches++;
     }
   }
-  nbk = map([&](int k) { return h[k].R.nblk; });
-  hipLaunchKernelGGL(g_cg_init<T>, dim3(nbk), dim3(kRed), 0, st, g, S);
-  launches++;
-  MGCHK(hipGetLastError());
-
-  for (int it = 0; it < nitmax; it++) {
-    // V-cycle
-    for (int l = 0; l < maxtop; l++) {
-      sweep(l, false);
-      // one lane count for the launch: the largest of the systems taking part
-      // (0, one lane per aggregate, if any needs it); sums do not depend on it
-      int nl = 1;
-      for (int k = 0; k < n; k++)
-        if (l < top_of(k)) nl = (nl == 0 || ss[k]->lanes[l] == 0) ? 0 : std::max(nl, ss[k]->lanes[l]);
-      nbk = map([&](int k) {
-        return l < top_of(k) ? nb(host_levels<T>(ss[k])[l + 1].n * (nl > 0 ? nl : 1), blk) : 0;
-      });
-      if (nbk > 0) {
-        launch_restrict_group<T>(nl, nbk, blk, g, S, l, st);
-        launches++;
-      }
+  nbk = sh.map([&](int k) { return R.ss[k]->lc < R.ss[k]->nlev ? 1 : 0; });
+  if (nbk > 0) {
+    hipLaunchKernelGGL(g_mg_coarse<T>, dim3(nbk), dim3(kCoarseThreads), 0, st, sh.g, S);
+    G->launches++;
+  }
+  bool any_flat = false;
+  for (int k = 0; k < n; k++) any_flat = any_flat || R.ss[k]->lc == R.ss[k]->nlev;
+  if (any_flat)
+    for (int r = 0; r < kCoarseSweeps; r++) {
+      sweep(-1, false);
+      sweep(-1, true);
     }
-    nbk = map([&](int k) { return ss[k]->lc < ss[k]->nlev ? 1 : 0; });
+  for (int l = R.maxtop - 1; l >= 0; l--) {
+    nbk = sh.map([&](int k) { return l < sh.top_of(k) ? nb(host_levels<T>(R.ss[k])[l].n, blk) : 0; });
     if (nbk > 0) {
-      hipLaunchKernelGGL(g_mg_coarse<T>, dim3(nbk), dim3(kCoarseThreads), 0, st, g, S);
-      launches++;
+      hipLaunchKernelGGL(g_mg_prolong<T>, dim3(nbk), dim3(blk), 0, st, sh.g, S, l);
+      G->launches++;
     }
-    bool any_flat = false;
-    for (int k = 0; k < n; k++) any_flat = any_flat || ss[k]->lc == ss[k]->nlev;
-    if (any_flat)
-      for (int r = 0; r < kCoarseSweeps; r++) {
-        sweep(-1, false);
-        sweep(-1, true);
-      }
-    for (int l = maxtop - 1; l >= 0; l--) {
-      nbk = map([&](int k) { return l < top_of(k) ? nb(host_levels<T>(ss[k])[l].n, blk) : 0; });
-      if (nbk > 0) {
-        hipLaunchKernelGGL(g_mg_prolong<T>, dim3(nbk), dim3(blk), 0, st, g, S, l);
-        launches++;
-      }
-      sweep(l, true);
-    }
-    // CG
-    nbk = map([&](int k) { return h[k].R.nblk; });
-    hipLaunchKernelGGL(g_cg_rz<T>, dim3(nbk), dim3(kRed), 0, st, g, S);
-    const GroupMap gR = g;
-    nbk = map([&](int k) { return nb(ss[k]->ne, blk); });
-    hipLaunchKernelGGL(g_cg_p<T>, dim3(nbk), dim3(blk), 0, st, g, S);
-    hipLaunchKernelGGL(g_cg_spmv<T>, dim3(gR.b0[gR.n]), dim3(kRed), 0, st, gR, S);
-    hipLaunchKernelGGL(g_cg_update<T>, dim3(gR.b0[gR.n]), dim3(kRed), 0, st, gR, S);
-    hipLaunchKernelGGL(g_cg_flags<T>, dim3(1), dim3(kMaxGroup), 0, st, n, S, G->flags);
-    launches += 5;
-    MGCHK(hipGetLastError());
-    int *hf = G->hflags + (it & 1) * kMaxGroup;
-    MGCHK(hipMemcpyAsync(hf, G->flags, sizeof(int) * n, hipMemcpyDeviceToHost, st));
-    MGCHK(hipEventRecord(G->ev[it & 1], st));
-    if (it > 0) {
-      MGCHK(hipEventSynchronize(G->ev[(it - 1) & 1]));
-      const int *pf = G->hflags + ((it - 1) & 1) * kMaxGroup;
-      bool any = false;
-      for (int k = 0; k < n; k++) any = any || pf[k] != 0;
-      if (!any) break;
-    }
+    sweep(l, true);
   }
-  hipLaunchKernelGGL(g_cg_log<T>, dim3(1), dim3(kMaxGroup), 0, st, n, S);
-  launches++;
+  // CG
+  nbk = sh.map([&](int k) { return R.h[k].R.nblk; });
+  hipLaunchKernelGGL(g_cg_rz<T>, dim3(nbk), dim3(kRed), 0, st, sh.g, S);
+  const GroupMap gR = sh.g;
+  nbk = sh.map([&](int k) { return nb(R.ss[k]->ne, blk); });
+  hipLaunchKernelGGL(g_cg_p<T>, dim3(nbk), dim3(blk), 0, st, sh.g, S);
+  hipLaunchKernelGGL(g_cg_spmv<T>, dim3(gR.b0[gR.n]), dim3(kRed), 0, st, gR, S);
+  hipLaunchKernelGGL(g_cg_update<T>, dim3(gR.b0[gR.n]), dim3(kRed), 0, st, gR, S);
+  hipLaunchKernelGGL(g_cg_flags<T>, dim3(1), dim3(kMaxGroup), 0, st, n, S, G->flags);
+  G->launches += 5;
   MGCHK(hipGetLastError());
-  G->launches = launches;
-  for (int k = 0; k < n; k++) ss[k]->launches = launches;
+  MGCHK(hipMemcpyAsync(G->hflags + slot * kMaxGroup, G->flags, sizeof(int) * n,
+                       hipMemcpyDeviceToHost, st));
+  return CFDL_OK;
+}
+
+template <typename T>
+int group_finish(MgGroup *G, hipStream_t st) {
+  GroupRun<T> &R = run_of<T>(G);
+  const MgSlot<T> *S = static_cast<const MgSlot<T> *>(G->slots);
+  hipLaunchKernelGGL(g_cg_log<T>, dim3(1), dim3(kMaxGroup), 0, st, R.n, S);
+  G->launches++;
+  MGCHK(hipGetLastError());
+  for (MgCgSolver *s : R.ss) s->launches = G->launches;
   return CFDL_OK;
 }
 }  // namespace
 
-int mgcg_solve_group(MgGroup *G, MgCgSolver *const *s, const MgCgSystem *sys, int n,
-                     void *stream, int block) {
+int mgcg_group_begin(MgGroup *G, MgCgSolver *const *s, const MgCgSystem *sys, int n, void *stream,
+                     int block, int *changed) {
   if (!G || !s || !sys || n < 1 || n > kMaxGroup) {
     set_error("mgcg_solve_group: 1 .. " + std::to_string(kMaxGroup) + " systems");
     return CFDL_EINVAL;
@@ -1336,9 +1391,54 @@ int mgcg_solve_group(MgGroup *G, MgCgSolver *const *s, const MgCgSystem *sys, in
         return CFDL_EINVAL;
       }
   }
+  int dummy = 0;
+  if (!changed) changed = &dummy;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  return s[0]->fp32 ? solve_group<float>(G, s, sys, n, st, block)
-                    : solve_group<double>(G, s, sys, n, st, block);
+  G->fp32 = s[0]->fp32;
+  return G->fp32 ? group_begin<float>(G, s, sys, n, st, block, changed)
+                 : group_begin<double>(G, s, sys, n, st, block, changed);
+}
+
+int mgcg_group_setup(MgGroup *G, void *stream) {
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  return G->fp32 ? group_setup<float>(G, st) : group_setup<double>(G, st);
+}
+
+int mgcg_group_iteration(MgGroup *G, void *stream, int slot) {
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  return G->fp32 ? group_iteration<float>(G, st, slot) : group_iteration<double>(G, st, slot);
+}
+
+int mgcg_group_finish(MgGroup *G, void *stream) {
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  return G->fp32 ? group_finish<float>(G, st) : group_finish<double>(G, st);
+}
+
+int mgcg_group_nit(const MgGroup *G) { return G->fp32 ? G->rf.nitmax : G->rd.nitmax; }
+
+bool mgcg_group_any_active(const MgGroup *G, int slot) {
+  const int n = G->fp32 ? G->rf.n : G->rd.n;
+  const int *f = G->hflags + slot * kMaxGroup;
+  for (int k = 0; k < n; k++)
+    if (f[k]) return true;
+  return false;
+}
+
+int mgcg_solve_group(MgGroup *G, MgCgSolver *const *s, const MgCgSystem *sys, int n,
+                     void *stream, int block) {
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  MGCALL(mgcg_group_begin(G, s, sys, n, stream, block, nullptr));
+  MGCALL(mgcg_group_setup(G, stream));
+  // iteration it is queued before the host reads the flags after it-1
+  for (int it = 0; it < mgcg_group_nit(G); it++) {
+    MGCALL(mgcg_group_iteration(G, stream, it & 1));
+    MGCHK(hipEventRecord(G->ev[it & 1], st));
+    if (it > 0) {
+      MGCHK(hipEventSynchronize(G->ev[(it - 1) & 1]));
+      if (!mgcg_group_any_active(G, (it - 1) & 1)) break;
+    }
+  }
+  return mgcg_group_finish(G, stream);
 }
 
 }  // namespace cfdl

@@ -368,7 +368,14 @@ int cfdl_calc_residual_noalloc(int ne, const int *row, const int *nbr, const dou
    first.  cfdl_batch_set_block sets the lane count of the element-wise
    launches (a multiple of 64 in 64 .. 1024; 0, the default, lets the batch
    time the candidates at the first run after each change of composition and
-   keep the fastest); results do not depend on it.  Not thread-safe. */
+   keep the fastest); results do not depend on it.  Each outer iteration
+   runs as captured HIP graphs of its segments (the steps before the
+   momentum solve, one momentum-solver iteration, the steps between the
+   solves, one CG iteration, the steps after; captured once per composition
+   and lane count, replayed while the host checks the cases' stop flags
+   between solver iterations); cfdl_batch_set_graphs(b, 0) launches the
+   same work directly, with the same results, and cfdl_batch_graph_stats
+   counts captures and graph launches.  Not thread-safe. */
 typedef struct cfdl_batch cfdl_batch;
 int cfdl_batch_create(cfdl_batch **out);
 int cfdl_batch_attach(cfdl_batch *b, cfdl_ctx *c);
@@ -376,6 +383,8 @@ int cfdl_batch_detach(cfdl_batch *b, cfdl_ctx *c);
 int cfdl_batch_size(cfdl_batch *b, int *n);
 int cfdl_batch_set_block(cfdl_batch *b, int block);
 int cfdl_batch_block(cfdl_batch *b, int *block);
+int cfdl_batch_set_graphs(cfdl_batch *b, int on);
+int cfdl_batch_graph_stats(cfdl_batch *b, int64_t *captures, int64_t *launches);
 int cfdl_batch_run(cfdl_batch *b, int ntstep, int ncoef);
 int cfdl_batch_synchronize(cfdl_batch *b);
 void cfdl_batch_destroy(cfdl_batch *b);

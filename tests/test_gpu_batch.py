@@ -176,3 +176,34 @@ def test_batch_rejects_cases_it_cannot_run():
     b.run(1, 1)
     b.close()
     eq.run(1, 1)  # detached by close: runs on its own stream again
+
+
+def test_batch_graphs_equal_direct_launches_after_every_run():
+    """The captured graphs of the iteration's segments (replayed many times
+    per run, recaptured when a case joins) against the same batch launched
+    directly and against every case alone, compared after every run."""
+    specs = SPECS[:4]
+    alone = [make_case(*s) for s in specs]
+    with_graphs = [make_case(*s) for s in specs]
+    direct = [make_case(*s) for s in specs]
+    bg, bd = cfdl.Batch(), cfdl.Batch(graphs=False)
+    for k in range(3):
+        bg.attach(with_graphs[k])
+        bd.attach(direct[k])
+    for r in range(4):
+        if r == 2:
+            bg.attach(with_graphs[3])
+            bd.attach(direct[3])
+        bg.run(1, 2)
+        bd.run(1, 2)
+        for k in range(len(specs)):
+            if k < 3 or r >= 2:
+                alone[k].run(1, 2)
+        for k in range(len(specs)):
+            same(with_graphs[k], direct[k])
+            same(with_graphs[k], alone[k])
+    captures, launches = bg.graph_stats()
+    assert captures >= 8 and launches > 2 * captures
+    assert bd.graph_stats() == (0, 0)
+    bg.close()
+    bd.close()

@@ -671,8 +671,20 @@ def serving_run(cfdl, nreq, rate, tol, max_outer, chunk, max_cases, seed=2026120
     reqs = serving.make_requests(nreq, rate, seed)
     progress("serving %d requests, %g per s, tol %g or %d outer iterations"
              % (nreq, rate, tol, max_outer))
+    t0 = time.perf_counter()
+    last = [t0]
+    seen = [0]
+
+    def on_run(ncases):
+        seen[0] += 1
+        now = time.perf_counter()
+        if now - last[0] > 20.0:
+            last[0] = now
+            progress("serving: %.0f s, batch run %d, %d cases in the batch"
+                     % (now - t0, seen[0], ncases))
+
     out = serving.serve(reqs, max_cases=max_cases, tol=tol, max_outer=max_outer, chunk=chunk,
-                        warm=warm)
+                        warm=warm, on_run=on_run)
     cell_its = sum(r.n ** 3 * out["outer"][r.idx] for r in reqs)
     return {"metric": "batched solves/s with p50/p99 latency",
             "solves_per_s": round(out["solves_per_s"], 3),

@@ -123,6 +123,50 @@ def load_library(path=None):
     return lib
 
 
+_cgns = None
+
+# libcfdl_cgns.so (include/cfdl_cgns.h): CGNS mesh files over HDF5
+_CGNS_SIGS = {
+    "cfdl_cgns_last_error": (C.c_char_p, []),
+    "cfdl_mesh_read_cgns": (C.c_int, [C.c_char_p, C.POINTER(VP)]),
+    "cfdl_mesh_write_cgns": (C.c_int, [VP, C.c_char_p]),
+    "cfdl_cgns_section_name": (C.c_int, [VP, C.c_int, C.c_char_p, C.c_int]),
+}
+
+
+def cgns_exported_names():
+    return list(_CGNS_SIGS)
+
+
+def cgns_library_path():
+    return os.path.join(_PKG, "libcfdl_cgns.so")
+
+
+def load_cgns_library():
+    """Load libcfdl_cgns.so once (after libcfdl_hip.so); raise if it is not built."""
+    global _cgns
+    if _cgns is not None:
+        return _cgns
+    load_library()
+    path = cgns_library_path()
+    if not os.path.exists(path):
+        raise CfdlError("libcfdl_cgns.so not found at %s (built when HDF5 headers are present)"
+                        % path)
+    lib = C.CDLL(path)
+    for name, (res, args) in _CGNS_SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _cgns = lib
+    return lib
+
+
+def check_cgns(rc):
+    if rc != 0:
+        msg = _cgns.cfdl_cgns_last_error().decode(errors="replace")
+        raise CfdlError("cfdl cgns error %d: %s" % (rc, msg))
+
+
 def check(rc):
     if rc != 0:
         msg = _lib.cfdl_last_error().decode(errors="replace")

@@ -8,7 +8,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import PD, PI, VP, check, load_library
+from ._lib import PD, PI, VP, check, check_cgns, load_cgns_library, load_library
 
 HEXA_8, QUAD_4 = 17, 7
 
@@ -50,6 +50,28 @@ class Mesh:
             arr = (C.c_char_p * len(names))(*[n.encode() for n in names])
             check(lib.cfdl_mesh_set_section_names(h, len(names), arr))
         return cls(h, names)
+
+    @classmethod
+    def read_cgns(cls, path):
+        """The mesh of a CGNS file (first base, first zone; HDF5 storage),
+        as the reference's cell_input reads it (cfdl_mesh_read_cgns)."""
+        lib = load_cgns_library()
+        h = VP()
+        check_cgns(lib.cfdl_mesh_read_cgns(str(path).encode(), C.byref(h)))
+        ints = [C.c_int() for _ in range(6)]  # ne, nf, nbf, nsec, nsubd, nvx
+        check(load_library().cfdl_mesh_info(h, *[C.byref(i) for i in ints]))
+        names = []
+        buf = C.create_string_buffer(64)
+        for k in range(ints[3].value):
+            check_cgns(lib.cfdl_cgns_section_name(h, k, buf, 64))
+            names.append(buf.value.decode())
+        return cls(h, names)
+
+    def write_cgns(self, path):
+        """Vertices, named sections and element lists as a CGNS file
+        (cfdl_mesh_write_cgns)."""
+        lib = load_cgns_library()
+        check_cgns(lib.cfdl_mesh_write_cgns(self._h, str(path).encode()))
 
     def renumber(self, seed):
         """Seeded random cell permutation, then reverse Cuthill-McKee

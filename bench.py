@@ -611,14 +611,14 @@ def main():
     achieved = BYTES_PER_CELL_SWEEP * cells / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else 0.0
     traffic_kib = args.traffic_kib
     schedule_env = any(os.environ.get(k) for k in ("CFDL_TILE", "CFDL_TILE_SPAN", "CFDL_GS_MODE",
-                                                  "CFDL_TILE_LDS", "CFDL_TILE_PIPE"))
+                                                  "CFDL_TILE_LDS", "CFDL_TILE_PIPE", "CFDL_TILE_WAVES"))
     if traffic_kib is None and args.n == 128 and not schedule_env:
         traffic_kib = TRAFFIC_KIB_128
     traffic = None if traffic_kib is None else round(traffic_kib * 1024.0)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "algorithmic_bytes_per_launch": round(BYTES_PER_CELL_SWEEP * cells),
-                "kernel": "k_gs_tiles_lds<true, true, 1> (pressure-correction Gauss-Seidel, space-time tiles in LDS)",
+                "kernel": "k_gs_tiles_lds<true, true, 1, 4> (pressure-correction Gauss-Seidel, space-time tiles in LDS)",
                 "launch_ms": round(kernel_ms, 5), "cells_per_launch": round(cells, 1),
                 "sampled_launches": int(st["sampled_launches"])}
     hbm_whole = ne * iteration_bytes(log[-4 * args.steps:], "reference") / secs / (HBM_PEAK_GBS * 1e9)

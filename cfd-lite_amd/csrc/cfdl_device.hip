@@ -1185,19 +1185,21 @@ k_gs_tiles(SysDesc s, const int *__restrict__ bseg, const int *__restrict__ seg,
   }
 }
 
-// The same sweep with the batch resident in LDS.  One wave loads the
+// The same sweep with the batch resident in LDS.  NW wavefronts load the
 // batch's values, coefficients and local slot table (contiguous ranges
 // starting at its first cell c0 and first CSR entry r0) plus the values of
-// its outside neighbours (xslot), walks the segments on LDS only and writes
-// the batch back.  Outside neighbours are constant during the launch: a
-// cell of another tile is never updated in the same launch.  lnbr gives,
-// per CSR entry, the LDS slot of the neighbour: < nc a batch cell, else
-// outside value nc + j.  Every load of a gather pass is issued before the
-// first LDS store, so the gather costs two memory latencies (the outside
-// indices, then their values).  The segment count is at most 63 (host).
-constexpr int kLdsThreads = 64;
+// its outside neighbours (xslot); the segments are walked on LDS only (by
+// wavefront 0 alone in the pipelined walk) and the NW wavefronts write the
+// batch back.  Outside neighbours are constant during the launch: a cell of
+// another tile is never updated in the same launch.  lnbr gives, per CSR
+// entry, the LDS slot of the neighbour: < nc a batch cell, else outside
+// value nc + j.  Every load of a gather pass is issued before the first LDS
+// store, so the gather costs two memory latencies (the outside indices, then
+// their values) plus the issue of its loads, which NW wavefronts share.  The
+// segment count is at most 63 (host).
+constexpr int kLdsThreads = 64;  // one wavefront: the pipelined walk's width
 
-// Orders the LDS accesses of the one wavefront of an LDS-sweep workgroup: a
+// Orders the LDS accesses of the wavefront that walks an LDS sweep alone: a
 // wavefront's LDS instructions execute in issue order, so what one lane
 // stores is seen by every later load of any lane without s_barrier or a
 // wait for the store.  The wavefront-scope fences keep the compiler from
@@ -1208,13 +1210,14 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <bool HEX, bool PIPE, int NC>
-__global__ void __launch_bounds__(kLdsThreads)
+template <bool HEX, bool PIPE, int NC, int NW>
+__global__ void __launch_bounds__(kLdsThreads * NW)
 k_gs_tiles_lds(SysDesc s, const int4 *__restrict__ bdesc, const int *__restrict__ seg,
                const unsigned short *__restrict__ lnbr, const int *__restrict__ xslot,
                int batch0, int backward, double sor, double rsor,
                const int *__restrict__ active) {
   extern __shared__ double lds[];
+  constexpr int T = kLdsThreads * NW;  // threads of the gather and write-back
   // the batch's descriptor (segment, cell, CSR and outside-neighbour ranges)
   // comes in one load, issued with the active flags
   const int b = batch0 + blockIdx.x;
@@ -1260,32 +1263,33 @@ k_gs_tiles_lds(SysDesc s, const int4 *__restrict__ bdesc, const int *__restrict_
       reinterpret_cast<size_t>((__attribute__((address_space(3))) double *)pphi));
   const double *pa = s.ap + c0, *pr = s.rap + c0, *pn = s.anb + r0;
   const unsigned short *pl = lnbr + r0;
-  // segment bound of lane q (lanes past nq reload the last one); used after
-  // the gather, so its load overlaps the gather's
+  // segment bound of lane q of wavefront 0 (lanes past nq reload the last
+  // one); used after the gather, so its load overlaps the gather's
   const int sg_raw = seg[s0 + min(static_cast<int>(threadIdx.x), nq)];
-  // gather: each pass covers U*64 cells (values, coefficients, slots) and
-  // 512 outside neighbours; every load of a pass is issued before its LDS
+  // gather: each pass covers U*T cells (values, coefficients, slots) and
+  // X*T outside neighbours; every load of a pass is issued before its LDS
   // stores, so a pass costs two memory latencies (the outside indices, then
   // their values)
-  constexpr int U = NC == 1 ? 8 : 4;
-  const int nrc = (nc + U * kLdsThreads - 1) / (U * kLdsThreads);
-  const int nrx = (nx + 8 * kLdsThreads - 1) / (8 * kLdsThreads);
+  constexpr int U = NW == 1 ? (NC == 1 ? 8 : 4) : 2;
+  constexpr int X = NW == 1 ? 8 : 2;
+  const int nrc = (nc + U * T - 1) / (U * T);
+  const int nrx = (nx + X * T - 1) / (X * T);
   const int nr = nrc > nrx ? nrc : nrx;
   for (int r = 0; r < nr; r++) {
-    const int cb = r * U * kLdsThreads, xb = r * 8 * kLdsThreads;
-    int xs[8];
-    double va[U], vr[U], vn[U][6], vp[NC][U], vb[NC][U], vx[NC][8];
+    const int cb = r * U * T, xb = r * X * T;
+    int xs[X];
+    double va[U], vr[U], vn[U][6], vp[NC][U], vb[NC][U], vx[NC][X];
     unsigned short sl[U][6];
 #pragma unroll
-    for (int u = 0; u < 8; u++) {
-      const int j = xb + u * kLdsThreads + threadIdx.x;
+    for (int u = 0; u < X; u++) {
+      const int j = xb + u * T + threadIdx.x;
       xs[u] = -1;
-      if (xb + u * kLdsThreads < nx) xs[u] = j < nx ? xslot[x0 + j] : -1;
+      if (xb + u * T < nx) xs[u] = j < nx ? xslot[x0 + j] : -1;
     }
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      const int i = cb + u * kLdsThreads + threadIdx.x;
-      if (cb + u * kLdsThreads < nc) {  // wave-uniform: rows of this pass exist
+      const int i = cb + u * T + threadIdx.x;
+      if (cb + u * T < nc) {  // workgroup-uniform: rows of this pass exist
         const int ic = i < nc ? i : 0;  // lanes past the end reload row 0
         va[u] = pa[ic];
         vr[u] = pr[ic];
@@ -1304,12 +1308,12 @@ k_gs_tiles_lds(SysDesc s, const int4 *__restrict__ bdesc, const int *__restrict_
       }
     }
 #pragma unroll
-    for (int u = 0; u < 8; u++)
+    for (int u = 0; u < X; u++)
 #pragma unroll
       for (int c = 0; c < NC; c++) vx[c][u] = xs[u] >= 0 ? s.phi[c][xs[u]] : 0.0;
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      const int i = cb + u * kLdsThreads + threadIdx.x;
+      const int i = cb + u * T + threadIdx.x;
       if (i < nc) {
         if constexpr (PIPE) {
           pap[i] = va[u];
@@ -1343,8 +1347,8 @@ k_gs_tiles_lds(SysDesc s, const int4 *__restrict__ bdesc, const int *__restrict_
       }
     }
 #pragma unroll
-    for (int u = 0; u < 8; u++) {
-      const int j = xb + u * kLdsThreads + threadIdx.x;
+    for (int u = 0; u < X; u++) {
+      const int j = xb + u * T + threadIdx.x;
       if (j < nx)
 #pragma unroll
         for (int c = 0; c < NC; c++) {
@@ -1356,134 +1360,137 @@ k_gs_tiles_lds(SysDesc s, const int4 *__restrict__ bdesc, const int *__restrict_
     }
   }
   if (!HEX) {
-    for (int k = threadIdx.x; k < nent; k += kLdsThreads) {
+    for (int k = threadIdx.x; k < nent; k += T) {
       lanb[k] = pn[k];
       lnb[k] = pl[k];
     }
-    for (int i = threadIdx.x; i <= nc; i += kLdsThreads) lrow[i] = s.row[c0 + i] - r0;
+    for (int i = threadIdx.x; i <= nc; i += T) lrow[i] = s.row[c0 + i] - r0;
   }
   const int sg = sg_raw - c0;
   if (!PIPE && threadIdx.x <= nq) lseg[threadIdx.x] = sg;
   __syncthreads();
   if constexpr (PIPE) {
-    // Every segment has at most 64 cells: lane l updates cell sg[q] + l of
-    // segment q (bounds from lane q of sg).  Two operand sets alternate: while
-    // segment j computes from one, the other receives segment j+1's ap, 1/ap,
-    // b, anb, neighbour addresses and own old value (its cells are not
-    // written at level j); right after segment j's stores it receives its
-    // neighbour values.  The workgroup is
-    // one wavefront, whose LDS accesses execute in issue order, so those
-    // reads see the stores without waiting for them.  Lanes without a cell
-    // read row 0 and store to lds[0..3].
-    // RCP: the divisions by ap and sor use div_rcp2.  Without a branch in the
-    // walk, each lane keeps the least and greatest frexp exponent of its
-    // dividends x and a NaN detector (0*nv is NaN for an infinite or NaN
-    // result, which an out-of-range 1/ap or an overflow produces); if any
-    // lane saw a dividend outside [-kRcpExp, kRcpExp] or a non-finite
-    // result, the batch is walked again with hardware division from its
-    // original values, which global memory still holds.
-    const int lane = threadIdx.x;
-    const double sm1 = sor - 1.0;
-    struct Ops {
-      int i;
-      bool on;
-      double a, ya, bv[NC], t[NC], an[6], v[NC][6];
-      unsigned sl[6];
-    };
-    // operands and own values of segment qq (no cell when !valid)
-    auto load_ops = [&](int qq, bool valid, Ops &o) {
-      int ii = 0;
-      bool oo = false;
-      if (valid) {
-        ii = __builtin_amdgcn_readlane(sg, qq) + lane;
-        oo = ii < __builtin_amdgcn_readlane(sg, qq + 1);
-      }
-      o.i = ii;
-      o.on = oo;
-      const int ic = oo ? ii : 0;
-      o.a = pap[ic];
-      o.ya = pra[ic];
-#pragma unroll
-      for (int c = 0; c < NC; c++) {
-        o.bv[c] = pb[c * nc + ic];
-        o.t[c] = sm1 * o.a * pphi[NC * ic + c];
-      }
-#pragma unroll
-      for (int k = 0; k < 6; k++) {
-        o.an[k] = panb[6 * ic + k];
-        o.sl[k] = psl[6 * ic + k];
-      }
-    };
-    // neighbour values through LDS addresses formed from the stored byte
-    // offsets (pphi's own LDS address is added once, at the gather)
-    auto load_v = [&](Ops &o) {
-#pragma unroll
-      for (int k = 0; k < 6; k++)
-#pragma unroll
-        for (int c = 0; c < NC; c++)
-          o.v[c][k] = *reinterpret_cast<const __attribute__((address_space(3))) double *>(
-              static_cast<size_t>(o.sl[k] + 8 * c));
-    };
-    auto walk = [&](bool rcp) -> bool {
-      int emin = 0, emax = 0;
-      double nan_seen = 0.0;
-      // level j of the walk: segment q(j), then the operands of q(j + 1)
-      auto step = [&](int j, Ops &cur, Ops &nxt) {
-        const int qn = backward ? nq - 2 - j : j + 1;
-        load_ops(qn, j + 1 < nq, nxt);
-        // the operand loads go out before this level's arithmetic
-        __builtin_amdgcn_sched_barrier(0);
+    // wavefront 0 walks; the others wait at the barrier before the write-back
+    if (NW == 1 || threadIdx.x < kLdsThreads) {
+      // Every segment has at most 64 cells: lane l updates cell sg[q] + l of
+      // segment q (bounds from lane q of sg).  Two operand sets alternate: while
+      // segment j computes from one, the other receives segment j+1's ap, 1/ap,
+      // b, anb, neighbour addresses and own old value (its cells are not
+      // written at level j); right after segment j's stores it receives its
+      // neighbour values.  The walk is
+      // one wavefront's, whose LDS accesses execute in issue order, so those
+      // reads see the stores without waiting for them.  Lanes without a cell
+      // read row 0 and store to lds[0..3].
+      // RCP: the divisions by ap and sor use div_rcp2.  Without a branch in the
+      // walk, each lane keeps the least and greatest frexp exponent of its
+      // dividends x and a NaN detector (0*nv is NaN for an infinite or NaN
+      // result, which an out-of-range 1/ap or an overflow produces); if any
+      // lane saw a dividend outside [-kRcpExp, kRcpExp] or a non-finite
+      // result, the batch is walked again with hardware division from its
+      // original values, which global memory still holds.
+      const int lane = threadIdx.x;
+      const double sm1 = sor - 1.0;
+      struct Ops {
+        int i;
+        bool on;
+        double a, ya, bv[NC], t[NC], an[6], v[NC][6];
+        unsigned sl[6];
+      };
+      // operands and own values of segment qq (no cell when !valid)
+      auto load_ops = [&](int qq, bool valid, Ops &o) {
+        int ii = 0;
+        bool oo = false;
+        if (valid) {
+          ii = __builtin_amdgcn_readlane(sg, qq) + lane;
+          oo = ii < __builtin_amdgcn_readlane(sg, qq + 1);
+        }
+        o.i = ii;
+        o.on = oo;
+        const int ic = oo ? ii : 0;
+        o.a = pap[ic];
+        o.ya = pra[ic];
 #pragma unroll
         for (int c = 0; c < NC; c++) {
-          if (!act[c]) continue;
-          double sum = cur.bv[c];
-#pragma unroll
-          for (int k = 0; k < 6; k++) sum = sum + cur.an[k] * cur.v[c][k];
-          const double x = sum + cur.t[c];
-          double nv;
-          if (rcp) {
-            const int ex = __builtin_amdgcn_frexp_exp(x);  // 0 for x = 0
-            emin = min(emin, ex);
-            emax = max(emax, ex);
-            nv = div_rcp2(x, cur.a, cur.ya, sor, rsor);
-            nan_seen = fma(0.0, nv, nan_seen);
-          } else {
-            nv = x / cur.a / sor;
-          }
-          // unconditional store, so no branch separates it from the next
-          // segment's loads
-          pphi[cur.on ? NC * cur.i + c : c - 4] = nv;
+          o.bv[c] = pb[c * nc + ic];
+          o.t[c] = sm1 * o.a * pphi[NC * ic + c];
         }
-        wave_sync();  // the workgroup is one wavefront (kLdsThreads = 64)
-        load_v(nxt);  // past the last segment: row 0's values, unused
-      };
-      Ops A, B;
-      load_ops(backward ? nq - 1 : 0, true, A);
-      load_v(A);
-      int j = 0;
-      for (; j + 1 < nq; j += 2) {
-        step(j, A, B);
-        step(j + 1, B, A);
-      }
-      if (j < nq) step(j, A, B);
-      return emin < -kRcpExp || emax > kRcpExp || nan_seen != nan_seen;
-    };
-    if (rsor > 0.0) {
-      if (__builtin_amdgcn_ballot_w64(walk(true))) {
-        for (int r = threadIdx.x; r < nc; r += kLdsThreads)
 #pragma unroll
-          for (int c = 0; c < NC; c++) pphi[NC * r + c] = s.phi[c][c0 + r];
-        wave_sync();
+        for (int k = 0; k < 6; k++) {
+          o.an[k] = panb[6 * ic + k];
+          o.sl[k] = psl[6 * ic + k];
+        }
+      };
+      // neighbour values through LDS addresses formed from the stored byte
+      // offsets (pphi's own LDS address is added once, at the gather)
+      auto load_v = [&](Ops &o) {
+#pragma unroll
+        for (int k = 0; k < 6; k++)
+#pragma unroll
+          for (int c = 0; c < NC; c++)
+            o.v[c][k] = *reinterpret_cast<const __attribute__((address_space(3))) double *>(
+                static_cast<size_t>(o.sl[k] + 8 * c));
+      };
+      auto walk = [&](bool rcp) -> bool {
+        int emin = 0, emax = 0;
+        double nan_seen = 0.0;
+        // level j of the walk: segment q(j), then the operands of q(j + 1)
+        auto step = [&](int j, Ops &cur, Ops &nxt) {
+          const int qn = backward ? nq - 2 - j : j + 1;
+          load_ops(qn, j + 1 < nq, nxt);
+          // the operand loads go out before this level's arithmetic
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int c = 0; c < NC; c++) {
+            if (!act[c]) continue;
+            double sum = cur.bv[c];
+#pragma unroll
+            for (int k = 0; k < 6; k++) sum = sum + cur.an[k] * cur.v[c][k];
+            const double x = sum + cur.t[c];
+            double nv;
+            if (rcp) {
+              const int ex = __builtin_amdgcn_frexp_exp(x);  // 0 for x = 0
+              emin = min(emin, ex);
+              emax = max(emax, ex);
+              nv = div_rcp2(x, cur.a, cur.ya, sor, rsor);
+              nan_seen = fma(0.0, nv, nan_seen);
+            } else {
+              nv = x / cur.a / sor;
+            }
+            // unconditional store, so no branch separates it from the next
+            // segment's loads
+            pphi[cur.on ? NC * cur.i + c : c - 4] = nv;
+          }
+          wave_sync();  // the walk is one wavefront (kLdsThreads = 64)
+          load_v(nxt);  // past the last segment: row 0's values, unused
+        };
+        Ops A, B;
+        load_ops(backward ? nq - 1 : 0, true, A);
+        load_v(A);
+        int j = 0;
+        for (; j + 1 < nq; j += 2) {
+          step(j, A, B);
+          step(j + 1, B, A);
+        }
+        if (j < nq) step(j, A, B);
+        return emin < -kRcpExp || emax > kRcpExp || nan_seen != nan_seen;
+      };
+      if (rsor > 0.0) {
+        if (__builtin_amdgcn_ballot_w64(walk(true))) {
+          for (int r = threadIdx.x; r < nc; r += kLdsThreads)
+#pragma unroll
+            for (int c = 0; c < NC; c++) pphi[NC * r + c] = s.phi[c][c0 + r];
+          wave_sync();
+          walk(false);
+        }
+      } else {
         walk(false);
       }
-    } else {
-      walk(false);
     }
   } else {
     for (int j = 0; j < nq; j++) {
       const int q = backward ? nq - 1 - j : j;
       const int i1 = lseg[q + 1];
-      for (int i = lseg[q] + threadIdx.x; i < i1; i += kLdsThreads) {
+      for (int i = lseg[q] + threadIdx.x; i < i1; i += T) {
         const double a = lap[i];
 #pragma unroll
         for (int c = 0; c < NC; c++) {
@@ -1509,10 +1516,12 @@ k_gs_tiles_lds(SysDesc s, const int4 *__restrict__ bdesc, const int *__restrict_
       __syncthreads();
     }
   }
+  // the pipelined walk's stores were made by wavefront 0 alone
+  if constexpr (PIPE && NW > 1) __syncthreads();
 #pragma unroll
   for (int c = 0; c < NC; c++) {
     if (!act[c]) continue;
-    for (int r = threadIdx.x; r < nc; r += kLdsThreads)
+    for (int r = threadIdx.x; r < nc; r += T)
       s.phi[c][c0 + r] = PIPE ? pphi[NC * r + c] : lphi[c * np + r];
   }
 }
@@ -2025,6 +2034,7 @@ struct cfdl_ctx {
     unsigned short *lnbr = nullptr;
     bool hex = false;   // every row has six entries
     bool pipe = false;  // hex, and no level segment has more than 64 cells
+    int nw = 4;         // wavefronts per batch in the gather and write-back (1 or 4)
     int nbatch = 0, nseg = 0;
     size_t lds_bytes = 0;  // 0: batches too large for LDS, global-memory kernel
   } ts, tm;
@@ -2215,6 +2225,22 @@ static int pc_sweep32(cfdl_ctx *c, const SysDesc32 &sys, bool forward, const int
   return CFDL_OK;
 }
 
+// One launch of the tiled sweep: batches [b0, b0 + nb), NW wavefronts each.
+template <int NC, int NW>
+static void tiled_launch(cfdl_ctx *c, const cfdl_ctx::TileSched &t, const SysDesc &sys, int b0,
+                         int nb, int bwd, double sor, double rsor, const int *active) {
+  const dim3 blk(kLdsThreads * NW);
+  if (t.pipe)
+    hipLaunchKernelGGL((k_gs_tiles_lds<true, true, NC, NW>), dim3(nb), blk, t.lds_bytes, c->st,
+                       sys, t.bdesc, t.seg, t.lnbr, t.xslot, b0, bwd, sor, rsor, active);
+  else if (t.hex)
+    hipLaunchKernelGGL((k_gs_tiles_lds<true, false, NC, NW>), dim3(nb), blk, t.lds_bytes, c->st,
+                       sys, t.bdesc, t.seg, t.lnbr, t.xslot, b0, bwd, sor, rsor, active);
+  else
+    hipLaunchKernelGGL((k_gs_tiles_lds<false, false, NC, NW>), dim3(nb), blk, t.lds_bytes, c->st,
+                       sys, t.bdesc, t.seg, t.lnbr, t.xslot, b0, bwd, sor, rsor, active);
+}
+
 // One tiled sweep of a system in its tile numbering: the launches of the
 // schedule in order (forward) or in reverse (backward), NC right-hand sides.
 template <int NC>
@@ -2230,18 +2256,10 @@ static void tiled_launches(cfdl_ctx *c, const cfdl_ctx::TileSched &t, const SysD
   for (int i = 0; i < nl; i++) {
     const int m = forward ? i : nl - 1 - i;
     const int b0 = t.launch_ptr[m], nb = t.launch_ptr[m + 1] - b0;
-    if (t.pipe)
-      hipLaunchKernelGGL((k_gs_tiles_lds<true, true, NC>), dim3(nb), dim3(kLdsThreads),
-                         t.lds_bytes, c->st, sys, t.bdesc, t.seg, t.lnbr, t.xslot, b0, bwd, sor,
-                         rsor, active);
-    else if (t.hex)
-      hipLaunchKernelGGL((k_gs_tiles_lds<true, false, NC>), dim3(nb), dim3(kLdsThreads),
-                         t.lds_bytes, c->st, sys, t.bdesc, t.seg, t.lnbr, t.xslot, b0, bwd, sor,
-                         rsor, active);
+    if (t.nw == 4)
+      tiled_launch<NC, 4>(c, t, sys, b0, nb, bwd, sor, rsor, active);
     else
-      hipLaunchKernelGGL((k_gs_tiles_lds<false, false, NC>), dim3(nb), dim3(kLdsThreads),
-                         t.lds_bytes, c->st, sys, t.bdesc, t.seg, t.lnbr, t.xslot, b0, bwd, sor,
-                         rsor, active);
+      tiled_launch<NC, 1>(c, t, sys, b0, nb, bwd, sor, rsor, active);
   }
 }
 
@@ -2656,6 +2674,8 @@ static int build_tile_tables(cfdl_ctx *c, const std::vector<int> &rowp,
   if (xslot.empty()) xslot.push_back(0);
   const char *pp = std::getenv("CFDL_TILE_PIPE");
   t.pipe = hex && addr16 && seg_max <= kLdsThreads && !(pp && pp[0] == '0');
+  const char *tw = std::getenv("CFDL_TILE_WAVES");
+  t.nw = (tw && tw[0] == '1') ? 1 : 4;
   if (t.pipe) lds_max = lds_pipe_max;
   if (trace_on()) {
     std::fprintf(stderr, "cfdl: tiled sweep: %d launches, %d batches, %d segments (largest "
@@ -2670,12 +2690,18 @@ static int build_tile_tables(cfdl_ctx *c, const std::vector<int> &rowp,
   HIPCHK(hipDeviceGetAttribute(&lds_dev, hipDeviceAttributeMaxSharedMemoryPerBlock, c->dev));
   size_t lds_cap = 64 * 1024;
   if (static_cast<size_t>(lds_dev) > lds_cap && lds_max > lds_cap) {
-    const void *kern[] = {reinterpret_cast<const void *>(&k_gs_tiles_lds<true, true, 1>),
-                          reinterpret_cast<const void *>(&k_gs_tiles_lds<true, false, 1>),
-                          reinterpret_cast<const void *>(&k_gs_tiles_lds<false, false, 1>),
-                          reinterpret_cast<const void *>(&k_gs_tiles_lds<true, true, 3>),
-                          reinterpret_cast<const void *>(&k_gs_tiles_lds<true, false, 3>),
-                          reinterpret_cast<const void *>(&k_gs_tiles_lds<false, false, 3>)};
+    const void *kern[] = {reinterpret_cast<const void *>(&k_gs_tiles_lds<true, true, 1, 1>),
+                          reinterpret_cast<const void *>(&k_gs_tiles_lds<true, false, 1, 1>),
+                          reinterpret_cast<const void *>(&k_gs_tiles_lds<false, false, 1, 1>),
+                          reinterpret_cast<const void *>(&k_gs_tiles_lds<true, true, 3, 1>),
+                          reinterpret_cast<const void *>(&k_gs_tiles_lds<true, false, 3, 1>),
+                          reinterpret_cast<const void *>(&k_gs_tiles_lds<false, false, 3, 1>),
+                          reinterpret_cast<const void *>(&k_gs_tiles_lds<true, true, 1, 4>),
+                          reinterpret_cast<const void *>(&k_gs_tiles_lds<true, false, 1, 4>),
+                          reinterpret_cast<const void *>(&k_gs_tiles_lds<false, false, 1, 4>),
+                          reinterpret_cast<const void *>(&k_gs_tiles_lds<true, true, 3, 4>),
+                          reinterpret_cast<const void *>(&k_gs_tiles_lds<true, false, 3, 4>),
+                          reinterpret_cast<const void *>(&k_gs_tiles_lds<false, false, 3, 4>)};
     bool ok = true;
     for (const void *k : kern)
       ok = ok && hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, lds_dev) ==

@@ -292,24 +292,30 @@ def test_set_get_roundtrip_and_errors():
         eq.set_relaxation(0.0, 1.0)
 
 
-@pytest.mark.parametrize("tile,mode,lds,span,pipe", [
-    ("2", "tiled", "1", "16", "1"), ("3", "tiled", "1", "3", "1"), ("3", "tiled", "0", "16", "1"),
-    ("8", "tiled", "1", "16", "1"), ("8", "tiled", "1", "16", "0"), ("8", "level", "1", "16", "1"),
-    ("12", "tiled", "1", "32", "1"), ("4x2x3", "tiled", "1", "5", "1"),
-    ("16x8x4", "tiled", "1", "28", "1")])
-def test_sweep_schedules_bitwise(monkeypatch, tile, mode, lds, span, pipe):
+@pytest.mark.parametrize("tile,mode,lds,span,pipe,waves", [
+    ("2", "tiled", "1", "16", "1", "4"), ("3", "tiled", "1", "3", "1", "4"),
+    ("3", "tiled", "0", "16", "1", "4"), ("8", "tiled", "1", "16", "1", "4"),
+    ("8", "tiled", "1", "16", "0", "4"), ("8", "level", "1", "16", "1", "4"),
+    ("12", "tiled", "1", "32", "1", "4"), ("4x2x3", "tiled", "1", "5", "1", "4"),
+    ("16x8x4", "tiled", "1", "28", "1", "4"), ("8", "tiled", "1", "16", "1", "1"),
+    ("8", "tiled", "1", "16", "0", "1"), ("12", "tiled", "1", "32", "1", "1"),
+    ("16x8x4", "tiled", "1", "28", "1", "1")])
+def test_sweep_schedules_bitwise(monkeypatch, tile, mode, lds, span, pipe, waves):
     """The pressure-correction sweeps run as one launch per dependency level
     or as space-time tiles launched batch by batch (in LDS, or from global
     memory with CFDL_TILE_LDS=0).  In LDS, level segments of at most 64 cells
-    take the register-pipelined walk unless CFDL_TILE_PIPE=0.  Small tiles and
-    spans put many tile boundaries into small meshes; tile 12 with span 32
-    needs more than 64 KB of LDS per batch and has wider segments; tiles may
-    have a different edge per axis.  Every schedule gives the oracle's bits."""
+    take the register-pipelined walk unless CFDL_TILE_PIPE=0, and a batch is
+    gathered and written back by four wavefronts, or one with
+    CFDL_TILE_WAVES=1.  Small tiles and spans put many tile boundaries into
+    small meshes; tile 12 with span 32 needs more than 64 KB of LDS per batch
+    and has wider segments; tiles may have a different edge per axis.  Every
+    schedule gives the oracle's bits."""
     monkeypatch.setenv("CFDL_TILE", tile)
     monkeypatch.setenv("CFDL_GS_MODE", mode)
     monkeypatch.setenv("CFDL_TILE_LDS", lds)
     monkeypatch.setenv("CFDL_TILE_SPAN", span)
     monkeypatch.setenv("CFDL_TILE_PIPE", pipe)
+    monkeypatch.setenv("CFDL_TILE_WAVES", waves)
     eq, oc = make_pair(12, 0.15, 4)
     eq.run(1, 2)
     oc.run(1, 2)

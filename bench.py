@@ -611,7 +611,8 @@ def main():
     achieved = BYTES_PER_CELL_SWEEP * cells / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else 0.0
     traffic_kib = args.traffic_kib
     schedule_env = any(os.environ.get(k) for k in ("CFDL_TILE", "CFDL_TILE_SPAN", "CFDL_GS_MODE",
-                                                  "CFDL_TILE_LDS", "CFDL_TILE_PIPE", "CFDL_TILE_WAVES"))
+                                                  "CFDL_TILE_LDS", "CFDL_TILE_PIPE", "CFDL_TILE_WAVES",
+                                                  "CFDL_TILE_PREFETCH"))
     if traffic_kib is None and args.n == 128 and not schedule_env:
         traffic_kib = TRAFFIC_KIB_128
     traffic = None if traffic_kib is None else round(traffic_kib * 1024.0)

@@ -1198,6 +1198,7 @@ k_gs_tiles(SysDesc s, const int *__restrict__ bseg, const int *__restrict__ seg,
 // their values) plus the issue of its loads, which NW wavefronts share.  The
 // segment count is at most 63 (host).
 constexpr int kLdsThreads = 64;  // one wavefront: the pipelined walk's width
+constexpr int kXcds = 8;         // XCDs of an MI355X; each has its own L2
 
 // Orders the LDS accesses of the wavefront that walks an LDS sweep alone: a
 // wavefront's LDS instructions execute in issue order, so what one lane
@@ -1210,12 +1211,65 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Brings batch b's read-only data and its own cells' values into the L2 of
+// the XCD that runs this workgroup: one byte of every 128-byte line of its
+// descriptor-given ranges (segment bounds, ap, 1/ap, right-hand sides,
+// values, anb, slot table, outside-neighbour indices), nt threads, at most
+// kTouch lines per thread.  The loaded bytes are discarded.  A batch's own
+// cells are written only by the batch itself, so their values stay what the
+// next launch reads.
+constexpr int kTouch = 4;
+template <int NC>
+__device__ void touch_batch(const SysDesc &s, const int4 *__restrict__ bdesc,
+                            const int *__restrict__ seg, const unsigned short *__restrict__ lnbr,
+                            const int *__restrict__ xslot, int b, int t, int nt) {
+  const int4 d0 = bdesc[2 * b], d1 = bdesc[2 * b + 1];
+  constexpr int NR = 5 + 2 * NC;
+  const unsigned char *base[NR];
+  int len[NR];  // bytes
+  base[0] = reinterpret_cast<const unsigned char *>(seg + d0.x);
+  len[0] = 4 * (d0.y + 1);
+  base[1] = reinterpret_cast<const unsigned char *>(s.ap + d0.z);
+  base[2] = reinterpret_cast<const unsigned char *>(s.rap + d0.z);
+  len[1] = len[2] = 8 * d0.w;
+  base[3] = reinterpret_cast<const unsigned char *>(s.anb + d1.x);
+  len[3] = 8 * d1.y;
+  base[4] = reinterpret_cast<const unsigned char *>(lnbr + d1.x);
+  len[4] = 2 * d1.y;
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    base[5 + 2 * c] = reinterpret_cast<const unsigned char *>(s.b[c] + d0.z);
+    base[6 + 2 * c] = reinterpret_cast<const unsigned char *>(s.phi[c] + d0.z);
+    len[5 + 2 * c] = len[6 + 2 * c] = 8 * d0.w;
+  }
+  // the outside-neighbour indices come last
+  const unsigned char *xb = reinterpret_cast<const unsigned char *>(xslot + d1.z);
+  const int xl = 4 * d1.w;
+  unsigned v[kTouch];
+#pragma unroll
+  for (int k = 0; k < kTouch; k++) {
+    int line = t + k * nt;
+    const unsigned char *p = nullptr;
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      const int nl = (len[r] + 127) >> 7;
+      if (!p && line < nl) p = base[r] + min(line << 7, len[r] - 1);
+      if (!p) line -= nl;
+    }
+    if (!p && line < ((xl + 127) >> 7)) p = xb + min(line << 7, xl - 1);
+    v[k] = p ? *p : 0;
+  }
+  // the loads complete here; their bytes are not used
+#pragma unroll
+  for (int k = 0; k < kTouch; k++) asm volatile("" ::"v"(v[k]));
+}
+
 template <bool HEX, bool PIPE, int NC, int NW>
 __global__ void __launch_bounds__(kLdsThreads * NW)
 k_gs_tiles_lds(SysDesc s, const int4 *__restrict__ bdesc, const int *__restrict__ seg,
                const unsigned short *__restrict__ lnbr, const int *__restrict__ xslot,
                int batch0, int backward, double sor, double rsor,
-               const int *__restrict__ active) {
+               const int *__restrict__ active, int next0, int nb_next, int rstride) {
   extern __shared__ double lds[];
   constexpr int T = kLdsThreads * NW;  // threads of the gather and write-back
   // the batch's descriptor (segment, cell, CSR and outside-neighbour ranges)
@@ -1231,6 +1285,14 @@ k_gs_tiles_lds(SysDesc s, const int4 *__restrict__ bdesc, const int *__restrict_
     act[c] = active[c] != 0;
     any = any || act[c];
   }
+  // The descriptor's second half and the kernel arguments of the gather
+  // arrive with the first half and the flags: without these uses the
+  // compiler loads them after the branch below, one more scalar round trip
+  // before the gather's first load.
+  asm volatile("" ::"s"(d1.x), "s"(d1.y), "s"(d1.z), "s"(d1.w), "s"(s.ap), "s"(s.rap),
+               "s"(s.anb), "s"(lnbr), "s"(seg), "s"(xslot));
+#pragma unroll
+  for (int c = 0; c < NC; c++) asm volatile("" ::"s"(s.phi[c]), "s"(s.b[c]));
   // (no batch is empty: testing nc here makes both loads precede the branch)
   if (!any | (d0.w == 0)) return;
   const int s0 = d0.x, nq = d0.y, c0 = d0.z, nc = d0.w;
@@ -1485,6 +1547,13 @@ k_gs_tiles_lds(SysDesc s, const int4 *__restrict__ bdesc, const int *__restrict_
       } else {
         walk(false);
       }
+    } else {
+      // wavefronts 1.. meanwhile bring the next launch's batches that will
+      // run in this workgroup's slot (same block index modulo the XCD count,
+      // so the same XCD) into L2: blocks blockIdx.x + k * rstride
+      for (int j = blockIdx.x; j < nb_next; j += rstride)
+        touch_batch<NC>(s, bdesc, seg, lnbr, xslot, next0 + j,
+                        static_cast<int>(threadIdx.x) - kLdsThreads, T - kLdsThreads);
     }
   } else {
     for (int j = 0; j < nq; j++) {
@@ -2035,6 +2104,7 @@ struct cfdl_ctx {
     bool hex = false;   // every row has six entries
     bool pipe = false;  // hex, and no level segment has more than 64 cells
     int nw = 4;         // wavefronts per batch in the gather and write-back (1 or 4)
+    bool prefetch = true;  // idle wavefronts bring the next launch's batches into L2
     int nbatch = 0, nseg = 0;
     size_t lds_bytes = 0;  // 0: batches too large for LDS, global-memory kernel
   } ts, tm;
@@ -2225,20 +2295,29 @@ static int pc_sweep32(cfdl_ctx *c, const SysDesc32 &sys, bool forward, const int
   return CFDL_OK;
 }
 
-// One launch of the tiled sweep: batches [b0, b0 + nb), NW wavefronts each.
+// One launch of the tiled sweep: batches [b0, b0 + nb), NW wavefronts each;
+// the next launch in sweep order has nb1 batches from b1 (nb1 = 0: none, or
+// no prefetch).  Workgroups go to the XCDs round robin by block index,
+// starting at the same XCD in every launch, so block j + k * 8 * ceil(nb/8)
+// of the next launch runs on block j's XCD.
 template <int NC, int NW>
 static void tiled_launch(cfdl_ctx *c, const cfdl_ctx::TileSched &t, const SysDesc &sys, int b0,
-                         int nb, int bwd, double sor, double rsor, const int *active) {
+                         int nb, int b1, int nb1, int bwd, double sor, double rsor,
+                         const int *active) {
   const dim3 blk(kLdsThreads * NW);
+  const int rs = kXcds * ((nb + kXcds - 1) / kXcds);
   if (t.pipe)
     hipLaunchKernelGGL((k_gs_tiles_lds<true, true, NC, NW>), dim3(nb), blk, t.lds_bytes, c->st,
-                       sys, t.bdesc, t.seg, t.lnbr, t.xslot, b0, bwd, sor, rsor, active);
+                       sys, t.bdesc, t.seg, t.lnbr, t.xslot, b0, bwd, sor, rsor, active, b1, nb1,
+                       rs);
   else if (t.hex)
     hipLaunchKernelGGL((k_gs_tiles_lds<true, false, NC, NW>), dim3(nb), blk, t.lds_bytes, c->st,
-                       sys, t.bdesc, t.seg, t.lnbr, t.xslot, b0, bwd, sor, rsor, active);
+                       sys, t.bdesc, t.seg, t.lnbr, t.xslot, b0, bwd, sor, rsor, active, b1, nb1,
+                       rs);
   else
     hipLaunchKernelGGL((k_gs_tiles_lds<false, false, NC, NW>), dim3(nb), blk, t.lds_bytes, c->st,
-                       sys, t.bdesc, t.seg, t.lnbr, t.xslot, b0, bwd, sor, rsor, active);
+                       sys, t.bdesc, t.seg, t.lnbr, t.xslot, b0, bwd, sor, rsor, active, b1, nb1,
+                       rs);
 }
 
 // One tiled sweep of a system in its tile numbering: the launches of the
@@ -2256,10 +2335,14 @@ static void tiled_launches(cfdl_ctx *c, const cfdl_ctx::TileSched &t, const SysD
   for (int i = 0; i < nl; i++) {
     const int m = forward ? i : nl - 1 - i;
     const int b0 = t.launch_ptr[m], nb = t.launch_ptr[m + 1] - b0;
+    const int m1 = forward ? m + 1 : m - 1;
+    const bool has_next = t.prefetch && m1 >= 0 && m1 < nl;
+    const int b1 = has_next ? t.launch_ptr[m1] : 0;
+    const int nb1 = has_next ? t.launch_ptr[m1 + 1] - b1 : 0;
     if (t.nw == 4)
-      tiled_launch<NC, 4>(c, t, sys, b0, nb, bwd, sor, rsor, active);
+      tiled_launch<NC, 4>(c, t, sys, b0, nb, b1, nb1, bwd, sor, rsor, active);
     else
-      tiled_launch<NC, 1>(c, t, sys, b0, nb, bwd, sor, rsor, active);
+      tiled_launch<NC, 1>(c, t, sys, b0, nb, b1, nb1, bwd, sor, rsor, active);
   }
 }
 
@@ -2620,6 +2703,7 @@ static int build_tile_tables(cfdl_ctx *c, const std::vector<int> &rowp,
   std::vector<unsigned short> lnbr(rowp[ne]);
   std::unordered_map<int, int> ext;
   size_t lds_max = 0, lds_pipe_max = 0;
+  int nc_max = 0, nx_max = 0;
   int seg_max = 0;
   bool addr16 = true;
   bool fits = true, hex = true;
@@ -2667,6 +2751,8 @@ static int build_tile_tables(cfdl_ctx *c, const std::vector<int> &rowp,
     if (32 + 8 * nrhs * np > 0xffff) addr16 = false;
     lds_max = std::max(lds_max, bytes);
     lds_pipe_max = std::max(lds_pipe_max, bytes_pipe);
+    nc_max = std::max(nc_max, nc);
+    nx_max = std::max(nx_max, static_cast<int>(ext.size()));
   }
   bseg[t.nbatch] = static_cast<int>(seg.size());
   for (int m = 0; m < nl; m++) t.launch_ptr[m + 1] += t.launch_ptr[m];
@@ -2676,11 +2762,14 @@ static int build_tile_tables(cfdl_ctx *c, const std::vector<int> &rowp,
   t.pipe = hex && addr16 && seg_max <= kLdsThreads && !(pp && pp[0] == '0');
   const char *tw = std::getenv("CFDL_TILE_WAVES");
   t.nw = (tw && tw[0] == '1') ? 1 : 4;
+  const char *pf = std::getenv("CFDL_TILE_PREFETCH");
+  t.prefetch = t.nw > 1 && !(pf && pf[0] == '0');
   if (t.pipe) lds_max = lds_pipe_max;
   if (trace_on()) {
     std::fprintf(stderr, "cfdl: tiled sweep: %d launches, %d batches, %d segments (largest "
-                 "%d cells), largest batch LDS %zu bytes\n", nl, t.nbatch, t.nseg, seg_max,
-                 lds_max);
+                 "%d cells), largest batch LDS %zu bytes, at most %d cells and %d outside "
+                 "neighbours per batch\n", nl, t.nbatch, t.nseg, seg_max, lds_max, nc_max,
+                 nx_max);
   }
   const char *no_lds = std::getenv("CFDL_TILE_LDS");
   const bool lds_ok = !(no_lds && no_lds[0] == '0');

@@ -305,7 +305,8 @@ def test_sweep_schedules_bitwise(monkeypatch, tile, mode, lds, span, pipe, waves
     or as space-time tiles launched batch by batch (in LDS, or from global
     memory with CFDL_TILE_LDS=0).  In LDS, level segments of at most 64 cells
     take the register-pipelined walk unless CFDL_TILE_PIPE=0, and a batch is
-    gathered and written back by four wavefronts, or one with
+    gathered and written back by four wavefronts (whose idle three prefetch
+    the next launch's batches during the walk), or by one with
     CFDL_TILE_WAVES=1.  Small tiles and spans put many tile boundaries into
     small meshes; tile 12 with span 32 needs more than 64 KB of LDS per batch
     and has wider segments; tiles may have a different edge per axis.  Every

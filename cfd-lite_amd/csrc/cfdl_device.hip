@@ -1331,9 +1331,10 @@ k_gs_tiles_lds(SysDesc s, const int4 *__restrict__ bdesc, const int *__restrict_
   // gather: each pass covers U*T cells (values, coefficients, slots) and
   // X*T outside neighbours; every load of a pass is issued before its LDS
   // stores, so a pass costs two memory latencies (the outside indices, then
-  // their values)
-  constexpr int U = NW == 1 ? (NC == 1 ? 8 : 4) : 2;
-  constexpr int X = NW == 1 ? 8 : 2;
+  // their values).  With four wavefronts one pass takes 768 cells and 768
+  // outside neighbours, more than the 16 x 8 x 4 tiles' batches have.
+  constexpr int U = NW == 1 ? (NC == 1 ? 8 : 4) : 3;
+  constexpr int X = NW == 1 ? 8 : 3;
   const int nrc = (nc + U * T - 1) / (U * T);
   const int nrx = (nx + X * T - 1) / (X * T);
   const int nr = nrc > nrx ? nrc : nrx;

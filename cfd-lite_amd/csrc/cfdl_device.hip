@@ -1502,6 +1502,7 @@ k_gs_tiles_lds(SysDesc s, const int4 *__restrict__ bdesc, const int *__restrict_
           load_ops(qn, j + 1 < nq, nxt);
           // the operand loads go out before this level's arithmetic
           __builtin_amdgcn_sched_barrier(0);
+          double xg[NC], ng[NC];
 #pragma unroll
           for (int c = 0; c < NC; c++) {
             if (!act[c]) continue;
@@ -1510,21 +1511,31 @@ k_gs_tiles_lds(SysDesc s, const int4 *__restrict__ bdesc, const int *__restrict_
             for (int k = 0; k < 6; k++) sum = sum + cur.an[k] * cur.v[c][k];
             const double x = sum + cur.t[c];
             double nv;
-            if (rcp) {
-              const int ex = __builtin_amdgcn_frexp_exp(x);  // 0 for x = 0
-              emin = min(emin, ex);
-              emax = max(emax, ex);
+            if (rcp)
               nv = div_rcp2(x, cur.a, cur.ya, sor, rsor);
-              nan_seen = fma(0.0, nv, nan_seen);
-            } else {
+            else
               nv = x / cur.a / sor;
-            }
+            xg[c] = x;
+            ng[c] = nv;
             // unconditional store, so no branch separates it from the next
             // segment's loads
             pphi[cur.on ? NC * cur.i + c : c - 4] = nv;
           }
           wave_sync();  // the walk is one wavefront (kLdsThreads = 64)
           load_v(nxt);  // past the last segment: row 0's values, unused
+          // the guard's bookkeeping waits with the next level's values, off
+          // the chain from this level's dividend to its store
+          __builtin_amdgcn_sched_barrier(0);
+          if (rcp) {
+#pragma unroll
+            for (int c = 0; c < NC; c++) {
+              if (!act[c]) continue;
+              const int ex = __builtin_amdgcn_frexp_exp(xg[c]);  // 0 for x = 0
+              emin = min(emin, ex);
+              emax = max(emax, ex);
+              nan_seen = fma(0.0, ng[c], nan_seen);
+            }
+          }
         };
         Ops A, B;
         load_ops(backward ? nq - 1 : 0, true, A);

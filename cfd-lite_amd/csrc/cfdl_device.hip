@@ -1219,8 +1219,19 @@ __device__ inline void wave_sync() {
 // cells are written only by the batch itself, so their values stay what the
 // next launch reads.
 constexpr int kTouch = 4;
+
+// The arrays a tiled sweep reads and writes, as a compact kernel argument:
+// the argument block is a cost of each of the sweep's thousands of launches.
 template <int NC>
-__device__ void touch_batch(const SysDesc &s, const int4 *__restrict__ bdesc,
+struct TileSys {
+  double *phi[NC];
+  const double *b[NC];
+  const double *ap, *anb, *rap;
+  const int *row;
+};
+
+template <int NC>
+__device__ void touch_batch(const TileSys<NC> &s, const int4 *__restrict__ bdesc,
                             const int *__restrict__ seg, const unsigned short *__restrict__ lnbr,
                             const int *__restrict__ xslot, int b, int t, int nt) {
   const int4 d0 = bdesc[2 * b], d1 = bdesc[2 * b + 1];
@@ -1266,7 +1277,7 @@ __device__ void touch_batch(const SysDesc &s, const int4 *__restrict__ bdesc,
 
 template <bool HEX, bool PIPE, int NC, int NW>
 __global__ void __launch_bounds__(kLdsThreads * NW)
-k_gs_tiles_lds(SysDesc s, const int4 *__restrict__ bdesc, const int *__restrict__ seg,
+k_gs_tiles_lds(TileSys<NC> s, const int4 *__restrict__ bdesc, const int *__restrict__ seg,
                const unsigned short *__restrict__ lnbr, const int *__restrict__ xslot,
                int batch0, int backward, double sor, double rsor,
                const int *__restrict__ active, int next0, int nb_next, int rstride) {
@@ -2318,17 +2329,26 @@ static void tiled_launch(cfdl_ctx *c, const cfdl_ctx::TileSched &t, const SysDes
                          const int *active) {
   const dim3 blk(kLdsThreads * NW);
   const int rs = kXcds * ((nb + kXcds - 1) / kXcds);
+  TileSys<NC> ts;
+  for (int c = 0; c < NC; c++) {
+    ts.phi[c] = sys.phi[c];
+    ts.b[c] = sys.b[c];
+  }
+  ts.ap = sys.ap;
+  ts.anb = sys.anb;
+  ts.rap = sys.rap;
+  ts.row = sys.row;
   if (t.pipe)
     hipLaunchKernelGGL((k_gs_tiles_lds<true, true, NC, NW>), dim3(nb), blk, t.lds_bytes, c->st,
-                       sys, t.bdesc, t.seg, t.lnbr, t.xslot, b0, bwd, sor, rsor, active, b1, nb1,
+                       ts, t.bdesc, t.seg, t.lnbr, t.xslot, b0, bwd, sor, rsor, active, b1, nb1,
                        rs);
   else if (t.hex)
     hipLaunchKernelGGL((k_gs_tiles_lds<true, false, NC, NW>), dim3(nb), blk, t.lds_bytes, c->st,
-                       sys, t.bdesc, t.seg, t.lnbr, t.xslot, b0, bwd, sor, rsor, active, b1, nb1,
+                       ts, t.bdesc, t.seg, t.lnbr, t.xslot, b0, bwd, sor, rsor, active, b1, nb1,
                        rs);
   else
     hipLaunchKernelGGL((k_gs_tiles_lds<false, false, NC, NW>), dim3(nb), blk, t.lds_bytes, c->st,
-                       sys, t.bdesc, t.seg, t.lnbr, t.xslot, b0, bwd, sor, rsor, active, b1, nb1,
+                       ts, t.bdesc, t.seg, t.lnbr, t.xslot, b0, bwd, sor, rsor, active, b1, nb1,
                        rs);
 }
 

@@ -1222,6 +1222,9 @@ constexpr int kTouch = 4;
 
 // The arrays a tiled sweep reads and writes, as a compact kernel argument:
 // the argument block is a cost of each of the sweep's thousands of launches.
+// The kernel's first three arguments (batch0, bdesc, active: all its first
+// loads need) are scalars and pointers, which the build preloads into
+// scalar registers at wave launch (-amdgpu-kernarg-preload-count=3).
 template <int NC>
 struct TileSys {
   double *phi[NC];
@@ -1277,16 +1280,17 @@ __device__ void touch_batch(const TileSys<NC> &s, const int4 *__restrict__ bdesc
 
 template <bool HEX, bool PIPE, int NC, int NW>
 __global__ void __launch_bounds__(kLdsThreads * NW)
-k_gs_tiles_lds(TileSys<NC> s, const int4 *__restrict__ bdesc, const int *__restrict__ seg,
+k_gs_tiles_lds(int batch0, const int4 *__restrict__ bdesc, const int *__restrict__ active,
+               TileSys<NC> s, const int *__restrict__ seg,
                const unsigned short *__restrict__ lnbr, const int *__restrict__ xslot,
-               int batch0, int backward, double sor, double rsor,
-               const int *__restrict__ active, int next0, int nb_next, int rstride) {
+               int backward, double sor, double rsor, int next0, int nb_next, int rstride) {
   extern __shared__ double lds[];
   constexpr int T = kLdsThreads * NW;  // threads of the gather and write-back
   // the batch's descriptor (segment, cell, CSR and outside-neighbour ranges)
   // comes in one load, issued with the active flags
   const int b = batch0 + blockIdx.x;
   const int4 d0 = bdesc[2 * b], d1 = bdesc[2 * b + 1];
+  __builtin_amdgcn_sched_barrier(0);  // the descriptor load goes out first
   // NC right-hand sides share ap/anb (u, v, w of a momentum solve); a
   // right-hand side whose solve has stopped is neither updated nor written
   bool act[NC];
@@ -2340,15 +2344,15 @@ static void tiled_launch(cfdl_ctx *c, const cfdl_ctx::TileSched &t, const SysDes
   ts.row = sys.row;
   if (t.pipe)
     hipLaunchKernelGGL((k_gs_tiles_lds<true, true, NC, NW>), dim3(nb), blk, t.lds_bytes, c->st,
-                       ts, t.bdesc, t.seg, t.lnbr, t.xslot, b0, bwd, sor, rsor, active, b1, nb1,
+                       b0, t.bdesc, active, ts, t.seg, t.lnbr, t.xslot, bwd, sor, rsor, b1, nb1,
                        rs);
   else if (t.hex)
     hipLaunchKernelGGL((k_gs_tiles_lds<true, false, NC, NW>), dim3(nb), blk, t.lds_bytes, c->st,
-                       ts, t.bdesc, t.seg, t.lnbr, t.xslot, b0, bwd, sor, rsor, active, b1, nb1,
+                       b0, t.bdesc, active, ts, t.seg, t.lnbr, t.xslot, bwd, sor, rsor, b1, nb1,
                        rs);
   else
     hipLaunchKernelGGL((k_gs_tiles_lds<false, false, NC, NW>), dim3(nb), blk, t.lds_bytes, c->st,
-                       ts, t.bdesc, t.seg, t.lnbr, t.xslot, b0, bwd, sor, rsor, active, b1, nb1,
+                       b0, t.bdesc, active, ts, t.seg, t.lnbr, t.xslot, bwd, sor, rsor, b1, nb1,
                        rs);
 }
 

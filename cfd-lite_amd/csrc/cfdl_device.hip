@@ -1193,12 +1193,18 @@ k_gs_tiles(SysDesc s, const int *__restrict__ bseg, const int *__restrict__ seg,
 // batch back.  Outside neighbours are constant during the launch: a cell of
 // another tile is never updated in the same launch.  lnbr gives, per CSR
 // entry, the LDS slot of the neighbour: < nc a batch cell, else outside
-// value nc + j.  Every load of a gather pass is issued before the first LDS
+// value nc + j; for the pipelined walk the host stores the LDS byte address
+// of that slot's values instead (kPipeBase + 8*NC*slot).  Every load of a
+// gather pass is issued before the first LDS
 // store, so the gather costs two memory latencies (the outside indices, then
 // their values) plus the issue of its loads, which NW wavefronts share.  The
 // segment count is at most 63 (host).
 constexpr int kLdsThreads = 64;  // one wavefront: the pipelined walk's width
 constexpr int kXcds = 8;         // XCDs of an MI355X; each has its own L2
+// LDS byte address of the pipelined walk's first value: the kernel has no
+// static LDS, so its dynamic block starts at 0, and lds[0..3] take the
+// stores of lanes without a cell
+constexpr unsigned kPipeBase = 32;
 
 // Orders the LDS accesses of the wavefront that walks an LDS sweep alone: a
 // wavefront's LDS instructions execute in issue order, so what one lane
@@ -1327,17 +1333,14 @@ k_gs_tiles_lds(int batch0, const int4 *__restrict__ bdesc, const int *__restrict
   // without a cell; values are interleaved by component (slot s, component
   // c at pphi[NC*s + c]); per cell ap, 1/ap and b (lane-contiguous arrays),
   // its six anb, and the LDS byte addresses of its six neighbours' values
-  // as 16-bit numbers.  The kernel has no static LDS, so the dynamic block
-  // starts at address 0, pphi at 32, and the host keeps 32 + 8*NC*np within
-  // 16 bits for the batches of this walk.
-  double *pphi = lds + 4;
+  // as 16-bit numbers, which the host computes (pphi is at kPipeBase, and
+  // the host keeps kPipeBase + 8*NC*np within 16 bits for this walk).
+  double *pphi = lds + kPipeBase / 8;
   double *pap = pphi + NC * np;
   double *pra = pap + nc;
   double *pb = pra + nc;            // NC x nc
   double *panb = pb + NC * nc;      // 6 per cell
   unsigned short *psl = reinterpret_cast<unsigned short *>(panb + 6 * nc);  // 6 per cell
-  const unsigned lds_base = static_cast<unsigned>(
-      reinterpret_cast<size_t>((__attribute__((address_space(3))) double *)pphi));
   const double *pa = s.ap + c0, *pr = s.rap + c0, *pn = s.anb + r0;
   const unsigned short *pl = lnbr + r0;
   // segment bound of lane q of wavefront 0 (lanes past nq reload the last
@@ -1348,47 +1351,58 @@ k_gs_tiles_lds(int batch0, const int4 *__restrict__ bdesc, const int *__restrict
   // stores, so a pass costs two memory latencies (the outside indices, then
   // their values).  With four wavefronts one pass takes 768 cells and 768
   // outside neighbours, more than the 16 x 8 x 4 tiles' batches have.
+  // Loads under a branch make the compiler's wait for a later use count
+  // conservatively, so: the index and outside-value loads are unconditional
+  // (lanes past the end reload an entry in range); the rows of the pass's
+  // later thirds (when they exist) go out before the indices and the first
+  // third's rows after them, so the wait for the indices leaves exactly the
+  // first third's loads in flight (loads return in issue order).
   constexpr int U = NW == 1 ? (NC == 1 ? 8 : 4) : 3;
   constexpr int X = NW == 1 ? 8 : 3;
   const int nrc = (nc + U * T - 1) / (U * T);
   const int nrx = (nx + X * T - 1) / (X * T);
   const int nr = nrc > nrx ? nrc : nrx;
-  for (int r = 0; r < nr; r++) {
+  auto gather = [&](int r) {
     const int cb = r * U * T, xb = r * X * T;
     int xs[X];
     double va[U], vr[U], vn[U][6], vp[NC][U], vb[NC][U], vx[NC][X];
     unsigned short sl[U][6];
-#pragma unroll
-    for (int u = 0; u < X; u++) {
-      const int j = xb + u * T + threadIdx.x;
-      xs[u] = -1;
-      if (xb + u * T < nx) xs[u] = j < nx ? xslot[x0 + j] : -1;
-    }
-#pragma unroll
-    for (int u = 0; u < U; u++) {
+    auto load_rows = [&](int u) {
       const int i = cb + u * T + threadIdx.x;
-      if (cb + u * T < nc) {  // workgroup-uniform: rows of this pass exist
-        const int ic = i < nc ? i : 0;  // lanes past the end reload row 0
-        va[u] = pa[ic];
-        vr[u] = pr[ic];
+      const int ic = i < nc ? i : 0;  // lanes past the end reload row 0
+      va[u] = pa[ic];
+      vr[u] = pr[ic];
 #pragma unroll
-        for (int c = 0; c < NC; c++) {
-          vp[c][u] = s.phi[c][c0 + ic];
-          vb[c][u] = s.b[c][c0 + ic];
-        }
-        if (HEX) {
+      for (int c = 0; c < NC; c++) {
+        vp[c][u] = s.phi[c][c0 + ic];
+        vb[c][u] = s.b[c][c0 + ic];
+      }
+      if (HEX) {
 #pragma unroll
-          for (int k = 0; k < 6; k++) {
-            vn[u][k] = pn[6 * ic + k];
-            sl[u][k] = pl[6 * ic + k];
-          }
+        for (int k = 0; k < 6; k++) {
+          vn[u][k] = pn[6 * ic + k];
+          sl[u][k] = pl[6 * ic + k];
         }
       }
+      // keeps the compiler from moving the loads into the stores' branch,
+      // after the outside-value loads and their wait
+      asm volatile("" ::: "memory");
+    };
+#pragma unroll
+    for (int u = U - 1; u > 0; u--)
+      if (cb + u * T < nc) load_rows(u);  // workgroup-uniform: rows of this pass exist
+#pragma unroll
+    for (int u = 0; u < X; u++) {
+      const int j = min(xb + u * T + static_cast<int>(threadIdx.x), nx - 1);
+      xs[u] = xslot[max(x0 + j, 0)];
     }
+    __builtin_amdgcn_sched_barrier(0);
+    if (r == 0 || cb < nc) load_rows(0);  // the first pass always has rows
+    __builtin_amdgcn_sched_barrier(0);   // the indices' first use comes after all row loads
 #pragma unroll
     for (int u = 0; u < X; u++)
 #pragma unroll
-      for (int c = 0; c < NC; c++) vx[c][u] = xs[u] >= 0 ? s.phi[c][xs[u]] : 0.0;
+      for (int c = 0; c < NC; c++) vx[c][u] = s.phi[c][static_cast<unsigned>(xs[u])];
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int i = cb + u * T + threadIdx.x;
@@ -1404,7 +1418,7 @@ k_gs_tiles_lds(int batch0, const int4 *__restrict__ bdesc, const int *__restrict
 #pragma unroll
           for (int k = 0; k < 6; k++) {
             panb[6 * i + k] = vn[u][k];
-            psl[6 * i + k] = static_cast<unsigned short>(lds_base + 8u * NC * sl[u][k]);
+            psl[6 * i + k] = sl[u][k];  // LDS byte address (host)
           }
         } else {
           lap[i] = va[u];
@@ -1436,7 +1450,13 @@ k_gs_tiles_lds(int batch0, const int4 *__restrict__ bdesc, const int *__restrict
             lphi[c * np + nc + j] = vx[c][u];
         }
     }
-  }
+  };
+  // The first pass, which is the only one for batches of up to U*T cells and
+  // X*T outside neighbours, is outside the loop: in the loop the compiler
+  // waits for the previous pass's loads before reusing their registers, which
+  // serialises the index loads and the coefficient loads of the first pass.
+  gather(0);
+  for (int r = 1; r < nr; r++) gather(r);
   if (!HEX) {
     for (int k = threadIdx.x; k < nent; k += T) {
       lanb[k] = pn[k];
@@ -2127,7 +2147,10 @@ struct cfdl_ctx {
     // per batch: {first segment, segments, first cell, cells, first CSR entry,
     // entries, first outside neighbour, outside neighbours}
     int4 *bdesc = nullptr;
-    unsigned short *lnbr = nullptr;
+    // per CSR entry: the neighbour's LDS slot, or with pipe its values' LDS
+    // byte address for the table's right-hand-side count (lnbr) and for one
+    // right-hand side (lnbr1, tables of several: the energy solve_gs)
+    unsigned short *lnbr = nullptr, *lnbr1 = nullptr;
     bool hex = false;   // every row has six entries
     bool pipe = false;  // hex, and no level segment has more than 64 cells
     int nw = 4;         // wavefronts per batch in the gather and write-back (1 or 4)
@@ -2342,17 +2365,18 @@ static void tiled_launch(cfdl_ctx *c, const cfdl_ctx::TileSched &t, const SysDes
   ts.anb = sys.anb;
   ts.rap = sys.rap;
   ts.row = sys.row;
+  const unsigned short *ln = (NC == 1 && t.lnbr1) ? t.lnbr1 : t.lnbr;
   if (t.pipe)
     hipLaunchKernelGGL((k_gs_tiles_lds<true, true, NC, NW>), dim3(nb), blk, t.lds_bytes, c->st,
-                       b0, t.bdesc, active, ts, t.seg, t.lnbr, t.xslot, bwd, sor, rsor, b1, nb1,
+                       b0, t.bdesc, active, ts, t.seg, ln, t.xslot, bwd, sor, rsor, b1, nb1,
                        rs);
   else if (t.hex)
     hipLaunchKernelGGL((k_gs_tiles_lds<true, false, NC, NW>), dim3(nb), blk, t.lds_bytes, c->st,
-                       b0, t.bdesc, active, ts, t.seg, t.lnbr, t.xslot, bwd, sor, rsor, b1, nb1,
+                       b0, t.bdesc, active, ts, t.seg, ln, t.xslot, bwd, sor, rsor, b1, nb1,
                        rs);
   else
     hipLaunchKernelGGL((k_gs_tiles_lds<false, false, NC, NW>), dim3(nb), blk, t.lds_bytes, c->st,
-                       b0, t.bdesc, active, ts, t.seg, t.lnbr, t.xslot, bwd, sor, rsor, b1, nb1,
+                       b0, t.bdesc, active, ts, t.seg, ln, t.xslot, bwd, sor, rsor, b1, nb1,
                        rs);
 }
 
@@ -2784,7 +2808,7 @@ static int build_tile_tables(cfdl_ctx *c, const std::vector<int> &rowp,
     const size_t bytes_pipe = 8 * (4 + nrhs * np + (2 + nrhs + 6) * static_cast<size_t>(nc)) +
                               2 * 6 * static_cast<size_t>(nc);
     // the pipelined walk addresses values with 16-bit LDS byte addresses
-    if (32 + 8 * nrhs * np > 0xffff) addr16 = false;
+    if (kPipeBase + 8 * nrhs * np > 0xffff) addr16 = false;
     lds_max = std::max(lds_max, bytes);
     lds_pipe_max = std::max(lds_pipe_max, bytes_pipe);
     nc_max = std::max(nc_max, nc);
@@ -2796,6 +2820,16 @@ static int build_tile_tables(cfdl_ctx *c, const std::vector<int> &rowp,
   if (xslot.empty()) xslot.push_back(0);
   const char *pp = std::getenv("CFDL_TILE_PIPE");
   t.pipe = hex && addr16 && seg_max <= kLdsThreads && !(pp && pp[0] == '0');
+  // the pipelined walk reads its neighbours' values at these LDS addresses
+  if (t.pipe) {
+    if (nrhs != 1) {
+      std::vector<unsigned short> a1(lnbr.size());
+      for (size_t k = 0; k < lnbr.size(); k++)
+        a1[k] = static_cast<unsigned short>(kPipeBase + 8u * lnbr[k]);
+      CALL(upload(c, &t.lnbr1, a1));
+    }
+    for (auto &v : lnbr) v = static_cast<unsigned short>(kPipeBase + 8u * nrhs * v);
+  }
   const char *tw = std::getenv("CFDL_TILE_WAVES");
   t.nw = (tw && tw[0] == '1') ? 1 : 4;
   const char *pf = std::getenv("CFDL_TILE_PREFETCH");

@@ -1488,11 +1488,15 @@ k_gs_tiles_lds(int batch0, const int4 *__restrict__ bdesc, const int *__restrict
       // original values, which global memory still holds.
       const int lane = threadIdx.x;
       const double sm1 = sor - 1.0;
+      // Nothing of segment j+1's operands is used before segment j's store:
+      // the compiler would wait there for those loads (an LDS round trip on
+      // the chain), so the own values stay raw (p) and the six 16-bit
+      // addresses stay packed in three words (w) until they are needed.
       struct Ops {
         int i;
         bool on;
-        double a, ya, bv[NC], t[NC], an[6], v[NC][6];
-        unsigned sl[6];
+        double a, ya, bv[NC], p[NC], an[6], v[NC][6];
+        unsigned w[3];
       };
       // operands and own values of segment qq (no cell when !valid)
       auto load_ops = [&](int qq, bool valid, Ops &o) {
@@ -1510,23 +1514,29 @@ k_gs_tiles_lds(int batch0, const int4 *__restrict__ bdesc, const int *__restrict
 #pragma unroll
         for (int c = 0; c < NC; c++) {
           o.bv[c] = pb[c * nc + ic];
-          o.t[c] = sm1 * o.a * pphi[NC * ic + c];
+          o.p[c] = pphi[NC * ic + c];
         }
 #pragma unroll
-        for (int k = 0; k < 6; k++) {
-          o.an[k] = panb[6 * ic + k];
-          o.sl[k] = psl[6 * ic + k];
-        }
+        for (int k = 0; k < 6; k++) o.an[k] = panb[6 * ic + k];
+        unsigned short h[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) h[k] = psl[6 * ic + k];
+#pragma unroll
+        for (int k = 0; k < 3; k++) o.w[k] = h[2 * k] | static_cast<unsigned>(h[2 * k + 1]) << 16;
       };
       // neighbour values through LDS addresses formed from the stored byte
       // offsets (pphi's own LDS address is added once, at the gather)
       auto load_v = [&](Ops &o) {
+        // the empty asm keeps the unpacking here, after the store
+        asm volatile("" : "+v"(o.w[0]), "+v"(o.w[1]), "+v"(o.w[2]));
 #pragma unroll
-        for (int k = 0; k < 6; k++)
+        for (int k = 0; k < 6; k++) {
+          const unsigned sl = k & 1 ? o.w[k >> 1] >> 16 : o.w[k >> 1] & 0xffffu;
 #pragma unroll
           for (int c = 0; c < NC; c++)
             o.v[c][k] = *reinterpret_cast<const __attribute__((address_space(3))) double *>(
-                static_cast<size_t>(o.sl[k] + 8 * c));
+                static_cast<size_t>(sl + 8 * c));
+        }
       };
       auto walk = [&](bool rcp) -> bool {
         int emin = 0, emax = 0;
@@ -1544,7 +1554,7 @@ k_gs_tiles_lds(int batch0, const int4 *__restrict__ bdesc, const int *__restrict
             double sum = cur.bv[c];
 #pragma unroll
             for (int k = 0; k < 6; k++) sum = sum + cur.an[k] * cur.v[c][k];
-            const double x = sum + cur.t[c];
+            const double x = sum + sm1 * cur.a * cur.p[c];
             double nv;
             if (rcp)
               nv = div_rcp2(x, cur.a, cur.ya, sor, rsor);

@@ -1644,11 +1644,15 @@ k_gs_tiles_lds(int batch0, const int4 *__restrict__ bdesc, const int *__restrict
   }
   // the pipelined walk's stores were made by wavefront 0 alone
   if constexpr (PIPE && NW > 1) __syncthreads();
+  // System-scope stores write through the XCD's L2, so the end of the
+  // kernel has none of these lines to write back before the next launch,
+  // whose batches on other XCDs read them (about 1% of the sweep's time).
 #pragma unroll
   for (int c = 0; c < NC; c++) {
     if (!act[c]) continue;
     for (int r = threadIdx.x; r < nc; r += T)
-      s.phi[c][c0 + r] = PIPE ? pphi[NC * r + c] : lphi[c * np + r];
+      __hip_atomic_store(&s.phi[c][c0 + r], PIPE ? pphi[NC * r + c] : lphi[c * np + r],
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 

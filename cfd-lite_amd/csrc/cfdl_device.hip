@@ -1488,27 +1488,31 @@ k_gs_tiles_lds(int batch0, const int4 *__restrict__ bdesc, const int *__restrict
       // original values, which global memory still holds.
       const int lane = threadIdx.x;
       const double sm1 = sor - 1.0;
-      // Nothing of segment j+1's operands is used before segment j's store:
-      // the compiler would wait there for those loads (an LDS round trip on
-      // the chain), so the own values stay raw (p) and the six 16-bit
-      // addresses stay packed in three words (w) until they are needed.
+      // lane j: the j-th segment in visit order (segment nq-1-j backward) as
+      // start | end << 16 (bounds below 2^13 for this walk), one read per level
+      const int qv = backward ? max(nq - 1 - lane, 0) : min(lane, nq - 1);
+      const unsigned seg_pk = static_cast<unsigned>(__shfl(sg, qv)) |
+                              static_cast<unsigned>(__shfl(sg, qv + 1)) << 16;
+      // The next segment's neighbour addresses are read first and unpacked
+      // before this level's arithmetic (they are next in the LDS queue after
+      // this level's values); its other operands are used only after this
+      // level's store, so the own values stay raw (p).
       struct Ops {
         int i;
         bool on;
         double a, ya, bv[NC], p[NC], an[6], v[NC][6];
-        unsigned w[3];
+        unsigned sl[6];
       };
-      // operands and own values of segment qq (no cell when !valid)
-      auto load_ops = [&](int qq, bool valid, Ops &o) {
-        int ii = 0;
-        bool oo = false;
-        if (valid) {
-          ii = __builtin_amdgcn_readlane(sg, qq) + lane;
-          oo = ii < __builtin_amdgcn_readlane(sg, qq + 1);
-        }
+      // operands and own values of the jj-th segment in visit order
+      auto load_ops = [&](int jj, Ops &o) {
+        const unsigned pk = __builtin_amdgcn_readlane(seg_pk, jj);
+        const int ii = static_cast<int>(pk & 0xffffu) + lane;
+        const bool oo = ii < static_cast<int>(pk >> 16);
         o.i = ii;
         o.on = oo;
         const int ic = oo ? ii : 0;
+#pragma unroll
+        for (int k = 0; k < 6; k++) o.sl[k] = psl[6 * ic + k];
         o.a = pap[ic];
         o.ya = pra[ic];
 #pragma unroll
@@ -1518,33 +1522,24 @@ k_gs_tiles_lds(int batch0, const int4 *__restrict__ bdesc, const int *__restrict
         }
 #pragma unroll
         for (int k = 0; k < 6; k++) o.an[k] = panb[6 * ic + k];
-        unsigned short h[6];
-#pragma unroll
-        for (int k = 0; k < 6; k++) h[k] = psl[6 * ic + k];
-#pragma unroll
-        for (int k = 0; k < 3; k++) o.w[k] = h[2 * k] | static_cast<unsigned>(h[2 * k + 1]) << 16;
       };
       // neighbour values through LDS addresses formed from the stored byte
       // offsets (pphi's own LDS address is added once, at the gather)
       auto load_v = [&](Ops &o) {
-        // the empty asm keeps the unpacking here, after the store
-        asm volatile("" : "+v"(o.w[0]), "+v"(o.w[1]), "+v"(o.w[2]));
 #pragma unroll
-        for (int k = 0; k < 6; k++) {
-          const unsigned sl = k & 1 ? o.w[k >> 1] >> 16 : o.w[k >> 1] & 0xffffu;
+        for (int k = 0; k < 6; k++)
 #pragma unroll
           for (int c = 0; c < NC; c++)
             o.v[c][k] = *reinterpret_cast<const __attribute__((address_space(3))) double *>(
-                static_cast<size_t>(sl + 8 * c));
-        }
+                static_cast<size_t>(o.sl[k] + 8 * c));
       };
       auto walk = [&](bool rcp) -> bool {
         int emin = 0, emax = 0;
         double nan_seen = 0.0;
-        // level j of the walk: segment q(j), then the operands of q(j + 1)
-        auto step = [&](int j, Ops &cur, Ops &nxt) {
-          const int qn = backward ? nq - 2 - j : j + 1;
-          load_ops(qn, j + 1 < nq, nxt);
+        // level j of the walk: the j-th segment, then (more) the operands of
+        // the next
+        auto step = [&](int j, Ops &cur, Ops &nxt, bool more) {
+          if (more) load_ops(j + 1, nxt);
           // the operand loads go out before this level's arithmetic
           __builtin_amdgcn_sched_barrier(0);
           double xg[NC], ng[NC];
@@ -1567,7 +1562,7 @@ k_gs_tiles_lds(int batch0, const int4 *__restrict__ bdesc, const int *__restrict
             pphi[cur.on ? NC * cur.i + c : c - 4] = nv;
           }
           wave_sync();  // the walk is one wavefront (kLdsThreads = 64)
-          load_v(nxt);  // past the last segment: row 0's values, unused
+          if (more) load_v(nxt);
           // the guard's bookkeeping waits with the next level's values, off
           // the chain from this level's dividend to its store
           __builtin_amdgcn_sched_barrier(0);
@@ -1583,14 +1578,21 @@ k_gs_tiles_lds(int batch0, const int4 *__restrict__ bdesc, const int *__restrict
           }
         };
         Ops A, B;
-        load_ops(backward ? nq - 1 : 0, true, A);
+        load_ops(0, A);
         load_v(A);
+        // the loop's levels all have a next segment; the last one or two
+        // levels follow it
         int j = 0;
-        for (; j + 1 < nq; j += 2) {
-          step(j, A, B);
-          step(j + 1, B, A);
+        for (; j + 2 < nq; j += 2) {
+          step(j, A, B, true);
+          step(j + 1, B, A, true);
         }
-        if (j < nq) step(j, A, B);
+        if (j + 1 < nq) {
+          step(j, A, B, true);
+          step(j + 1, B, A, false);
+        } else {
+          step(j, A, B, false);
+        }
         return emin < -kRcpExp || emax > kRcpExp || nan_seen != nan_seen;
       };
       if (rsor > 0.0) {

@@ -31,6 +31,7 @@
 #include <cstdlib>
 #include <cstddef>
 #include <cstring>
+#include <type_traits>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -1194,17 +1195,15 @@ k_gs_tiles(SysDesc s, const int *__restrict__ bseg, const int *__restrict__ seg,
 // another tile is never updated in the same launch.  lnbr gives, per CSR
 // entry, the LDS slot of the neighbour: < nc a batch cell, else outside
 // value nc + j; for the pipelined walk the host stores the LDS byte address
-// of that slot's values instead (kPipeBase + 8*NC*slot).  Every load of a
+// of that slot's values instead (8*NC*slot).  Every load of a
 // gather pass is issued before the first LDS
 // store, so the gather costs two memory latencies (the outside indices, then
 // their values) plus the issue of its loads, which NW wavefronts share.  The
 // segment count is at most 63 (host).
 constexpr int kLdsThreads = 64;  // one wavefront: the pipelined walk's width
 constexpr int kXcds = 8;         // XCDs of an MI355X; each has its own L2
-// LDS byte address of the pipelined walk's first value: the kernel has no
-// static LDS, so its dynamic block starts at 0, and lds[0..3] take the
-// stores of lanes without a cell
-constexpr unsigned kPipeBase = 32;
+// The pipelined walk's values start at LDS byte address 0: the kernel has
+// no static LDS, so its dynamic block starts there.
 
 // Orders the LDS accesses of the wavefront that walks an LDS sweep alone: a
 // wavefront's LDS instructions execute in issue order, so what one lane
@@ -1329,13 +1328,13 @@ k_gs_tiles_lds(int batch0, const int4 *__restrict__ bdesc, const int *__restrict
   int *lseg = reinterpret_cast<int *>(lanb + nent);  // nq + 1 bounds relative to c0
   int *lrow = lseg + nq + 1;                          // nc + 1 row starts (general rows)
   unsigned short *lnb = reinterpret_cast<unsigned short *>(lrow + (HEX ? 0 : nc + 1));
-  // LDS layout of the pipelined walk: lds[0..3] takes the stores of lanes
-  // without a cell; values are interleaved by component (slot s, component
-  // c at pphi[NC*s + c]); per cell ap, 1/ap and b (lane-contiguous arrays),
-  // its six anb, and the LDS byte addresses of its six neighbours' values
-  // as 16-bit numbers, which the host computes (pphi is at kPipeBase, and
-  // the host keeps kPipeBase + 8*NC*np within 16 bits for this walk).
-  double *pphi = lds + kPipeBase / 8;
+  // LDS layout of the pipelined walk: values interleaved by component (slot
+  // s, component c at pphi[NC*s + c]); per cell ap, 1/ap and b (lane-
+  // contiguous arrays), its six anb, and the LDS byte addresses of its six
+  // neighbours' values as 16-bit numbers, which the host computes (pphi is
+  // at address 0 and the host keeps 8*NC*np within 16 bits for this walk);
+  // then 32 bytes that take the stores of lanes without a cell.
+  double *pphi = lds;
   double *pap = pphi + NC * np;
   double *pra = pap + nc;
   double *pb = pra + nc;            // NC x nc
@@ -1488,40 +1487,48 @@ k_gs_tiles_lds(int batch0, const int4 *__restrict__ bdesc, const int *__restrict
       // original values, which global memory still holds.
       const int lane = threadIdx.x;
       const double sm1 = sor - 1.0;
-      // lane j: the j-th segment in visit order (segment nq-1-j backward) as
-      // start | end << 16 (bounds below 2^13 for this walk), one read per level
-      const int qv = backward ? max(nq - 1 - lane, 0) : min(lane, nq - 1);
-      const unsigned seg_pk = static_cast<unsigned>(__shfl(sg, qv)) |
-                              static_cast<unsigned>(__shfl(sg, qv + 1)) << 16;
       // The next segment's neighbour addresses are read first and unpacked
       // before this level's arithmetic (they are next in the LDS queue after
       // this level's values); its other operands are used only after this
       // level's store, so the own values stay raw (p).
       struct Ops {
-        int i;
+        int st, en;   // the segment's cells [st, en) (uniform)
+        unsigned c8;  // 8 x the lane's cell (row 0 without a cell)
         bool on;
         double a, ya, bv[NC], p[NC], an[6], v[NC][6];
         unsigned sl[6];
       };
-      // operands and own values of the jj-th segment in visit order
-      auto load_ops = [&](int jj, Ops &o) {
-        const unsigned pk = __builtin_amdgcn_readlane(seg_pk, jj);
-        const int ii = static_cast<int>(pk & 0xffffu) + lane;
-        const bool oo = ii < static_cast<int>(pk >> 16);
-        o.i = ii;
+      // LDS byte addresses of the walk's arrays (the dynamic block starts at
+      // 0), so that an operand's address is one add or 24-bit multiply-add
+      typedef const __attribute__((address_space(3))) double *LdsD;
+      typedef const __attribute__((address_space(3))) unsigned short *LdsH;
+      const unsigned b_ap = 8u * NC * np, b_ra = b_ap + 8u * nc, b_b = b_ra + 8u * nc,
+                     b_an = b_b + 8u * NC * nc, b_sl = b_an + 48u * nc,
+                     b_dum = (b_sl + 12u * nc + 7u) & ~7u;
+      auto lds_f64 = [](unsigned a) { return *reinterpret_cast<LdsD>(static_cast<size_t>(a)); };
+      // operands and own values of the segment of cells [st, en)
+      auto load_ops = [&](int st, int en, Ops &o) {
+        o.st = st;
+        o.en = en;
+        const int ii = st + lane;
+        const bool oo = ii < en;
         o.on = oo;
-        const int ic = oo ? ii : 0;
+        const unsigned ic = oo ? ii : 0;
+        const LdsH hl = reinterpret_cast<LdsH>(static_cast<size_t>(b_sl));
 #pragma unroll
-        for (int k = 0; k < 6; k++) o.sl[k] = psl[6 * ic + k];
-        o.a = pap[ic];
-        o.ya = pra[ic];
+        for (int k = 0; k < 6; k++) o.sl[k] = hl[__umul24(ic, 6u) + k];
+        const unsigned c8 = 8u * ic;
+        o.c8 = c8;
+        o.a = lds_f64(b_ap + c8);
+        o.ya = lds_f64(b_ra + c8);
 #pragma unroll
         for (int c = 0; c < NC; c++) {
-          o.bv[c] = pb[c * nc + ic];
-          o.p[c] = pphi[NC * ic + c];
+          o.bv[c] = lds_f64(b_b + 8u * c * nc + c8);
+          o.p[c] = lds_f64(NC * c8 + 8u * c);
         }
+        const unsigned an = __umul24(ic, 48u) + b_an;
 #pragma unroll
-        for (int k = 0; k < 6; k++) o.an[k] = panb[6 * ic + k];
+        for (int k = 0; k < 6; k++) o.an[k] = lds_f64(an + 8u * k);
       };
       // neighbour values through LDS addresses formed from the stored byte
       // offsets (pphi's own LDS address is added once, at the gather)
@@ -1533,13 +1540,22 @@ k_gs_tiles_lds(int batch0, const int4 *__restrict__ bdesc, const int *__restrict
             o.v[c][k] = *reinterpret_cast<const __attribute__((address_space(3))) double *>(
                 static_cast<size_t>(o.sl[k] + 8 * c));
       };
-      auto walk = [&](bool rcp) -> bool {
+      // The visit direction is a compile-time argument (bwd is a
+      // std::integral_constant): the next segment shares one bound with the
+      // current one, so a level reads one segment bound (lane q of sg).
+      auto walk = [&](bool rcp, auto bwd) __attribute__((always_inline)) -> bool {
+        constexpr bool BWD = decltype(bwd)::value;
         int emin = 0, emax = 0;
         double nan_seen = 0.0;
         // level j of the walk: the j-th segment, then (more) the operands of
         // the next
         auto step = [&](int j, Ops &cur, Ops &nxt, bool more) {
-          if (more) load_ops(j + 1, nxt);
+          if (more) {
+            if (BWD)
+              load_ops(__builtin_amdgcn_readlane(sg, nq - 2 - j), cur.st, nxt);
+            else
+              load_ops(cur.en, __builtin_amdgcn_readlane(sg, j + 2), nxt);
+          }
           // the operand loads go out before this level's arithmetic
           __builtin_amdgcn_sched_barrier(0);
           double xg[NC], ng[NC];
@@ -1558,8 +1574,9 @@ k_gs_tiles_lds(int batch0, const int4 *__restrict__ bdesc, const int *__restrict
             xg[c] = x;
             ng[c] = nv;
             // unconditional store, so no branch separates it from the next
-            // segment's loads
-            pphi[cur.on ? NC * cur.i + c : c - 4] = nv;
+            // segment's loads; lanes without a cell store past the table
+            *reinterpret_cast<__attribute__((address_space(3))) double *>(
+                static_cast<size_t>((cur.on ? NC * cur.c8 : b_dum) + 8u * c)) = nv;
           }
           wave_sync();  // the walk is one wavefront (kLdsThreads = 64)
           if (more) load_v(nxt);
@@ -1578,7 +1595,10 @@ k_gs_tiles_lds(int batch0, const int4 *__restrict__ bdesc, const int *__restrict
           }
         };
         Ops A, B;
-        load_ops(0, A);
+        if (BWD)
+          load_ops(__builtin_amdgcn_readlane(sg, nq - 1), __builtin_amdgcn_readlane(sg, nq), A);
+        else
+          load_ops(__builtin_amdgcn_readlane(sg, 0), __builtin_amdgcn_readlane(sg, 1), A);
         load_v(A);
         // the loop's levels all have a next segment; the last one or two
         // levels follow it
@@ -1595,16 +1615,19 @@ k_gs_tiles_lds(int batch0, const int4 *__restrict__ bdesc, const int *__restrict
         }
         return emin < -kRcpExp || emax > kRcpExp || nan_seen != nan_seen;
       };
+      auto walk_dir = [&](bool rcp) __attribute__((always_inline)) -> bool {
+        return backward ? walk(rcp, std::true_type{}) : walk(rcp, std::false_type{});
+      };
       if (rsor > 0.0) {
-        if (__builtin_amdgcn_ballot_w64(walk(true))) {
+        if (__builtin_amdgcn_ballot_w64(walk_dir(true))) {
           for (int r = threadIdx.x; r < nc; r += kLdsThreads)
 #pragma unroll
             for (int c = 0; c < NC; c++) pphi[NC * r + c] = s.phi[c][c0 + r];
           wave_sync();
-          walk(false);
+          walk_dir(false);
         }
       } else {
-        walk(false);
+        walk_dir(false);
       }
     } else {
       // wavefronts 1.. meanwhile bring the next launch's batches that will
@@ -2821,10 +2844,11 @@ static int build_tile_tables(cfdl_ctx *c, const std::vector<int> &rowp,
     const size_t np = static_cast<size_t>(nc) + ext.size();
     const size_t bytes = 8 * ((2 * nrhs + 2) * static_cast<size_t>(nc) + nrhs * ext.size() + nent) +
                          4 * (nq + 1 + (hex ? 0 : nc + 1)) + 2 * nent;
-    const size_t bytes_pipe = 8 * (4 + nrhs * np + (2 + nrhs + 6) * static_cast<size_t>(nc)) +
-                              2 * 6 * static_cast<size_t>(nc);
+    const size_t bytes_pipe =
+        ((8 * (nrhs * np + (2 + nrhs + 6) * static_cast<size_t>(nc)) + 12 * static_cast<size_t>(nc) +
+          7) & ~static_cast<size_t>(7)) + 32;
     // the pipelined walk addresses values with 16-bit LDS byte addresses
-    if (kPipeBase + 8 * nrhs * np > 0xffff) addr16 = false;
+    if (8 * nrhs * np > 0xffff) addr16 = false;
     lds_max = std::max(lds_max, bytes);
     lds_pipe_max = std::max(lds_pipe_max, bytes_pipe);
     nc_max = std::max(nc_max, nc);
@@ -2841,10 +2865,10 @@ static int build_tile_tables(cfdl_ctx *c, const std::vector<int> &rowp,
     if (nrhs != 1) {
       std::vector<unsigned short> a1(lnbr.size());
       for (size_t k = 0; k < lnbr.size(); k++)
-        a1[k] = static_cast<unsigned short>(kPipeBase + 8u * lnbr[k]);
+        a1[k] = static_cast<unsigned short>(8u * lnbr[k]);
       CALL(upload(c, &t.lnbr1, a1));
     }
-    for (auto &v : lnbr) v = static_cast<unsigned short>(kPipeBase + 8u * nrhs * v);
+    for (auto &v : lnbr) v = static_cast<unsigned short>(8u * nrhs * v);
   }
   const char *tw = std::getenv("CFDL_TILE_WAVES");
   t.nw = (tw && tw[0] == '1') ? 1 : 4;

@@ -1332,14 +1332,17 @@ k_gs_tiles_lds(int batch0, const int4 *__restrict__ bdesc, const int *__restrict
   // s, component c at pphi[NC*s + c]); per cell ap, 1/ap and b (lane-
   // contiguous arrays), its six anb, and the LDS byte addresses of its six
   // neighbours' values as 16-bit numbers, which the host computes (pphi is
-  // at address 0 and the host keeps 8*NC*np within 16 bits for this walk);
-  // then 32 bytes that take the stores of lanes without a cell.
+  // at address 0 and the host keeps 8*NC*np within 16 bits for this walk)
+  // in a 16-byte record per cell (one aligned read; the fourth word is
+  // unused); then 32 bytes that take the stores of lanes without a cell.
+  typedef unsigned U4 __attribute__((ext_vector_type(4)));
   double *pphi = lds;
   double *pap = pphi + NC * np;
   double *pra = pap + nc;
   double *pb = pra + nc;            // NC x nc
   double *panb = pb + NC * nc;      // 6 per cell
-  unsigned short *psl = reinterpret_cast<unsigned short *>(panb + 6 * nc);  // 6 per cell
+  U4 *psl = reinterpret_cast<U4 *>(reinterpret_cast<char *>(lds) +
+                                   ((8 * (NC * np + (8 + NC) * nc) + 15) & ~15));  // 1 per cell
   const double *pa = s.ap + c0, *pr = s.rap + c0, *pn = s.anb + r0;
   const unsigned short *pl = lnbr + r0;
   // segment bound of lane q of wavefront 0 (lanes past nq reload the last
@@ -1365,7 +1368,7 @@ k_gs_tiles_lds(int batch0, const int4 *__restrict__ bdesc, const int *__restrict
     const int cb = r * U * T, xb = r * X * T;
     int xs[X];
     double va[U], vr[U], vn[U][6], vp[NC][U], vb[NC][U], vx[NC][X];
-    unsigned short sl[U][6];
+    unsigned sl[U][3];  // six 16-bit slots or LDS addresses, two per word
     auto load_rows = [&](int u) {
       const int i = cb + u * T + threadIdx.x;
       const int ic = i < nc ? i : 0;  // lanes past the end reload row 0
@@ -1380,7 +1383,8 @@ k_gs_tiles_lds(int batch0, const int4 *__restrict__ bdesc, const int *__restrict
 #pragma unroll
         for (int k = 0; k < 6; k++) {
           vn[u][k] = pn[6 * ic + k];
-          sl[u][k] = pl[6 * ic + k];
+          if (k < 3)  // pl is 4-byte aligned: a hexahedral batch's rows start at 6 * c0
+            sl[u][k] = reinterpret_cast<const unsigned *>(pl)[3 * ic + k];
         }
       }
       // keeps the compiler from moving the loads into the stores' branch,
@@ -1417,7 +1421,7 @@ k_gs_tiles_lds(int batch0, const int4 *__restrict__ bdesc, const int *__restrict
 #pragma unroll
           for (int k = 0; k < 6; k++) {
             panb[6 * i + k] = vn[u][k];
-            psl[6 * i + k] = sl[u][k];  // LDS byte address (host)
+            if (k == 0) psl[i] = U4{sl[u][0], sl[u][1], sl[u][2], 0u};  // LDS addresses (host)
           }
         } else {
           lap[i] = va[u];
@@ -1431,7 +1435,7 @@ k_gs_tiles_lds(int batch0, const int4 *__restrict__ bdesc, const int *__restrict
 #pragma unroll
             for (int k = 0; k < 6; k++) {
               lanb[6 * i + k] = vn[u][k];
-              lnb[6 * i + k] = sl[u][k];
+              lnb[6 * i + k] = static_cast<unsigned short>(sl[u][k >> 1] >> (k & 1 ? 16 : 0));
             }
           }
         }
@@ -1493,7 +1497,7 @@ k_gs_tiles_lds(int batch0, const int4 *__restrict__ bdesc, const int *__restrict
       // level's store, so the own values stay raw (p).
       struct Ops {
         int st, en;   // the segment's cells [st, en) (uniform)
-        unsigned c8;  // 8 x the lane's cell (row 0 without a cell)
+        unsigned c8;  // 8 x the lane's cell (a later one without a cell)
         bool on;
         double a, ya, bv[NC], p[NC], an[6], v[NC][6];
         unsigned sl[6];
@@ -1501,22 +1505,23 @@ k_gs_tiles_lds(int batch0, const int4 *__restrict__ bdesc, const int *__restrict
       // LDS byte addresses of the walk's arrays (the dynamic block starts at
       // 0), so that an operand's address is one add or 24-bit multiply-add
       typedef const __attribute__((address_space(3))) double *LdsD;
-      typedef const __attribute__((address_space(3))) unsigned short *LdsH;
       const unsigned b_ap = 8u * NC * np, b_ra = b_ap + 8u * nc, b_b = b_ra + 8u * nc,
-                     b_an = b_b + 8u * NC * nc, b_sl = b_an + 48u * nc,
-                     b_dum = (b_sl + 12u * nc + 7u) & ~7u;
+                     b_an = b_b + 8u * NC * nc, b_sl = (b_an + 48u * nc + 15u) & ~15u,
+                     b_dum = b_sl + 16u * nc;
       auto lds_f64 = [](unsigned a) { return *reinterpret_cast<LdsD>(static_cast<size_t>(a)); };
       // operands and own values of the segment of cells [st, en)
       auto load_ops = [&](int st, int en, Ops &o) {
         o.st = st;
         o.en = en;
         const int ii = st + lane;
-        const bool oo = ii < en;
-        o.on = oo;
-        const unsigned ic = oo ? ii : 0;
-        const LdsH hl = reinterpret_cast<LdsH>(static_cast<size_t>(b_sl));
+        o.on = ii < en;
+        // lanes past the segment read a later cell of the batch (its last
+        // past the end): real operands, so the division guard stays valid
+        const unsigned ic = min(ii, nc - 1);
+        const U4 w = *reinterpret_cast<const __attribute__((address_space(3))) U4 *>(
+            static_cast<size_t>(b_sl + 16u * ic));
 #pragma unroll
-        for (int k = 0; k < 6; k++) o.sl[k] = hl[__umul24(ic, 6u) + k];
+        for (int k = 0; k < 6; k++) o.sl[k] = (k & 1 ? w[k >> 1] >> 16 : w[k >> 1] & 0xffffu);
         const unsigned c8 = 8u * ic;
         o.c8 = c8;
         o.a = lds_f64(b_ap + c8);
@@ -2845,8 +2850,8 @@ static int build_tile_tables(cfdl_ctx *c, const std::vector<int> &rowp,
     const size_t bytes = 8 * ((2 * nrhs + 2) * static_cast<size_t>(nc) + nrhs * ext.size() + nent) +
                          4 * (nq + 1 + (hex ? 0 : nc + 1)) + 2 * nent;
     const size_t bytes_pipe =
-        ((8 * (nrhs * np + (2 + nrhs + 6) * static_cast<size_t>(nc)) + 12 * static_cast<size_t>(nc) +
-          7) & ~static_cast<size_t>(7)) + 32;
+        ((8 * (nrhs * np + (2 + nrhs + 6) * static_cast<size_t>(nc)) + 15) & ~static_cast<size_t>(15)) +
+        16 * static_cast<size_t>(nc) + 32;
     // the pipelined walk addresses values with 16-bit LDS byte addresses
     if (8 * nrhs * np > 0xffff) addr16 = false;
     lds_max = std::max(lds_max, bytes);

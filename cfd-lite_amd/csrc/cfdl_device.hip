@@ -1518,7 +1518,10 @@ k_gs_tiles_lds(int batch0, const int4 *__restrict__ bdesc, const int *__restrict
         // lanes past the segment read a later cell of the batch (its last
         // past the end): real operands, so the division guard stays valid
         const unsigned ic = min(ii, nc - 1);
-        const U4 w = *reinterpret_cast<const __attribute__((address_space(3))) U4 *>(
+        // 12 of the record's 16 bytes (a fourth, dead register would be
+        // reused at once and make the compiler wait for this read)
+        typedef unsigned U3 __attribute__((ext_vector_type(3)));
+        const U3 w = *reinterpret_cast<const __attribute__((address_space(3))) U3 *>(
             static_cast<size_t>(b_sl + 16u * ic));
 #pragma unroll
         for (int k = 0; k < 6; k++) o.sl[k] = (k & 1 ? w[k >> 1] >> 16 : w[k >> 1] & 0xffffu);

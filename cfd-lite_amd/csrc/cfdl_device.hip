@@ -1329,20 +1329,22 @@ k_gs_tiles_lds(int batch0, const int4 *__restrict__ bdesc, const int *__restrict
   int *lrow = lseg + nq + 1;                          // nc + 1 row starts (general rows)
   unsigned short *lnb = reinterpret_cast<unsigned short *>(lrow + (HEX ? 0 : nc + 1));
   // LDS layout of the pipelined walk: values interleaved by component (slot
-  // s, component c at pphi[NC*s + c]); per cell ap, 1/ap and b (lane-
-  // contiguous arrays), its six anb, and the LDS byte addresses of its six
+  // s, component c at pphi[NC*s + c]); per cell (ap, 1/ap) as one 16-byte
+  // record, b, its six anb, and the LDS byte addresses of its six
   // neighbours' values as 16-bit numbers, which the host computes (pphi is
-  // at address 0 and the host keeps 8*NC*np within 16 bits for this walk)
-  // in a 16-byte record per cell (one aligned read; the fourth word is
-  // unused); then 32 bytes that take the stores of lanes without a cell.
+  // at address 0 and the host keeps 8*NC*np within 16 bits for this walk),
+  // in a 16-byte record (the fourth word unused); then 32 bytes that take
+  // the stores of lanes without a cell.  Records are 16-byte aligned, so
+  // each is one aligned read.
   typedef unsigned U4 __attribute__((ext_vector_type(4)));
+  typedef double D2 __attribute__((ext_vector_type(2)));
   double *pphi = lds;
-  double *pap = pphi + NC * np;
-  double *pra = pap + nc;
-  double *pb = pra + nc;            // NC x nc
-  double *panb = pb + NC * nc;      // 6 per cell
+  const unsigned o_ar = (8u * NC * np + 15u) & ~15u;  // byte offset of (ap, 1/ap)
+  D2 *par = reinterpret_cast<D2 *>(reinterpret_cast<char *>(lds) + o_ar);
+  double *pb = reinterpret_cast<double *>(par + nc);  // NC x nc
+  double *panb = pb + NC * nc;                          // 6 per cell
   U4 *psl = reinterpret_cast<U4 *>(reinterpret_cast<char *>(lds) +
-                                   ((8 * (NC * np + (8 + NC) * nc) + 15) & ~15));  // 1 per cell
+                                   ((o_ar + 8u * (2 + NC + 6) * nc + 15u) & ~15u));
   const double *pa = s.ap + c0, *pr = s.rap + c0, *pn = s.anb + r0;
   const unsigned short *pl = lnbr + r0;
   // segment bound of lane q of wavefront 0 (lanes past nq reload the last
@@ -1411,8 +1413,7 @@ k_gs_tiles_lds(int batch0, const int4 *__restrict__ bdesc, const int *__restrict
       const int i = cb + u * T + threadIdx.x;
       if (i < nc) {
         if constexpr (PIPE) {
-          pap[i] = va[u];
-          pra[i] = vr[u];
+          par[i] = D2{va[u], vr[u]};
 #pragma unroll
           for (int c = 0; c < NC; c++) {
             pphi[NC * i + c] = vp[c][u];
@@ -1505,9 +1506,8 @@ k_gs_tiles_lds(int batch0, const int4 *__restrict__ bdesc, const int *__restrict
       // LDS byte addresses of the walk's arrays (the dynamic block starts at
       // 0), so that an operand's address is one add or 24-bit multiply-add
       typedef const __attribute__((address_space(3))) double *LdsD;
-      const unsigned b_ap = 8u * NC * np, b_ra = b_ap + 8u * nc, b_b = b_ra + 8u * nc,
-                     b_an = b_b + 8u * NC * nc, b_sl = (b_an + 48u * nc + 15u) & ~15u,
-                     b_dum = b_sl + 16u * nc;
+      const unsigned b_ar = o_ar, b_b = b_ar + 16u * nc, b_an = b_b + 8u * NC * nc,
+                     b_sl = (b_an + 48u * nc + 15u) & ~15u, b_dum = b_sl + 16u * nc;
       auto lds_f64 = [](unsigned a) { return *reinterpret_cast<LdsD>(static_cast<size_t>(a)); };
       // operands and own values of the segment of cells [st, en)
       auto load_ops = [&](int st, int en, Ops &o) {
@@ -1523,12 +1523,15 @@ k_gs_tiles_lds(int batch0, const int4 *__restrict__ bdesc, const int *__restrict
         typedef unsigned U3 __attribute__((ext_vector_type(3)));
         const U3 w = *reinterpret_cast<const __attribute__((address_space(3))) U3 *>(
             static_cast<size_t>(b_sl + 16u * ic));
+        __builtin_amdgcn_sched_barrier(0);  // first in the LDS queue after the values
 #pragma unroll
         for (int k = 0; k < 6; k++) o.sl[k] = (k & 1 ? w[k >> 1] >> 16 : w[k >> 1] & 0xffffu);
         const unsigned c8 = 8u * ic;
         o.c8 = c8;
-        o.a = lds_f64(b_ap + c8);
-        o.ya = lds_f64(b_ra + c8);
+        const D2 ar = *reinterpret_cast<const __attribute__((address_space(3))) D2 *>(
+            static_cast<size_t>(b_ar + 2u * c8));
+        o.a = ar.x;
+        o.ya = ar.y;
 #pragma unroll
         for (int c = 0; c < NC; c++) {
           o.bv[c] = lds_f64(b_b + 8u * c * nc + c8);
@@ -2852,8 +2855,9 @@ static int build_tile_tables(cfdl_ctx *c, const std::vector<int> &rowp,
     const size_t np = static_cast<size_t>(nc) + ext.size();
     const size_t bytes = 8 * ((2 * nrhs + 2) * static_cast<size_t>(nc) + nrhs * ext.size() + nent) +
                          4 * (nq + 1 + (hex ? 0 : nc + 1)) + 2 * nent;
+    const size_t r16 = ~static_cast<size_t>(15);
     const size_t bytes_pipe =
-        ((8 * (nrhs * np + (2 + nrhs + 6) * static_cast<size_t>(nc)) + 15) & ~static_cast<size_t>(15)) +
+        ((((8 * nrhs * np + 15) & r16) + 8 * (2 + nrhs + 6) * static_cast<size_t>(nc) + 15) & r16) +
         16 * static_cast<size_t>(nc) + 32;
     // the pipelined walk addresses values with 16-bit LDS byte addresses
     if (8 * nrhs * np > 0xffff) addr16 = false;

@@ -1479,10 +1479,10 @@ k_gs_tiles_lds(int batch0, const int4 *__restrict__ bdesc, const int *__restrict
       // segment j computes from one, the other receives segment j+1's ap, 1/ap,
       // b, anb, neighbour addresses and own old value (its cells are not
       // written at level j); right after segment j's stores it receives its
-      // neighbour values.  The walk is
-      // one wavefront's, whose LDS accesses execute in issue order, so those
-      // reads see the stores without waiting for them.  Lanes without a cell
-      // read row 0 and store to lds[0..3].
+      // neighbour values.  The walk is one wavefront's, whose LDS accesses
+      // execute in issue order, so those reads see the stores without waiting
+      // for them.  Lanes without a cell read a later cell's operands and store
+      // to the 32 bytes after the table.
       // RCP: the divisions by ap and sor use div_rcp2.  Without a branch in the
       // walk, each lane keeps the least and greatest frexp exponent of its
       // dividends x and a NaN detector (0*nv is NaN for an infinite or NaN
@@ -1541,8 +1541,7 @@ k_gs_tiles_lds(int batch0, const int4 *__restrict__ bdesc, const int *__restrict
 #pragma unroll
         for (int k = 0; k < 6; k++) o.an[k] = lds_f64(an + 8u * k);
       };
-      // neighbour values through LDS addresses formed from the stored byte
-      // offsets (pphi's own LDS address is added once, at the gather)
+      // neighbour values at the LDS byte addresses the host stored
       auto load_v = [&](Ops &o) {
 #pragma unroll
         for (int k = 0; k < 6; k++)

@@ -1498,7 +1498,7 @@ k_gs_tiles_lds(int batch0, const int4 *__restrict__ bdesc, const int *__restrict
       // level's store, so the own values stay raw (p).
       struct Ops {
         int st, en;   // the segment's cells [st, en) (uniform)
-        unsigned c8;  // 8 x the lane's cell (a later one without a cell)
+        unsigned pv;  // LDS byte address of the lane's cell's values (a later cell without a cell)
         bool on;
         double a, ya, bv[NC], p[NC], an[6], v[NC][6];
         unsigned sl[6];
@@ -1527,7 +1527,7 @@ k_gs_tiles_lds(int batch0, const int4 *__restrict__ bdesc, const int *__restrict
 #pragma unroll
         for (int k = 0; k < 6; k++) o.sl[k] = (k & 1 ? w[k >> 1] >> 16 : w[k >> 1] & 0xffffu);
         const unsigned c8 = 8u * ic;
-        o.c8 = c8;
+        o.pv = NC == 1 ? c8 : __umul24(ic, 8u * NC);
         const D2 ar = *reinterpret_cast<const __attribute__((address_space(3))) D2 *>(
             static_cast<size_t>(b_ar + 2u * c8));
         o.a = ar.x;
@@ -1535,7 +1535,7 @@ k_gs_tiles_lds(int batch0, const int4 *__restrict__ bdesc, const int *__restrict
 #pragma unroll
         for (int c = 0; c < NC; c++) {
           o.bv[c] = lds_f64(b_b + 8u * c * nc + c8);
-          o.p[c] = lds_f64(NC * c8 + 8u * c);
+          o.p[c] = lds_f64(o.pv + 8u * c);
         }
         const unsigned an = __umul24(ic, 48u) + b_an;
 #pragma unroll
@@ -1569,9 +1569,12 @@ k_gs_tiles_lds(int batch0, const int4 *__restrict__ bdesc, const int *__restrict
           // the operand loads go out before this level's arithmetic
           __builtin_amdgcn_sched_barrier(0);
           double xg[NC], ng[NC];
+          // with several right-hand sides every one is computed, and one whose
+          // solve has stopped (not act) stores past the table, like a lane
+          // without a cell (fewer instructions than a select per use)
 #pragma unroll
           for (int c = 0; c < NC; c++) {
-            if (!act[c]) continue;
+            if (NC == 1 && !act[c]) continue;
             double sum = cur.bv[c];
 #pragma unroll
             for (int k = 0; k < 6; k++) sum = sum + cur.an[k] * cur.v[c][k];
@@ -1586,7 +1589,7 @@ k_gs_tiles_lds(int batch0, const int4 *__restrict__ bdesc, const int *__restrict
             // unconditional store, so no branch separates it from the next
             // segment's loads; lanes without a cell store past the table
             *reinterpret_cast<__attribute__((address_space(3))) double *>(
-                static_cast<size_t>((cur.on ? NC * cur.c8 : b_dum) + 8u * c)) = nv;
+                static_cast<size_t>(((NC == 1 || act[c]) && cur.on ? cur.pv : b_dum) + 8u * c)) = nv;
           }
           wave_sync();  // the walk is one wavefront (kLdsThreads = 64)
           if (more) load_v(nxt);
@@ -1596,7 +1599,7 @@ k_gs_tiles_lds(int batch0, const int4 *__restrict__ bdesc, const int *__restrict
           if (rcp) {
 #pragma unroll
             for (int c = 0; c < NC; c++) {
-              if (!act[c]) continue;
+              if (NC == 1 && !act[c]) continue;
               const int ex = __builtin_amdgcn_frexp_exp(xg[c]);  // 0 for x = 0
               emin = min(emin, ex);
               emax = max(emax, ex);

@@ -146,11 +146,11 @@ def run_to_convergence(eq, ne, tstep_every, max_outer, tol, check_every=10):
 
 # HBM KiB per launch of the pc sweep kernel on the 128^3 cavity, default
 # schedule, from rocprofv3 counter runs (profiles/k_gs_tiles_lds_traffic_
-# cavity128_fp64.md): mean FETCH_SIZE 6243.1 KiB x 1.99975 (the factor a copy
+# cavity128_fp64.md): mean FETCH_SIZE 6246.0 KiB x 1.99975 (the factor a copy
 # of known size calibrates) + mean WRITE_SIZE 463.2 KiB x 1.0 over the 7400
 # launches of one outer iteration.  About half the fetch is the prefetch of
 # the next launch's batches, which that launch then fetches again.
-TRAFFIC_KIB_128 = 12947.7
+TRAFFIC_KIB_128 = 12953.6
 
 # host <-> device traffic of one drop-in solve_uvwp call
 # (cfd-lite_amd/fortran/dropin/mod_uvwp.f90: put_state, get_state, get_work)

@@ -299,7 +299,7 @@ def test_set_get_roundtrip_and_errors():
     ("12", "tiled", "1", "32", "1", "4"), ("4x2x3", "tiled", "1", "5", "1", "4"),
     ("16x8x4", "tiled", "1", "28", "1", "4"), ("8", "tiled", "1", "16", "1", "1"),
     ("8", "tiled", "1", "16", "0", "1"), ("12", "tiled", "1", "32", "1", "1"),
-    ("16x8x4", "tiled", "1", "28", "1", "1")])
+    ("16x8x4", "tiled", "1", "28", "1", "1"), ("12", "tiled", "1", "63", "1", "4")])
 def test_sweep_schedules_bitwise(monkeypatch, tile, mode, lds, span, pipe, waves):
     """The pressure-correction sweeps run as one launch per dependency level
     or as space-time tiles launched batch by batch (in LDS, or from global
@@ -309,8 +309,9 @@ def test_sweep_schedules_bitwise(monkeypatch, tile, mode, lds, span, pipe, waves
     the next launch's batches during the walk), or by one with
     CFDL_TILE_WAVES=1.  Small tiles and spans put many tile boundaries into
     small meshes; tile 12 with span 32 needs more than 64 KB of LDS per batch
-    and has wider segments; tiles may have a different edge per axis.  Every
-    schedule gives the oracle's bits."""
+    and has wider segments, and with span 63 a batch holds a subdomain's
+    whole sweep (the walk's most segments); tiles may have a different edge
+    per axis.  Every schedule gives the oracle's bits."""
     monkeypatch.setenv("CFDL_TILE", tile)
     monkeypatch.setenv("CFDL_GS_MODE", mode)
     monkeypatch.setenv("CFDL_TILE_LDS", lds)
